@@ -1,0 +1,162 @@
+/*
+ * w2k_lapw1.h -- C-ABI of the MI355X lapw1 hot path (HAMILT, HNS, SECLR4).
+ *
+ * The three Fortran routines that CALKPT calls per k-point
+ *   HAMILT  (SRC_lapw1/hamilt.F:1-2, called at calkpt.F:155-156)
+ *   HNS     (SRC_lapw1/hns.F:1-3,    called at calkpt.F:169-172)
+ *   SECLR4  (SRC_lapw1/seclr4.F:3,   called at calkpt.F:196)
+ * are replaced by a context that lives on one GPU.  The Fortran side reaches
+ * these entry points through ISO_C_BINDING (hack-wein2k_amd/fortran/
+ * w2k_dropin.F90 provides HAMILT/HNS/SECLR4 with the reference signatures).
+ *
+ * Storage conventions follow the reference's sequential build exactly
+ * (modules.F:441-481): one N x N column-major array HS with the overlap S in
+ * the upper triangle (diagonal included) and the Hamiltonian H in the strict
+ * lower triangle (complex build: conjugated), HSDIAG holding diag(H) after
+ * HAMILT.  Complex arrays are interleaved (re, im) double pairs, i.e. the
+ * Fortran COMPLEX*16 layout.
+ *
+ * Every entry point returns an int status: 0 on success, > 0 a LAPACK-style
+ * INFO from the eigensolver (the reference prints it and STOPs,
+ * seclr4.F:200-205, 230-235, 304-320), < 0 an argument or device error
+ * (W2K_E* below).  A context is not thread-safe; distinct contexts are
+ * independent (one HIP stream each).
+ */
+#ifndef W2K_LAPW1_H
+#define W2K_LAPW1_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* compile-time dimensions of the reference (param.inc_r:4-24) */
+#define W2K_LMAX   13
+#define W2K_LOMAX  3
+#define W2K_NLOAT  3
+#define W2K_NSLMAX 5
+
+/* error codes (< 0) */
+#define W2K_EINVAL   -1   /* bad argument / inconsistent sizes            */
+#define W2K_ENODEV   -2   /* no HIP device / device selection failed      */
+#define W2K_ENOMEM   -3   /* device or host allocation failed             */
+#define W2K_EDEVICE  -4   /* a HIP call or kernel launch failed           */
+#define W2K_ESTATE   -5   /* call out of order (e.g. HNS before HAMILT)   */
+#define W2K_ENUME    -6   /* more eigenvalues in the window than NUME     */
+
+/*
+ * Per-run (k-independent) data: everything HAMILT and HNS read from their
+ * argument lists and from the modules albl/loabc/lolog/atspdt/loint/rotmat/
+ * struk/out (SURVEY boundary table; hamilt.F:5-27, hns.F:5-17).  All arrays
+ * keep the reference's Fortran layouts and leading dimensions.  The pointers
+ * are read during w2k_create / w2k_set_run only (the data is copied).
+ */
+typedef struct w2k_run_desc {
+  int is_complex;        /* 0: lapw1 (REAL, inversion-symmetric), 1: lapw1c     */
+  int nat;               /* NAT, inequivalent atoms                             */
+  int ndf;               /* NDF, all atoms                                      */
+  int ltop;              /* LTOP (= NT = lmax+1), hamilt.F:1                    */
+  int lmaxd;             /* leading dim of radial arrays: LMAX (13)             */
+  int nnsl;              /* NNSL = LNSMAX+1, hns.F:1                            */
+  int lmmxd;             /* 2nd dim of VNS* arrays (LMMX=120, or fewer)         */
+  int ngaud;             /* leading dim of LQNS/GFAC (NGAU=2350, or fewer)      */
+  int ins;               /* INS: 0 = warped interstitial (hamilt.F:289-347)     */
+  const int *mult;       /* MULT(NAT)                                           */
+  const double *pos;     /* POS(3,NDF), fractional coordinates                  */
+  const double *rmt;     /* R(NAT)                                              */
+  const double *v;       /* V(NAT) = 4 pi R^3 / (3 Omega)                       */
+  double vi;             /* VOLUME = 1/Omega                                    */
+  /* radial parameters (0:LMAX-1, NAT): hamilt.F:52-55 U,UP,DU,DUP,CHN,E    */
+  const double *u, *up, *du, *dup, *chn, *e;
+  const int *lapw;       /* LAPW(0:LMAX-1,NAT) as 0/1 (lolog)                   */
+  /* local orbitals (loabc, lolog; modules.F:720-785, 875-898) */
+  int nlo;               /* NLO                                                 */
+  const int *ilo;        /* ILO(0:LOMAX,NAT)                                    */
+  const int *loor;       /* LOOR(0:LOMAX,NAT) as 0/1                            */
+  const double *alo, *blo, *clo, *elo;   /* (0:LOMAX, NLOAT, NAT)               */
+  const double *pi12lo, *pe12lo, *dplo;  /* (0:LOMAX, NAT)                      */
+  const double *rotloc;  /* ROTLOC(3,3,NAT)                                     */
+  const double *rotij;   /* ROTIJ(3,3,NDF)                                      */
+  /* warped interstitial potential box (module out, warpin.f:31-37) */
+  int warp1, warp2, warp3;
+  const double *warp;    /* COMPLEX WARP(-w1:w1,-w2:w2,-w3:w3), interleaved     */
+  /* non-spherical potential (atspdt, loint; hns.F:25-37) */
+  const int *lqind;      /* LQIND(NAT)                                          */
+  const int *lqns;       /* LQNS(6, ngaud, NAT)                                 */
+  const double *gfac;    /* COMPLEX GFAC(ngaud, NAT), interleaved               */
+  const double *vns1, *vns2, *vns3;        /* (0:NSLMAX-1, lmmxd, 0:NSLMAX-1, NAT) */
+  /* (LOMAX+2, lmmxd, NSLMAX, NAT): the allocation of init_loint(LOMAX+1,..)
+   * (atpar.f:75 with modules.F:1188-1195), indexed VNS1LO(L+1, LMX, L'+1) */
+  const double *vns1lo, *vns2lo, *vns3lo;
+} w2k_run_desc;
+
+typedef struct w2k_ctx w2k_ctx;
+
+/* Create a context on HIP device `device` and upload the per-run data.
+ * Replaces the per-run state HAMILT/HNS read from modules (no reference
+ * counterpart as a call: INILPW/ATPAR fill those modules once per run). */
+int w2k_create(w2k_ctx **ctx, int device, const w2k_run_desc *run);
+/* Replace the per-run data of an existing context. */
+int w2k_set_run(w2k_ctx *ctx, const w2k_run_desc *run);
+void w2k_destroy(w2k_ctx *ctx);
+
+/* Per-k basis (COORS/LOPW output, coors.f:1-236, lopw.f:1-144): NV plane
+ * waves followed by NLO local-orbital vectors; KZZ(3,NV+NLO) integer
+ * reciprocal coordinates, XK/YK/ZK/RK(NV+NLO) cartesian k+G and |k+G|.
+ * Allocates the device HS (order NV+NLO) and HSDIAG.  Host pointers. */
+int w2k_set_basis(w2k_ctx *ctx, int nv, int nlo, const int *kzz,
+                  const double *xk, const double *yk, const double *zk,
+                  const double *rk);
+
+/* HAMILT (hamilt.F:1-845): spherical MT + warped interstitial + LO block into
+ * the device HS/HSDIAG, in the reference layout.  Also keeps the AL/BL
+ * tables (module albl) for HNS.  */
+int w2k_hamilt(w2k_ctx *ctx);
+/* The DSWAP of calkpt.F:160-166 (diag(HS) <-> HSDIAG) on the device copy. */
+int w2k_swap_diag(w2k_ctx *ctx);
+/* HNS (hns.F:1-526): non-spherical V_LM terms into the lower triangle of
+ * the device HS (diagonal included).  Requires w2k_hamilt on this basis. */
+int w2k_hns(w2k_ctx *ctx);
+/* SECLR4 (seclr4.F:3-471): generalized eigensolve of the device HS/HSDIAG
+ * for the eigenvalues in (el, eu] (el >= eu: all, seclr4.F:149-152), at most
+ * `nume` of them (dstebz2.f:647-649).  On return *ne eigenpairs, *nbelw =
+ * number of eigenvalues <= el.  HS is destroyed (as in the reference).
+ * Eigenvalues/vectors stay on the device; fetch with w2k_get_eig. */
+int w2k_seclr4(w2k_ctx *ctx, double el, double eu, int nume, int *ne, int *nbelw);
+/* The whole CALKPT hot path on the device: HAMILT, diag swap, HNS (unless
+ * nohns != 0), SECLR4 (calkpt.F:155-196). */
+int w2k_calkpt(w2k_ctx *ctx, int nohns, double el, double eu, int nume,
+               int *ne, int *nbelw);
+
+/* Host <-> device transfers of the reference-layout arrays (ld >= N).  */
+int w2k_get_hs(w2k_ctx *ctx, double *hs, int ld, double *hsdiag);
+int w2k_put_hs(w2k_ctx *ctx, const double *hs, int ld, const double *hsdiag);
+/* diag(HS) and HSDIAG only: what the host touches between the calls of
+ * CALKPT (the DSWAP), so a Fortran caller can keep HS device-resident. */
+int w2k_get_diag(w2k_ctx *ctx, double *hs_diag, double *hsdiag);
+int w2k_put_diag(w2k_ctx *ctx, const double *hs_diag, const double *hsdiag);
+/* AL/BL tables (N, 0:LTOP-1, NAT), Fortran order, for inspection. */
+int w2k_get_albl(w2k_ctx *ctx, double *al, double *bl);
+/* EIGVAL(1:ne) and Z(1:N,1:ne) (ldz >= N); complex Z interleaved. */
+int w2k_get_eig(w2k_ctx *ctx, double *eigval, double *z, int ldz);
+
+/* Device-resident variant for callers that already hold device memory:
+ * the device addresses of HS (N*N, ld = N), HSDIAG, EIGVAL, Z (ld = N). */
+int w2k_device_buffers(w2k_ctx *ctx, void **hs, void **hsdiag, void **eigval, void **z);
+/* Stream of the context (hipStream_t as void*); all work is queued on it. */
+void *w2k_stream(w2k_ctx *ctx);
+/* Wait for all queued work of the context. */
+int w2k_sync(w2k_ctx *ctx);
+
+/* Kernel timing of the most recent calls, in milliseconds, measured with HIP
+ * events on the context stream: [0] HAMILT, [1] HNS, [2] SECLR4 total,
+ * [3] Cholesky, [4] reduction to standard form, [5] tridiagonalisation,
+ * [6] bisection + inverse iteration, [7] back-transforms. */
+int w2k_last_timings(w2k_ctx *ctx, double *ms, int n);
+
+/* Library version string. */
+const char *w2k_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* W2K_LAPW1_H */

@@ -1,0 +1,839 @@
+/*
+ * ORACLE -- test infrastructure only.  CPU restatement of the lapw1 hot-path
+ * assembly (HAMILT and HNS) used by tests/ and by bench.py's CPU-baseline leg
+ * as the checker.  The product library (hack-wein2k_amd/) never links it.
+ *
+ * Follows, statement by statement where the order of floating-point
+ * operations matters:
+ *   SPHBES  sphbes.f:47-142        DVBES1  dvbes1.f:167-184
+ *   YLM     ylm.f:160-263          ROTATE  rotate.f:123-129
+ *   HAMILT  hamilt.F:142-818       WARPIN  warpin.f:31-37
+ *   USTPHX  ustphx.F:74-95         LMSORT  lmsort.f (sort + dedupe semantics)
+ *   LOHNS   lohns.f                HNS     hns.F:150-488
+ *   DSYR2M/ZHER2M (lower, 'N','T') dsyr2m.f:66-77, zher2m.f:69-79
+ * Compiled with gcc -O2 for baseline x86-64 (no FMA contraction), like the
+ * flang build of the reference it is checked against.
+ *
+ * Pinned by: tests/test_oracle.py (reference golden vectors from oracle/_ref
+ * for every routine; scipy spherical_jn / sph_harm / eval_legendre for the
+ * special functions).
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/w2k_lapw1.h"
+
+typedef double complex cplx;
+
+#define LOMAXP (W2K_LOMAX + 1)
+
+/* ------------------------------------------------------------------ */
+/* special functions                                                    */
+
+/* sphbes.f:47-142: j_0..j_n(x) by downward recursion normalised with the
+ * sum rule; power series for x <= 0.1.  fj[0..n]. */
+void oracle_sphbes(int n, double x, double *fj)
+{
+  const double XLIM = 0.1, HF = 0.5, ONE = 1.0, TWO = 2.0, TNHF = 10.5, FFT = 15.0;
+  const double T25 = 1.0e25, TN25 = 1.0e-25, TN50 = 1.0e-50;
+  /* Fortran FJ(1..) == fj[0..] */
+  double *F = fj - 1;
+  if (n < 0 || x < 0.0) return;
+  if (x > XLIM) {
+    double cufac = 4.2;
+    if (x < (double)(n - 2)) cufac = TNHF / (n + HF - x);
+    int ns = (int)(n + 5 + x * cufac);
+    ns = (int)(ns + FFT / (ONE + sqrt(x)));
+    if (ns + 2 > 100) {
+      if (x > FFT) { for (int i = 0; i <= n; i++) fj[i] = NAN; return; }
+      ns = 98;
+    }
+    double ffo = 0.0, ffn = TN25, ffp;
+    int m = ns - 1;
+    double xi = ONE / x;
+    double fm = (m + m) + ONE;
+    double sdr = fm * TN50;
+    for (;;) {
+      ffp = fm * xi * ffn - ffo;
+      if (fabs(ffp) >= T25) {
+        sdr = sdr * TN50;
+        ffp = ffp * TN25;
+        ffn = ffn * TN25;
+      }
+      sdr = sdr + (fm - TWO) * ffp * ffp;
+      ffo = ffn;
+      ffn = ffp;
+      if (m <= n) break;
+      m = m - 1;
+      fm = fm - TWO;
+    }
+    /* F(M), F(M+1) with M the Fortran index */
+    double G[120]; /* Fortran FJ(1..ns+1) kept 1-based: G[m] == FJ(M) */
+    G[m] = ffn;
+    G[m + 1] = ffo;
+    do {
+      m = m - 1;
+      fm = fm - TWO;
+      for (;;) {
+        G[m] = fm * xi * G[m + 1] - G[m + 2];
+        if (fabs(G[m]) >= T25) {
+          int jj = m + 1, nsx = n + 1;
+          for (int j = jj; j <= nsx; j++) G[j] = G[j] * TN25;
+          sdr = sdr * TN50;
+          continue;
+        }
+        break;
+      }
+      sdr = sdr + (fm - TWO) * G[m] * G[m];
+    } while (m > 1);
+    double ser = ONE / sqrt(sdr);
+    int mm = n + 1;
+    for (m = 1; m <= mm; m++) F[m] = G[m] * ser;
+  } else {
+    double hfxsq = HF * x * x;
+    double xl = ONE, twm = ONE, ta, xlp;
+    int m = 0;
+    do {
+      m = m + 1;
+      ta = xl;
+      twm = twm + TWO;
+      xl = xl / twm;
+      ta = ta - xl * hfxsq;
+      xlp = xl / (twm + TWO);
+      F[m] = ta + HF * xlp * hfxsq * hfxsq;
+      xl = xl * x;
+    } while (m <= n);
+  }
+}
+
+/* dvbes1.f:167-184 (nt values, derivative with respect to x = sm*ri) */
+void oracle_dvbes1(const double *fj, double *dj, double sm, double ri, int nt)
+{
+  const double *FJ = fj - 1;
+  double *DJ = dj - 1;
+  double x = sm * ri;
+  if (x > 1.0e-5) {
+    double q2 = -1.0 / x, q3 = q2;
+    DJ[1] = -FJ[2];
+    int lm = 1;
+    for (int l = 2; l <= nt; l++) {
+      q3 = q3 + q2;
+      DJ[l] = FJ[lm] + q3 * FJ[l];
+      lm = lm + 1;
+    }
+  } else {
+    DJ[1] = 0.0;
+    DJ[2] = 1.0 / 3.0;
+    for (int l = 3; l <= nt; l++) DJ[l] = 0.0;
+  }
+}
+
+/* ylm.f:160-263: Condon-Shortley Y_lm, y[l*l+l+m] for l = 0..lmax */
+void oracle_ylm(const double *v, int lmax, cplx *y)
+{
+  cplx *Y = y - 1;
+  double pi = 4.0 * atan(1.0);
+  double yllr = 1.0 / sqrt(4.0 * pi), ylli = 0.0;
+  Y[1] = CMPLX(yllr, ylli);
+  if (lmax <= 0) return;
+  double a, b, c, ab, abc, abmax, abcmax, costh, sinth, cosph, sinph;
+  abmax = fmax(fabs(v[0]), fabs(v[1]));
+  if (abmax > 0.0) {
+    a = v[0] / abmax;
+    b = v[1] / abmax;
+    ab = sqrt(a * a + b * b);
+    cosph = a / ab;
+    sinph = b / ab;
+  } else {
+    cosph = 1.0;
+    sinph = 0.0;
+  }
+  abcmax = fmax(abmax, fabs(v[2]));
+  if (abcmax > 0.0) {
+    a = v[0] / abcmax;
+    b = v[1] / abcmax;
+    c = v[2] / abcmax;
+    ab = a * a + b * b;
+    abc = sqrt(ab + c * c);
+    costh = c / abc;
+    sinth = sqrt(ab) / abc;
+  } else {
+    costh = 1.0;
+    sinth = 0.0;
+  }
+  Y[3] = CMPLX(sqrt(3.0) * yllr * costh, 0.0);
+  double temp1 = -sqrt(1.5) * yllr * sinth;
+  Y[4] = CMPLX(temp1 * cosph, temp1 * sinph);
+  Y[2] = -conj(Y[4]);
+  for (int l = 2; l <= lmax; l++) {
+    int index = l * l + 1;
+    int index2 = index + 2 * l;
+    int msign = 1 - 2 * (l % 2);
+    double yl1l1r = creal(Y[index - 1]);
+    double yl1l1i = cimag(Y[index - 1]);
+    temp1 = -sqrt((double)(2 * l + 1) / (double)(2 * l)) * sinth;
+    yllr = temp1 * (cosph * yl1l1r - sinph * yl1l1i);
+    ylli = temp1 * (cosph * yl1l1i + sinph * yl1l1r);
+    Y[index2] = CMPLX(yllr, ylli);
+    Y[index] = msign * conj(Y[index2]);
+    index2 = index2 - 1;
+    index = index + 1;
+    double temp2 = sqrt((double)(2 * l + 1)) * costh;
+    double yll1r = temp2 * yl1l1r;
+    double yll1i = temp2 * yl1l1i;
+    Y[index2] = CMPLX(yll1r, yll1i);
+    Y[index] = -msign * conj(Y[index2]);
+    index2 = index2 - 1;
+    index = index + 1;
+    int i4l2 = index2 - 4 * l + 2;
+    int i2l = index2 - 2 * l;
+    double d4ll1c = costh * sqrt((double)(4 * l * l - 1));
+    double d2l13 = -sqrt((double)(2 * l + 1) / (double)(2 * l - 3));
+    for (int m = l - 2; m >= 0; m--) {
+      temp1 = 1.0 / sqrt((double)((l + m) * (l - m)));
+      temp2 = d4ll1c * temp1;
+      double temp3 = d2l13 * sqrt((double)((l + m - 1) * (l - m - 1))) * temp1;
+      double ylmr = temp2 * creal(Y[i2l]) + temp3 * creal(Y[i4l2]);
+      double ylmi = temp2 * cimag(Y[i2l]) + temp3 * cimag(Y[i4l2]);
+      Y[index2] = CMPLX(ylmr, ylmi);
+      Y[index] = msign * conj(Y[index2]);
+      msign = -msign;
+      index2 = index2 - 1;
+      index = index + 1;
+      i4l2 = i4l2 - 1;
+      i2l = i2l - 1;
+    }
+  }
+}
+
+/* rotate.f:123-129: rotvec(j) = sum_k vector(k) * rotmat(j,k) */
+static void rotate3(const double *vec, const double *rot, double *out)
+{
+  for (int jc = 0; jc < 3; jc++) {
+    double d = 0.0;
+    for (int j = 0; j < 3; j++) d = d + vec[j] * rot[jc + 3 * j];
+    out[jc] = d;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* index helpers (Fortran layouts of the run descriptor)                */
+#define RAD(arr, l, a) ((arr)[(l) + R->lmaxd * (a)])
+#define LOA(arr, l, jlo, a) ((arr)[(l) + LOMAXP * ((jlo) + W2K_NLOAT * (a))])
+#define LO2(arr, l, a) ((arr)[(l) + LOMAXP * (a)])
+
+static int lapw_of(const w2k_run_desc *R, int l, int a) { return R->lapw[l + R->lmaxd * a]; }
+static int ilo_of(const w2k_run_desc *R, int l, int a) { return R->ilo[l + LOMAXP * a]; }
+
+static cplx warpin(const w2k_run_desc *R, int x, int y, int z)
+{
+  if (abs(x) <= R->warp1 && abs(y) <= R->warp2 && abs(z) <= R->warp3) {
+    long n1 = 2 * R->warp1 + 1, n2 = 2 * R->warp2 + 1;
+    long idx = (x + R->warp1) + n1 * ((y + R->warp2) + n2 * (long)(z + R->warp3));
+    return CMPLX(R->warp[2 * idx], R->warp[2 * idx + 1]);
+  }
+  return 0.0;
+}
+
+/* AL/BL tables, hamilt.F:158-175 (rows n < nv; LO rows stay zero) */
+void oracle_albl(const w2k_run_desc *R, int nv, int n, const double *rk, double *al, double *bl)
+{
+  int lt = R->ltop;
+  double fj[W2K_LMAX + 2], dfj[W2K_LMAX + 2];
+  memset(al, 0, sizeof(double) * (size_t)n * lt * R->nat);
+  memset(bl, 0, sizeof(double) * (size_t)n * lt * R->nat);
+  for (int i = 0; i < nv; i++) {
+    double rkn = rk[i];
+    for (int a = 0; a < R->nat; a++) {
+      double r = R->rmt[a];
+      oracle_sphbes(lt - 1, r * rkn, fj);
+      oracle_dvbes1(fj, dfj, rkn, r, lt);
+      for (int l = 0; l < lt; l++) {
+        size_t o = i + (size_t)n * (l + (size_t)lt * a);
+        if (lapw_of(R, l, a)) {
+          al[o] = rkn * dfj[l] * RAD(R->up, l, a) - fj[l] * RAD(R->dup, l, a);
+          bl[o] = fj[l] * RAD(R->du, l, a) - rkn * dfj[l] * RAD(R->u, l, a);
+        } else {
+          al[o] = fj[l] / RAD(R->u, l, a) / (r * r);
+          bl[o] = 0.0;
+        }
+      }
+    }
+  }
+}
+
+/* store helpers: HS element (i,j), 0-based, real or complex interleaved */
+static inline void hs_set(double *hs, int ld, int cpx, int i, int j, cplx v)
+{
+  size_t o = (size_t)i + (size_t)ld * j;
+  if (cpx) { hs[2 * o] = creal(v); hs[2 * o + 1] = cimag(v); }
+  else hs[o] = creal(v);
+}
+static inline cplx hs_get(const double *hs, int ld, int cpx, int i, int j)
+{
+  size_t o = (size_t)i + (size_t)ld * j;
+  return cpx ? CMPLX(hs[2 * o], hs[2 * o + 1]) : CMPLX(hs[o], 0.0);
+}
+static inline void vec_set(double *v, int cpx, int i, cplx x)
+{
+  if (cpx) { v[2 * i] = creal(x); v[2 * i + 1] = cimag(x); } else v[i] = creal(x);
+}
+
+/*
+ * HAMILT.  kzz(3,n), xk..rk(n), n = nv + nlo.  hs (ld x n), hsdiag(n):
+ * real or complex (interleaved) per R->is_complex.  al/bl (n, ltop, nat).
+ */
+int oracle_hamilt(const w2k_run_desc *R, int nv, int nlo, const int *kzz, const double *xk,
+                  const double *yk, const double *zk, const double *rk, double *hs, int ld,
+                  double *hsdiag, double *al, double *bl)
+{
+  const int cpx = R->is_complex, nat = R->nat, lt = R->ltop;
+  const int n = nv + nlo;
+  const double ONE = 1.0, TOL = 1.0e-8, UTOP = 0.9999999, TWO = 2.0, THREE = 3.0, FOUR = 4.0, HALF = 0.5;
+  const double PI = FOUR * atan(ONE);
+  if (lt < 3 || lt > W2K_LMAX || n < 1 || ld < n) return W2K_EINVAL;
+
+  oracle_albl(R, nv, n, rk, al, bl);
+#define AL(j, l, a) al[(j) + (size_t)n * ((l) + (size_t)lt * (a))]
+#define BL(j, l, a) bl[(j) + (size_t)n * ((l) + (size_t)lt * (a))]
+
+  double *vifpr4 = malloc(sizeof(double) * nat);
+  for (int a = 0; a < nat; a++) {
+    double r = R->rmt[a];
+    vifpr4[a] = FOUR * PI * R->vi * (r * r * r * r);
+  }
+  double xl[W2K_LMAX];
+  for (int i = 1; i <= W2K_LMAX - 2; i++) xl[i] = (double)i / (double)(i + 1);
+
+  cplx *phase = malloc(sizeof(cplx) * (size_t)n * nat);
+  double *P = malloc(sizeof(double) * (size_t)n * lt);
+  double *x = malloc(sizeof(double) * n), *pplx = malloc(sizeof(double) * n), *pply = malloc(sizeof(double) * n);
+  double *slen = malloc(sizeof(double) * n), *tmp1 = malloc(sizeof(double) * n), *tmp2 = malloc(sizeof(double) * n);
+  cplx *spanel = malloc(sizeof(cplx) * n), *hpanel = malloc(sizeof(cplx) * n);
+
+  for (int i = 0; i < nv; i++) {
+    /* phase factors, hamilt.F:199-243 */
+    int indatm = 0;
+    for (int a = 0; a < nat; a++) {
+      for (int j = 0; j <= i; j++) phase[j + (size_t)n * a] = 0.0;
+      for (int jeq = 0; jeq < R->mult[a]; jeq++) {
+        const double *p = R->pos + 3 * indatm;
+        indatm++;
+        double argxx = p[0] * kzz[3 * i];
+        argxx = argxx + p[1] * kzz[3 * i + 1];
+        argxx = argxx + p[2] * kzz[3 * i + 2];
+        for (int j = 0; j <= i; j++) {
+          double argx = p[0] * kzz[3 * j] - argxx;
+          argx = p[1] * kzz[3 * j + 1] + argx;
+          argx = p[2] * kzz[3 * j + 2] + argx;
+          double h = TWO * PI * argx;
+          if (cpx) phase[j + (size_t)n * a] += CMPLX(cos(h), sin(h));
+          else phase[j + (size_t)n * a] += cos(h);
+        }
+      }
+    }
+    /* Legendre polynomials, hamilt.F:250-281 */
+    for (int j = 0; j <= i; j++) {
+      x[j] = xk[i] * xk[j] + yk[i] * yk[j] + zk[i] * zk[j];
+      double den = rk[i] * rk[j];
+      if (den > TOL) x[j] = x[j] / den;
+      pplx[j] = x[j] * x[j];
+      pply[j] = pplx[j] - ONE;
+      P[j + (size_t)n * 0] = ONE;
+      P[j + (size_t)n * 1] = x[j];
+    }
+    for (int k = 2; k <= lt - 2; k++) {
+      for (int j = 0; j <= i; j++) {
+        if (fabs(x[j]) < UTOP) {
+          P[j + (size_t)n * k] = pplx[j] + xl[k - 1] * pply[j];
+          pplx[j] = x[j] * P[j + (size_t)n * k];
+          pply[j] = pplx[j] - P[j + (size_t)n * (k - 1)];
+        } else {
+          P[j + (size_t)n * k] = x[j] * P[j + (size_t)n * (k - 1)];
+        }
+      }
+    }
+    for (int j = 0; j <= i; j++) {
+      if (fabs(x[j]) < UTOP) P[j + (size_t)n * (lt - 1)] = pplx[j] + xl[lt - 2] * pply[j];
+      else P[j + (size_t)n * (lt - 1)] = x[j] * P[j + (size_t)n * (lt - 2)];
+    }
+    /* step function and interstitial H, hamilt.F:289-366 */
+    if (R->ins == 0) {
+      for (int j = 0; j <= i; j++) {
+        spanel[j] = 0.0;
+        slen[j] = sqrt((xk[j] - xk[i]) * (xk[j] - xk[i]) + (yk[j] - yk[i]) * (yk[j] - yk[i]) +
+                       (zk[j] - zk[i]) * (zk[j] - zk[i]));
+      }
+      for (int a = 0; a < nat; a++) {
+        double v3 = THREE * R->v[a];
+        for (int j = 0; j < i; j++) {
+          double h1 = slen[j] * R->rmt[a];
+          double ty = h1 * h1 * h1;
+          double s = sin(h1), c = cos(h1);
+          ty = 1.0 / ty;
+          double besr = (h1 * c - s) * ty;
+          spanel[j] = spanel[j] + v3 * phase[j + (size_t)n * a] * besr;
+        }
+        spanel[i] = spanel[i] + ONE / (double)nat - R->v[a] * (double)R->mult[a];
+      }
+      for (int j = 0; j <= i; j++) {
+        double x2 = xk[j] * xk[i] + yk[j] * yk[i] + zk[j] * zk[i];
+        cplx w = warpin(R, kzz[3 * j] - kzz[3 * i], kzz[3 * j + 1] - kzz[3 * i + 1],
+                        kzz[3 * j + 2] - kzz[3 * i + 2]);
+        if (!cpx) w = creal(w);
+        hpanel[j] = x2 * spanel[j] + w;
+      }
+    } else {
+      for (int j = 0; j <= i; j++) {
+        double fk0 = xk[j] - xk[i], fk1 = yk[j] - yk[i], fk2 = zk[j] - zk[i];
+        double x2 = xk[j] * xk[i] + yk[j] * yk[i] + zk[j] * zk[i];
+        double s = sqrt(fk0 * fk0 + fk1 * fk1 + fk2 * fk2);
+        cplx u = 0.0;
+        if (s > 1.0e-10) {
+          for (int a = 0; a < nat; a++) {
+            double q = s * R->rmt[a];
+            double besr = (q * cos(q) - sin(q)) / (q * q * q);
+            u = u + R->v[a] * phase[j + (size_t)n * a] * besr;
+          }
+          u = THREE * u;
+        } else {
+          for (int a = 0; a < nat; a++) u = u + R->v[a] * (double)R->mult[a];
+          u = ONE - u;
+        }
+        spanel[j] = u;
+        hpanel[j] = x2 * spanel[j];
+      }
+    }
+    /* muffin-tin radial sums, hamilt.F:371-440 */
+    for (int a = 0; a < nat; a++) {
+      double r = R->rmt[a];
+      for (int j = 0; j <= i; j++) { tmp1[j] = 0.0; tmp2[j] = 0.0; }
+      for (int l = 0; l < lt; l++) {
+        double dll1 = (double)(l + l + 1);
+        double el = RAD(R->e, l, a), ul = RAD(R->u, l, a), upl = RAD(R->up, l, a);
+        double dul = RAD(R->du, l, a), dupl = RAD(R->dup, l, a), chn = RAD(R->chn, l, a);
+        if (!lapw_of(R, l, a)) {
+          double atmp = AL(i, l, a);
+          double atmp1 = atmp * ul * dul * (r * r);
+          for (int j = 0; j <= i; j++) {
+            double tmp3 = dll1 * P[j + (size_t)n * l];
+            double stmp = atmp * AL(j, l, a);
+            stmp = stmp * tmp3;
+            tmp2[j] = tmp2[j] + stmp;
+            double htmp = stmp * el;
+            tmp1[j] = tmp1[j] + htmp;
+            tmp1[j] = tmp1[j] + tmp3 * atmp1 * AL(j, l, a);
+          }
+        } else {
+          double atmp = AL(i, l, a);
+          double btmp = BL(i, l, a) * chn;
+          double atmp1 = (r * r) * (atmp * ul * dul + BL(i, l, a) * upl * dul);
+          double btmp1 = (r * r) * (BL(i, l, a) * upl * dupl + AL(i, l, a) * ul * dupl);
+          for (int j = 0; j <= i; j++) {
+            double tmp3 = dll1 * P[j + (size_t)n * l];
+            double stmp = atmp * AL(j, l, a);
+            stmp = stmp + btmp * BL(j, l, a);
+            stmp = stmp * tmp3;
+            tmp2[j] = tmp2[j] + stmp;
+            double htmp = atmp * BL(j, l, a);
+            htmp = htmp * tmp3;
+            htmp = htmp + stmp * el;
+            tmp1[j] = tmp1[j] + htmp;
+            tmp1[j] = tmp1[j] + tmp3 * (atmp1 * AL(j, l, a) + btmp1 * BL(j, l, a));
+          }
+        }
+      }
+      for (int j = 0; j <= i; j++) {
+        tmp1[j] = tmp1[j] * vifpr4[a];
+        tmp2[j] = tmp2[j] * vifpr4[a];
+        spanel[j] = spanel[j] + phase[j + (size_t)n * a] * tmp2[j];
+        hpanel[j] = hpanel[j] + phase[j + (size_t)n * a] * tmp1[j];
+      }
+    }
+    /* packing, hamilt.F:459-466 */
+    for (int j = 0; j < i; j++) hs_set(hs, ld, cpx, i, j, hpanel[j]);
+    for (int j = 0; j <= i; j++) hs_set(hs, ld, cpx, j, i, spanel[j]);
+    vec_set(hsdiag, cpx, i, hpanel[i]);
+  }
+
+  /* local-orbital columns, hamilt.F:475-818 */
+  if (nlo != 0) {
+    const int nlm = LOMAXP * LOMAXP;
+    int multmax = 1;
+    for (int a = 0; a < nat; a++) if (R->mult[a] > multmax) multmax = R->mult[a];
+    cplx *YL = calloc((size_t)nlm * n * multmax, sizeof(cplx));
+#define YLX(lm, j, jeq) YL[(lm) + (size_t)nlm * ((j) + (size_t)n * (jeq))]
+    cplx *phsc = malloc(sizeof(cplx) * n);
+    int indatm = 0, iatm = 0, nnlo = 0;
+    for (int a = 0; a < nat; a++) {
+      const int mu = R->mult[a];
+      for (int jeq = 0; jeq < mu; jeq++) {
+        iatm++;
+        for (int j = 0; j < nv; j++) {
+          double vec[3] = {xk[j], yk[j], zk[j]}, r1[3], r2[3];
+          rotate3(vec, R->rotij + 9 * (iatm - 1), r1);
+          rotate3(r1, R->rotloc + 9 * a, r2);
+          oracle_ylm(r2, W2K_LOMAX, &YLX(0, j, jeq));
+        }
+      }
+      double r = R->rmt[a];
+      double pi4r2v = FOUR * PI * sqrt(R->vi) * r * r;
+      double clr = ONE, cli = 0.0;
+      for (int l = 0; l <= W2K_LOMAX; l++) {
+        for (int jlo = 1; jlo <= ilo_of(R, l, a); jlo++) {
+          double alo = LOA(R->alo, l, jlo - 1, a), blo = LOA(R->blo, l, jlo - 1, a);
+          double clo = LOA(R->clo, l, jlo - 1, a), elo = LOA(R->elo, l, jlo - 1, a);
+          double pi12 = LO2(R->pi12lo, l, a), pe12 = LO2(R->pe12lo, l, a), dplo = LO2(R->dplo, l, a);
+          double el = RAD(R->e, l, a), chn = RAD(R->chn, l, a);
+          double c1r = alo + pi12 * clo;
+          double c2r = blo * chn + pe12 * clo;
+          double c11r = alo * el + HALF * (blo + clo * pi12 * (elo + el));
+          double c12r = blo * chn * el + HALF * (alo + clo * pi12 + clo * pe12 * (elo + el));
+          double c3r = clo + pe12 * blo + alo * pi12;
+          double c13r = clo * elo + HALF * (blo * pi12 + (elo + el) * (alo * pi12 + pe12 * blo));
+          double akinlor = HALF * (r * r) * (alo * RAD(R->du, l, a) + blo * RAD(R->dup, l, a) + clo * dplo);
+          const double ul = RAD(R->u, l, a), upl = RAD(R->up, l, a);
+          for (int jeqo = 1; jeqo <= mu; jeqo++) {
+            for (int mo1 = -l; mo1 <= l; mo1++) {
+              nnlo++;
+              const int col = nv + nnlo - 1; /* 0-based column, Fortran NV+NNLO */
+              for (int j = 0; j <= col; j++) { spanel[j] = 0.0; hpanel[j] = 0.0; }
+              for (int jeq = 0; jeq < mu; jeq++) {
+                indatm++;
+                const double *p = R->pos + 3 * (indatm - 1);
+                for (int j = 0; j <= col; j++) {
+                  double argx = p[0] * (kzz[3 * col] - kzz[3 * j]);
+                  double argy = p[1] * (kzz[3 * col + 1] - kzz[3 * j + 1]);
+                  double argz = p[2] * (kzz[3 * col + 2] - kzz[3 * j + 2]);
+                  double h = TWO * PI * (argx + argy + argz);
+                  phsc[j] = conj(CMPLX(cos(h), sin(h)));
+                }
+                {
+                  double vec[3] = {xk[col], yk[col], zk[col]}, r1[3], r2[3];
+                  rotate3(vec, R->rotij + 9 * (indatm - 1), r1);
+                  rotate3(r1, R->rotloc + 9 * a, r2);
+                  oracle_ylm(r2, l, &YLX(0, col, jeq));
+                }
+                for (int mo2 = -l; mo2 <= l; mo2++) {
+                  int l0m0 = l * (l + 1) + mo2;
+                  cplx yn = YLX(l0m0, col, jeq);
+                  double clylnr = clr * creal(yn) + cli * cimag(yn);
+                  double clylni = cli * creal(yn) - clr * cimag(yn);
+                  double c1rup = c1r * clylnr * pi4r2v, c1iup = c1r * clylni * pi4r2v;
+                  double c2rup = c2r * clylnr * pi4r2v, c2iup = c2r * clylni * pi4r2v;
+                  double c3rup = c3r * clylnr * pi4r2v, c3iup = c3r * clylni * pi4r2v;
+                  double c11rup = c11r * clylnr * pi4r2v, c11iup = c11r * clylni * pi4r2v;
+                  double c12rup = c12r * clylnr * pi4r2v, c12iup = c12r * clylni * pi4r2v;
+                  double c13rup = c13r * clylnr * pi4r2v, c13iup = c13r * clylni * pi4r2v;
+                  double akinlorup = akinlor * clylnr * pi4r2v, akinloiup = akinlor * clylni * pi4r2v;
+                  for (int j = 0; j < nv; j++) {
+                    cplx yj = YLX(l0m0, j, jeq);
+                    double clylr = clr * creal(yj) + cli * cimag(yj);
+                    double clyli = cli * creal(yj) - clr * cimag(yj);
+                    double c6r = clylr * AL(j, l, a) * pi4r2v, c6i = clyli * AL(j, l, a) * pi4r2v;
+                    double c7r = clylr * BL(j, l, a) * pi4r2v, c7i = clyli * BL(j, l, a) * pi4r2v;
+                    double pr = creal(phsc[j]), pim = cimag(phsc[j]);
+                    double cabr = c6r * c1rup + c7r * c2rup + c6i * c1iup + c7i * c2iup;
+                    double cabi = -c6i * c1rup - c7i * c2rup + c6r * c1iup + c7r * c2iup;
+                    spanel[j] = spanel[j] + (cabr * pr - cabi * pim);
+                    if (cpx) spanel[j] = spanel[j] + I * (cabr * pim + cabi * pr);
+                    cabr = c6r * c11rup + c7r * c12rup + c6i * c11iup + c7i * c12iup;
+                    cabi = -c6i * c11rup - c7i * c12rup + c6r * c11iup + c7r * c12iup;
+                    double akinbr = (c6r * ul + c7r * upl) * akinlorup + (c6i * ul + c7i * upl) * akinloiup;
+                    double akinbi = (c6r * ul + c7r * upl) * akinloiup - (c6i * ul + c7i * upl) * akinlorup;
+                    hpanel[j] = hpanel[j] + (cabr * pr - cabi * pim);
+                    if (cpx) hpanel[j] = hpanel[j] + I * (cabr * pim + cabi * pr);
+                    hpanel[j] = hpanel[j] + (akinbr * pr - akinbi * pim);
+                    if (cpx) hpanel[j] = hpanel[j] + I * (akinbr * pim + akinbi * pr);
+                  }
+                  for (int jlop = 1; jlop <= jlo; jlop++) {
+                    const int NVNNLO = nv + nnlo; /* Fortran index of the current column */
+                    int jst = NVNNLO - (mo1 + l) - (2 * l + 1) * (jeqo - 1) - (jlo - jlop) * (2 * l + 1) * mu;
+                    int jen = NVNNLO - (jlo - jlop) * (mo1 + l + 1) - (jlo - jlop) * (jeqo - 1) * (2 * l + 1);
+                    double alop = LOA(R->alo, l, jlop - 1, a), blop = LOA(R->blo, l, jlop - 1, a);
+                    double clop = LOA(R->clo, l, jlop - 1, a);
+                    for (int J = jst; J <= jen; J++) {
+                      int j = J - 1;
+                      cplx yj = YLX(l0m0, j, jeq);
+                      double clylr = clr * creal(yj) + cli * cimag(yj);
+                      double clyli = cli * creal(yj) - clr * cimag(yj);
+                      double c6r = clylr * alop * pi4r2v, c6i = clyli * alop * pi4r2v;
+                      double c7r = clylr * blop * pi4r2v, c7i = clyli * blop * pi4r2v;
+                      double c8r = clylr * clop * pi4r2v, c8i = clyli * clop * pi4r2v;
+                      double pr = creal(phsc[j]), pim = cimag(phsc[j]);
+                      double cabcr = c6r * c1rup + c6i * c1iup + c7r * c2rup + c7i * c2iup + c8r * c3rup + c8i * c3iup;
+                      double cabci = -c6i * c1rup + c6r * c1iup - c7i * c2rup + c7r * c2iup - c8i * c3rup + c8r * c3iup;
+                      spanel[j] = spanel[j] + (cabcr * pr - cabci * pim);
+                      if (cpx) spanel[j] = spanel[j] + I * (cabcr * pim + cabci * pr);
+                      cabcr = c6r * c11rup + c6i * c11iup + c7r * c12rup + c7i * c12iup + c8r * c13rup + c8i * c13iup;
+                      cabci = -c6i * c11rup + c6r * c11iup - c7i * c12rup + c7r * c12iup - c8i * c13rup + c8r * c13iup;
+                      hpanel[j] = hpanel[j] + (cabcr * pr - cabci * pim);
+                      if (cpx) hpanel[j] = hpanel[j] + I * (cabcr * pim + cabci * pr);
+                    }
+                  }
+                }
+              }
+              /* flush the column (hamilt.F:763-771) */
+              for (int j = 0; j < col; j++) hs_set(hs, ld, cpx, col, j, hpanel[j]);
+              for (int j = 0; j <= col; j++) hs_set(hs, ld, cpx, j, col, spanel[j]);
+              vec_set(hsdiag, cpx, col, hpanel[col]);
+              indatm -= mu;
+            }
+          }
+        }
+      }
+      /* i^l rotation placed after the l loop, as in hamilt.F:787-789 */
+      double t = clr;
+      clr = -cli;
+      cli = t;
+      indatm += mu;
+    }
+    free(YL);
+    free(phsc);
+  }
+  free(vifpr4); free(phase); free(P); free(x); free(pplx); free(pply);
+  free(slen); free(tmp1); free(tmp2); free(spanel); free(hpanel);
+#undef AL
+#undef BL
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* HNS                                                                  */
+
+typedef struct { int l0, m0, lp, mp, lmx; cplx gnt; } gaunt_entry;
+
+static int gcmp(const gaunt_entry *x, const gaunt_entry *y)
+{
+  if (x->l0 != y->l0) return x->l0 < y->l0 ? -1 : 1;
+  if (x->m0 != y->m0) return x->m0 < y->m0 ? -1 : 1;
+  if (x->lp != y->lp) return x->lp < y->lp ? -1 : 1;
+  if (x->mp != y->mp) return x->mp < y->mp ? -1 : 1;
+  if (x->lmx != y->lmx) return x->lmx < y->lmx ? -1 : 1;
+  return 0;
+}
+
+/* LMSORT (lmsort.f): stable ordering by (L0,M0,LP,MP,LMX) with
+ * L0=LQNS(3)-1, M0=LQNS(6), LP=LQNS(1)-1, MP=LQNS(5), LMX=LQNS(4);
+ * repeated (L0,M0,LP,MP,LMX) keep the first GFAC.  Returns count. */
+int oracle_lmsort(const w2k_run_desc *R, int a, gaunt_entry *out)
+{
+  int nq = R->lqind[a], cnt = 0;
+  for (int q = 0; q < nq; q++) {
+    const int *lq = R->lqns + 6 * (q + (size_t)R->ngaud * a);
+    gaunt_entry g;
+    g.l0 = lq[2] - 1; g.m0 = lq[5]; g.lp = lq[0] - 1; g.mp = lq[4]; g.lmx = lq[3];
+    size_t go = 2 * (q + (size_t)R->ngaud * a);
+    g.gnt = CMPLX(R->gfac[go], R->gfac[go + 1]);
+    /* insertion keeping stability (the reference inserts after equal keys) */
+    int pos = cnt;
+    while (pos > 0 && gcmp(&g, &out[pos - 1]) < 0) { out[pos] = out[pos - 1]; pos--; }
+    out[pos] = g;
+    cnt++;
+  }
+  int m = 0;
+  for (int q = 0; q < cnt; q++) {
+    if (m > 0 && gcmp(&out[q], &out[m - 1]) == 0) continue;
+    out[m++] = out[q];
+  }
+  return m;
+}
+
+/* LOHNS (lohns.f): 1-based LO row range i1..i2 of (l0, jlo) on atom a */
+void oracle_lohns(const w2k_run_desc *R, int a, int l0, int jlo, int *i1, int *i2)
+{
+  int s = 1;
+  for (int i = 0; i < a; i++)
+    for (int l = 0; l <= W2K_LOMAX; l++)
+      for (int k = 1; k <= ilo_of(R, l, i); k++) s += (2 * l + 1) * R->mult[i];
+  for (int l = 0; l < l0; l++)
+    for (int k = 1; k <= ilo_of(R, l, a); k++) s += (2 * l + 1) * R->mult[a];
+  int b = s;
+  for (int k = 1; k <= jlo; k++) { b = s; s += (2 * l0 + 1) * R->mult[a]; }
+  *i1 = b;
+  *i2 = s - 1;
+}
+
+#define VNS(arr, x, lmx, y, a) (R->arr[(x) + W2K_NSLMAX * ((lmx) + (size_t)R->lmmxd * ((y) + W2K_NSLMAX * (size_t)(a)))])
+#define VNSLO(arr, x, lmx, y, a) (R->arr[(x) + (W2K_LOMAX + 2) * ((lmx) + (size_t)R->lmmxd * ((y) + W2K_NSLMAX * (size_t)(a)))])
+
+/*
+ * HNS: adds the non-spherical contribution to the lower triangle (diagonal
+ * included) of hs, given the AL/BL tables of HAMILT.
+ */
+int oracle_hns(const w2k_run_desc *R, int nv, int nlo, const int *kzz, const double *xk,
+               const double *yk, const double *zk, const double *al, const double *bl,
+               double *hs, int ld)
+{
+  const int cpx = R->is_complex, nat = R->nat, lt = R->ltop, nnsl = R->nnsl;
+  const int n = nv + nlo;
+  const int nsl2 = W2K_NSLMAX * W2K_NSLMAX;
+  const double PI = 4.0 * atan(1.0);
+  if (n < 1 || ld < n || nnsl > W2K_NSLMAX) return W2K_EINVAL;
+#define AL(j, l, a) al[(j) + (size_t)n * ((l) + (size_t)lt * (a))]
+#define BL(j, l, a) bl[(j) + (size_t)n * ((l) + (size_t)lt * (a))]
+  cplx *YL = malloc(sizeof(cplx) * nsl2 * (size_t)n);
+  cplx *phs = malloc(sizeof(cplx) * n);
+  cplx *ALM = malloc(sizeof(cplx) * nsl2 * (size_t)n), *BLM = malloc(sizeof(cplx) * nsl2 * (size_t)n);
+  cplx *CLM = malloc(sizeof(cplx) * nsl2 * (size_t)n);
+  /* A1/B1/C1 (= ALM/BLM/CLM columns) and A2/B2/C2, as real and imaginary parts */
+  double *A1r = malloc(sizeof(double) * 3 * nsl2 * (size_t)n), *A1i = malloc(sizeof(double) * 3 * nsl2 * (size_t)n);
+  double *A2r = malloc(sizeof(double) * 3 * nsl2 * (size_t)n), *A2i = malloc(sizeof(double) * 3 * nsl2 * (size_t)n);
+  gaunt_entry *gl = malloc(sizeof(gaunt_entry) * (R->ngaud > 0 ? R->ngaud : 1));
+  int indatm = 0;
+  for (int a = 0; a < nat; a++) {
+    int ng = oracle_lmsort(R, a, gl);
+    double r = R->rmt[a];
+    double rhoatm = 4.0 * PI * sqrt(R->vi) * r * r;
+    for (int jeq = 0; jeq < R->mult[a]; jeq++) {
+      indatm++;
+      const double *p = R->pos + 3 * (indatm - 1);
+      for (int i = 0; i < n; i++) {
+        double argx = p[0] * kzz[3 * i], argy = p[1] * kzz[3 * i + 1], argz = p[2] * kzz[3 * i + 2];
+        double h = 2.0 * PI * (argx + argy + argz);
+        phs[i] = CMPLX(cos(h), sin(h));
+        double vec[3] = {xk[i], yk[i], zk[i]}, r1[3], r2[3];
+        rotate3(vec, R->rotij + 9 * (indatm - 1), r1);
+        rotate3(r1, R->rotloc + 9 * a, r2);
+        oracle_ylm(r2, nnsl - 1, YL + (size_t)nsl2 * i);
+      }
+      cplx cil = 1.0;
+      int l0m0 = 0;
+      for (int l0 = 0; l0 < nnsl; l0++) {
+        for (int m0 = -l0; m0 <= l0; m0++) {
+          for (int i = 0; i < n; i++) {
+            size_t o = i + (size_t)n * l0m0;
+            if (i < nv) {
+              cplx t = cil * phs[i] * conj(YL[l0m0 + (size_t)nsl2 * i]);
+              double ar = rhoatm * creal(t), ai = rhoatm * cimag(t);
+              BLM[o] = CMPLX(BL(i, l0, a) * ar, BL(i, l0, a) * ai);
+              ALM[o] = CMPLX(AL(i, l0, a) * ar, AL(i, l0, a) * ai);
+              CLM[o] = 0.0;
+            } else {
+              ALM[o] = BLM[o] = CLM[o] = 0.0;
+            }
+          }
+          if (l0 <= W2K_LOMAX) {
+            for (int jlo = 1; jlo <= ilo_of(R, l0, a); jlo++) {
+              int i1, i2;
+              oracle_lohns(R, a, l0, jlo, &i1, &i2);
+              double alo = LOA(R->alo, l0, jlo - 1, a), blo = LOA(R->blo, l0, jlo - 1, a);
+              double clo = LOA(R->clo, l0, jlo - 1, a);
+              for (int i = nv + i1 - 1; i <= nv + i2 - 1 && i < n; i++) {
+                size_t o = i + (size_t)n * l0m0;
+                cplx t = cil * phs[i] * conj(YL[l0m0 + (size_t)nsl2 * i]);
+                double ar = rhoatm * creal(t), ai = rhoatm * cimag(t);
+                BLM[o] = CMPLX(blo * ar, blo * ai);
+                CLM[o] = CMPLX(clo * ar, clo * ai);
+                ALM[o] = CMPLX(alo * ar, alo * ai);
+              }
+            }
+          }
+          l0m0++;
+        }
+        cil = cil * I;
+      }
+      /* couplings, hns.F:266-440 */
+      int lindex = 0;
+      int q = 0;
+      while (q < ng) {
+        int L0 = gl[q].l0, M0 = gl[q].m0;
+        int lm0 = L0 * (L0 + 1) + M0;
+        size_t ca = (size_t)n * lindex, cb = (size_t)n * (nsl2 + lindex), cc = (size_t)n * (2 * nsl2 + lindex);
+        for (int j = 0; j < n; j++) {
+          size_t o = j + (size_t)n * lm0;
+          A1r[ca + j] = creal(ALM[o]); A1i[ca + j] = cimag(ALM[o]);
+          A1r[cb + j] = creal(BLM[o]); A1i[cb + j] = cimag(BLM[o]);
+          A1r[cc + j] = creal(CLM[o]); A1i[cc + j] = cimag(CLM[o]);
+          A2r[ca + j] = A2i[ca + j] = A2r[cb + j] = A2i[cb + j] = A2r[cc + j] = A2i[cc + j] = 0.0;
+        }
+        while (q < ng && gl[q].l0 == L0 && gl[q].m0 == M0) {
+          int LP = gl[q].lp, MP = gl[q].mp;
+          int lpmp = LP * (LP + 1) + MP;
+          double tuur = 0, tuui = 0, tdur = 0, tdui = 0, tudr = 0, tudi = 0, tddr = 0, tddi = 0;
+          double tuur21 = 0, tuui21 = 0, tudr21 = 0, tudi21 = 0, tuur12 = 0, tuui12 = 0;
+          double tdur12 = 0, tdui12 = 0, tuur22 = 0, tuui22 = 0;
+          while (q < ng && gl[q].l0 == L0 && gl[q].m0 == M0 && gl[q].lp == LP && gl[q].mp == MP) {
+            int lmx = gl[q].lmx - 1;
+            double gr = creal(gl[q].gnt), gi = cimag(gl[q].gnt);
+            tuur = tuur + gr * VNS(vns1, LP, lmx, L0, a);
+            tuui = tuui + gi * VNS(vns1, LP, lmx, L0, a);
+            tdur = tdur + gr * VNS(vns3, L0, lmx, LP, a);
+            tdui = tdui + gi * VNS(vns3, L0, lmx, LP, a);
+            tudr = tudr + gr * VNS(vns3, LP, lmx, L0, a);
+            tudi = tudi + gi * VNS(vns3, LP, lmx, L0, a);
+            tddr = tddr + gr * VNS(vns2, LP, lmx, L0, a);
+            tddi = tddi + gi * VNS(vns2, LP, lmx, L0, a);
+            int loorl0 = 0;
+            if (L0 <= W2K_LOMAX) {
+              loorl0 = R->loor[L0 + LOMAXP * a];
+              if (loorl0) {
+                tuur12 = tuur12 + gr * VNSLO(vns1lo, L0, lmx, LP, a);
+                tuui12 = tuui12 + gi * VNSLO(vns1lo, L0, lmx, LP, a);
+                tdur12 = tdur12 + gr * VNSLO(vns2lo, L0, lmx, LP, a);
+                tdui12 = tdui12 + gi * VNSLO(vns2lo, L0, lmx, LP, a);
+              }
+            }
+            if (LP <= W2K_LOMAX && R->loor[LP + LOMAXP * a]) {
+              if (loorl0) {
+                tuur22 = tuur22 + gr * VNSLO(vns3lo, LP, lmx, L0, a);
+                tuui22 = tuui22 + gi * VNSLO(vns3lo, LP, lmx, L0, a);
+              }
+              tuur21 = tuur21 + gr * VNSLO(vns1lo, LP, lmx, L0, a);
+              tuui21 = tuui21 + gi * VNSLO(vns1lo, LP, lmx, L0, a);
+              tudr21 = tudr21 + gr * VNSLO(vns2lo, LP, lmx, L0, a);
+              tudi21 = tudi21 + gi * VNSLO(vns2lo, LP, lmx, L0, a);
+            }
+            q++;
+          }
+          for (int i = 0; i < n; i++) {
+            size_t o = i + (size_t)n * lpmp;
+            double almr = creal(ALM[o]), almi = cimag(ALM[o]);
+            double blmr = creal(BLM[o]), blmi = cimag(BLM[o]);
+            double clmr = creal(CLM[o]), clmi = cimag(CLM[o]);
+            double t;
+            t = tuur * almr; t = t + tuui * almi; t = t + tdur * blmr; t = t + tdui * blmi;
+            t = t + tuur21 * clmr; t = t + tuui21 * clmi; A2r[ca + i] = A2r[ca + i] + t;
+            t = tuui * almr; t = t - tuur * almi; t = t - tdur * blmi; t = t + tdui * blmr;
+            t = t - tuur21 * clmi; t = t + tuui21 * clmr; A2i[ca + i] = A2i[ca + i] + t;
+            t = tudr * almr; t = t + tudi * almi; t = t + tddr * blmr; t = t + tddi * blmi;
+            t = t + tudr21 * clmr; t = t + tudi21 * clmi; A2r[cb + i] = A2r[cb + i] + t;
+            t = tudi * almr; t = t - tudr * almi; t = t - tddr * blmi; t = t + tddi * blmr;
+            t = t - tudr21 * clmi; t = t + tudi21 * clmr; A2i[cb + i] = A2i[cb + i] + t;
+            t = tuur12 * almr; t = t + tuui12 * almi; t = t + tdur12 * blmr; t = t + tdui12 * blmi;
+            t = t + tuur22 * clmr; t = t + tuui22 * clmi; A2r[cc + i] = A2r[cc + i] + t;
+            t = tuui12 * almr; t = t - tuur12 * almi; t = t - tdur12 * blmi; t = t + tdui12 * blmr;
+            t = t - tuur22 * clmi; t = t + tuui22 * clmr; A2i[cc + i] = A2i[cc + i] + t;
+          }
+        }
+        lindex++;
+      }
+      /* rank-LINDEX lower updates, hns.F:449-487 */
+      for (int jc = 0; jc < n; jc++) {
+        for (int ir = jc; ir < n; ir++) {
+          double sr = 0.0, si = 0.0;
+          int lo_blk = (ir >= nv && jc >= nv);
+          for (int blk = 0; blk < (lo_blk ? 3 : 2); blk++) {
+            size_t base = (size_t)n * nsl2 * blk;
+            for (int k = 0; k < lindex; k++) {
+              size_t oi = base + (size_t)n * k + ir, oj = base + (size_t)n * k + jc;
+              double a2r = A2r[oi], a2i = A2i[oi], a1r = A1r[oj], a1i = A1i[oj];
+              sr += a2r * a1r - a2i * a1i;
+              si += a2r * a1i + a2i * a1r;
+            }
+          }
+          cplx h = hs_get(hs, ld, cpx, ir, jc);
+          h += cpx ? CMPLX(sr, si) : CMPLX(sr, 0.0);
+          hs_set(hs, ld, cpx, ir, jc, h);
+        }
+      }
+    }
+  }
+  free(YL); free(phs); free(ALM); free(BLM); free(CLM);
+  free(A1r); free(A1i); free(A2r); free(A2i); free(gl);
+#undef AL
+#undef BL
+  return 0;
+}

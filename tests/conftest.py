@@ -1,0 +1,81 @@
+"""Shared test setup: the `gpu` marker, the package loader and the fixtures."""
+from __future__ import annotations
+
+import importlib.util
+import os
+import sys
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG_DIR = os.path.join(REPO, "hack-wein2k_amd")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+
+def _load_pkg():
+    if "lapw1_pkg" in sys.modules:
+        return sys.modules["lapw1_pkg"]
+    spec = importlib.util.spec_from_file_location("lapw1_pkg", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["lapw1_pkg"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+PKG = _load_pkg()
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); run with -m gpu")
+
+
+class Fixture:
+    """One golden case: run-level state, per-k data, metadata."""
+
+    def __init__(self, name: str):
+        self.name = name
+        z = np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False)
+        self.raw = {k: z[k] for k in z.files}
+        self.is_complex = bool(int(self.raw["meta/complex"]))
+        self.nk = int(self.raw["meta/nk"])
+        self.run_arrays = {k[4:]: v for k, v in self.raw.items() if k.startswith("run/")}
+
+    def run(self):
+        from lapw1_pkg.lapw1 import RunData
+        return RunData(self.run_arrays, self.is_complex)
+
+    def k(self, i: int) -> dict:
+        p = f"k{i}/"
+        return {k[len(p):]: v for k, v in self.raw.items() if k.startswith(p)}
+
+    def basis(self, i: int):
+        from lapw1_pkg.lapw1 import Basis
+        kd = self.k(i)
+        return Basis(int(kd["NV"]), int(kd["NLO"]), kd["KZZ"], kd["XK"], kd["YK"], kd["ZK"], kd["RK"])
+
+    def has_full(self, i: int) -> bool:
+        return f"k{i}/HS_HAMILT" in self.raw
+
+
+CASES = ["c1_fcc", "mgo", "lo_mult2", "lo_cplx", "rs8"]
+
+
+@pytest.fixture(scope="session", params=CASES)
+def fixture_case(request):
+    return Fixture(request.param)
+
+
+def load_fixture(name: str) -> Fixture:
+    return Fixture(name)
+
+
+def relerr(a, b) -> float:
+    """max |a-b| / max |b| (the per-matrix relative measure used for H and S)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))

@@ -1,0 +1,205 @@
+"""Writes the golden fixtures under tests/golden/ by running the reference lapw1.
+
+Run in the build container only (needs /root/reference and the binaries that
+`make -f oracle/Makefile.ref` builds into oracle/_ref/):
+
+    python tests/golden/make_fixtures.py [case ...]
+
+For every case it writes the seeded WIEN2k input deck (w2k_inputs.py), runs the
+reference lapw1 through the dump driver (oracle/ref_dump.F90), and stores in
+<case>.npz:
+  run-level module state at CALKPT entry (cropped to the entries the hot path
+  reads: LQNS/GFAC to LQIND, VNS* to the LM indices in use, the WARP box to
+  its non-zero extent -- WARPIN returns 0 outside the box, warpin.f:31-37);
+  per k-point: the basis (KZZ, XK..RK), HS/HSDIAG after HAMILT and after HNS,
+  the AL/BL tables, and NE, NBELW, EIGVAL, Z after SECLR4.
+Large cells store per k-point only the basis, eigenvalues, a seeded sample of
+H/S entries, per-column sums and norms (the matrices are hundreds of MB).
+
+The MgO example shipped with the reference (SRC_elast/example-MgO/elast/eos)
+is run as well; its published eigenvalues (eos.output1_1) are stored next to
+the reference-built ones so tests can pin the build against published output.
+"""
+from __future__ import annotations
+
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, HERE)
+import w2k_inputs as W  # noqa: E402
+from w2kdump import read_dump  # noqa: E402
+
+REF_DIR = os.path.join(REPO, "oracle", "_ref")
+MGO_DIR = "/root/reference/SRC_elast/example-MgO/elast/eos"
+
+RUN_KEYS_SCALAR = ["NAT", "NDF", "LTOP", "LNSMAX", "INS", "NLO", "MULTMAX", "LMAXP",
+                   "LOMAXP", "NLOATP", "NSLMAXP", "LMMXP", "NGAUP", "VI"]
+RUN_KEYS_ARR = ["MULT", "POS", "RMT", "V", "P", "PE", "DP", "DPE", "PEI", "E", "LAPW",
+                "LOOR", "ILO", "ALO", "BLO", "CLO", "ELO", "PLO", "DPLO", "PELO", "DPELO",
+                "PEILO", "PI12LO", "PE12LO", "ROTLOC", "ROTIJ", "LQIND"]
+
+
+def run_reference(workdir: str, complex_: bool, threads: int = 8, dump: bool = True) -> float:
+    exe = os.path.join(REF_DIR, "lapw1_complex" if complex_ else "lapw1_real")
+    env = dict(os.environ, OPENBLAS_NUM_THREADS=str(threads))
+    t0 = time.time()
+    args = [exe] + ([] if dump else ["nodump"])
+    r = subprocess.run(args, cwd=workdir, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    dt = time.time() - t0
+    if r.returncode != 0 or not os.path.exists(os.path.join(workdir, "ref_times.txt")):
+        sys.stdout.write(r.stdout.decode(errors="replace")[-3000:])
+        raise RuntimeError(f"reference lapw1 failed in {workdir}")
+    return dt
+
+
+def crop_run(run: dict) -> dict:
+    out = {k: np.asarray(run[k]) for k in RUN_KEYS_SCALAR + RUN_KEYS_ARR}
+    nat = int(run["NAT"])
+    lqind = np.atleast_1d(run["LQIND"]).astype(int)
+    nq = max(int(lqind.max()), 1)
+    out["LQNS"] = run["LQNS"][:, :nq, :]
+    out["GFAC"] = run["GFAC"][:nq, :]
+    lmx_used = int(run["LQNS"][3, :nq, :].max()) if lqind.max() > 0 else 1
+    lmx_used = max(lmx_used, 1)
+    for k in ("VNS1", "VNS2", "VNS3", "VNS1LO", "VNS2LO", "VNS3LO"):
+        out[k] = run[k][:, :lmx_used, :, :]
+    if int(run["INS"]) == 0:
+        w = run["WARP"]
+        k1, k2, k3 = int(run["WARP1"]), int(run["WARP2"]), int(run["WARP3"])
+        nz = np.argwhere(w != 0)
+        if len(nz) == 0:
+            c1 = c2 = c3 = 0
+        else:
+            c = np.abs(nz - np.array([k1, k2, k3])).max(axis=0)
+            c1, c2, c3 = (int(x) for x in c)
+        out["WARP1"], out["WARP2"], out["WARP3"] = np.asarray(c1), np.asarray(c2), np.asarray(c3)
+        out["WARP"] = w[k1 - c1:k1 + c1 + 1, k2 - c2:k2 + c2 + 1, k3 - c3:k3 + c3 + 1]
+    return out
+
+
+def kdata(kd: dict, full: bool, rng: np.random.Generator | None = None) -> dict:
+    n = int(kd["HSROWS"])
+    out = {k: np.asarray(kd[k]) for k in ("NV", "NLO", "HSROWS", "SX", "SY", "SZ", "EL", "EU", "NUME",
+                                          "NE", "NBELW")}
+    out["KZZ"] = kd["KZZ"][:, :n]
+    for k in ("XK", "YK", "ZK", "RK"):
+        out[k] = kd[k][:n]
+    ne = int(kd["NE"])
+    out["EIGVAL"] = np.atleast_1d(kd["EIGVAL"])[:ne]
+    if full:
+        for k in ("HS_HAMILT", "HSDIAG_HAMILT", "HS_HNS", "HSDIAG_HNS", "AL", "BL"):
+            out[k] = kd[k]
+        out["Z"] = kd["Z"][:, :ne]
+    else:
+        for stage in ("HAMILT", "HNS"):
+            hs = kd[f"HS_{stage}"]
+            m = 4096
+            ii = rng.integers(0, n, size=m)
+            jj = rng.integers(0, n, size=m)
+            out[f"SAMPLE_I_{stage}"] = ii.astype(np.int32)
+            out[f"SAMPLE_J_{stage}"] = jj.astype(np.int32)
+            out[f"SAMPLE_V_{stage}"] = hs[ii, jj]
+            out[f"COLSUM_{stage}"] = hs.sum(axis=0)
+            out[f"ROWSUM_{stage}"] = hs.sum(axis=1)
+            out[f"HSDIAG_{stage}"] = kd[f"HSDIAG_{stage}"]
+            out[f"MAXABS_{stage}"] = np.asarray(np.abs(hs).max())
+        # eigenvectors: first/last columns only
+        out["Z_FIRST"] = kd["Z"][:, 0]
+        out["Z_LAST"] = kd["Z"][:, ne - 1]
+    return out
+
+
+def save_case(name: str, workdir: str, nk_full: int, complex_: bool, seconds: float, extra: dict | None = None):
+    run = read_dump(os.path.join(workdir, "run.bin"))
+    payload = {f"run/{k}": v for k, v in crop_run(run).items()}
+    ks = sorted(f for f in os.listdir(workdir) if re.match(r"k\d{4}\.bin$", f))
+    rng = np.random.Generator(np.random.PCG64(W.SEED_BASE + 999))
+    for i, kf in enumerate(ks):
+        kd = read_dump(os.path.join(workdir, kf))
+        for k, v in kdata(kd, i < nk_full, rng).items():
+            payload[f"k{i}/{k}"] = v
+    payload["meta/nk"] = np.asarray(len(ks))
+    payload["meta/complex"] = np.asarray(int(complex_))
+    payload["meta/ref_seconds"] = np.asarray(seconds)
+    times = dict(l.split() for l in open(os.path.join(workdir, "ref_times.txt")))
+    for k, v in times.items():
+        payload[f"meta/{k}"] = np.asarray(float(v))
+    if extra:
+        payload.update(extra)
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **payload)
+    print(f"{name}: {len(ks)} k-points, N={[int(read_dump(os.path.join(workdir, k))['HSROWS']) for k in ks[:1]]}, "
+          f"{os.path.getsize(path)/1024:.0f} KiB, reference {seconds:.2f} s")
+
+
+def parse_published_eigs(path: str, nblocks: int):
+    """Eigenvalue blocks ('EIGENVALUES ARE:') of a published lapw1 output."""
+    txt = open(path).read().split("\n")
+    blocks = []
+    i = 0
+    while i < len(txt) and len(blocks) < nblocks:
+        if "EIGENVALUES ARE:" in txt[i]:
+            vals = []
+            j = i + 1
+            while j < len(txt) and txt[j].strip() and re.fullmatch(r"[\s0-9.+\-Ee]+", txt[j]):
+                vals += [float(x) for x in txt[j].split()]
+                j += 1
+            blocks.append(np.array(vals))
+            i = j
+        else:
+            i += 1
+    return blocks
+
+
+def make_mgo(tmp: str):
+    d = os.path.join(tmp, "mgo")
+    os.makedirs(d)
+    sub_struct = [("ATOMS:", "ATOMS "), ("ATOM=", "ATOM ")]
+    sub_pot = [("ATOM NUMBER =", "ATOM NUMBER  "), ("ATOMNUMBER  =", "ATOMNUMBER   "),
+               ("NUMBER OF LM=", "NUMBER OF LM "), ("LM(R) FOR L=", "LM(R) FOR L ")]
+    # 100atomfix_struct / 100atomfix_clm edits (100atomfix_struct:9-14, 100atomfix_clm:8-15)
+    for ext, subs in (("struct", sub_struct), ("vsp", sub_pot), ("vns", sub_pot)):
+        s = open(os.path.join(MGO_DIR, f"eos.{ext}")).read()
+        for a, b in subs:
+            s = s.replace(a, b)
+        open(os.path.join(d, f"case.{ext}"), "w").write(s)
+    shutil.copy(os.path.join(MGO_DIR, "eos.in1"), os.path.join(d, "case.in1"))
+    # first 12 k-points of the published list (output1_1 holds k 1..55)
+    kl = open(os.path.join(MGO_DIR, "eos.klist")).read().split("\n")
+    open(os.path.join(d, "case.klist"), "w").write("\n".join(kl[:12] + ["END"]) + "\n")
+    open(os.path.join(d, "lapw1.def"), "w").write(W.DEF_TEMPLATE.format(c="case"))
+    sec = run_reference(d, False)
+    pub = parse_published_eigs(os.path.join(MGO_DIR, "eos.output1_1"), 12)
+    extra = {f"published/k{i}": b for i, b in enumerate(pub)}
+    save_case("mgo", d, 1, False, sec, extra)
+
+
+def main(argv):
+    names = argv or ["mgo", "c1_fcc", "lo_mult2", "lo_cplx", "rs8"]
+    tmp = tempfile.mkdtemp(prefix="w2kfix_")
+    try:
+        for name in names:
+            if name == "mgo":
+                make_mgo(tmp)
+                continue
+            case = W.CASES[name]()
+            d = os.path.join(tmp, name)
+            W.write_case(case, d)
+            sec = run_reference(d, case.complex)
+            nk_full = {"c2_rs64": 0, "c1_fcc": 2}.get(name, 1)
+            save_case(name, d, nk_full, case.complex, sec)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

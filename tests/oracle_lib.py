@@ -1,0 +1,115 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) -- test infrastructure.
+
+The oracle is the checker: tests compare the HIP library against it and it is
+itself pinned against the reference golden vectors (tests/golden/*.npz).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Tuple
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(REPO, "oracle", "liboracle.so")
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        from lapw1_pkg import lapw1 as L
+        l = ctypes.CDLL(ORACLE_SO)
+        RD = ctypes.POINTER(L.RunDesc)
+        l.oracle_sphbes.argtypes = [ctypes.c_int, ctypes.c_double, _c_dbl_p]
+        l.oracle_dvbes1.argtypes = [_c_dbl_p, _c_dbl_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        l.oracle_ylm.argtypes = [_c_dbl_p, ctypes.c_int, _c_dbl_p]
+        l.oracle_hamilt.argtypes = [RD, ctypes.c_int, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                                    _c_dbl_p, _c_dbl_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p]
+        l.oracle_hns.argtypes = [RD, ctypes.c_int, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                                 _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.c_int]
+        l.oracle_seclr4.argtypes = [ctypes.c_int, ctypes.c_int, _c_dbl_p, ctypes.c_int, _c_dbl_p, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p,
+                                    ctypes.c_int]
+        _lib = l
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_c_dbl_p)
+
+
+def sphbes(n: int, x: float) -> np.ndarray:
+    out = np.zeros(n + 1)
+    lib().oracle_sphbes(n, x, _dp(out))
+    return out
+
+
+def dvbes1(fj: np.ndarray, sm: float, ri: float) -> np.ndarray:
+    fj = np.ascontiguousarray(fj, np.float64)
+    out = np.zeros(len(fj))
+    lib().oracle_dvbes1(_dp(fj), _dp(out), sm, ri, len(fj))
+    return out
+
+
+def ylm(v, lmax: int) -> np.ndarray:
+    v = np.ascontiguousarray(v, np.float64)
+    out = np.zeros((lmax + 1) ** 2, np.complex128)
+    lib().oracle_ylm(_dp(v), lmax, _dp(out))
+    return out
+
+
+def hamilt(run, basis) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """HAMILT on the CPU: returns HS, HSDIAG, AL, BL (reference layouts)."""
+    n = basis.n
+    cdt = np.complex128 if run.is_complex else np.float64
+    hs = np.zeros((n, n), cdt, order="F")
+    hd = np.zeros(n, cdt)
+    al = np.zeros((n, run.ltop, run.nat), order="F")
+    bl = np.zeros((n, run.ltop, run.nat), order="F")
+    d = run.desc()
+    rc = lib().oracle_hamilt(ctypes.byref(d), basis.nv, basis.nlo, basis.kzz.ctypes.data_as(_c_int_p),
+                             _dp(basis.xk), _dp(basis.yk), _dp(basis.zk), _dp(basis.rk),
+                             _dp(hs), n, _dp(hd), _dp(al), _dp(bl))
+    assert rc == 0, rc
+    return hs, hd, al, bl
+
+
+def hns(run, basis, al, bl, hs) -> np.ndarray:
+    """HNS on the CPU, updating a copy of hs."""
+    n = basis.n
+    cdt = np.complex128 if run.is_complex else np.float64
+    hs = np.array(hs, dtype=cdt, order="F", copy=True)
+    al = np.asfortranarray(al, np.float64)
+    bl = np.asfortranarray(bl, np.float64)
+    d = run.desc()
+    rc = lib().oracle_hns(ctypes.byref(d), basis.nv, basis.nlo, basis.kzz.ctypes.data_as(_c_int_p),
+                          _dp(basis.xk), _dp(basis.yk), _dp(basis.zk), _dp(al), _dp(bl), _dp(hs), n)
+    assert rc == 0, rc
+    return hs
+
+
+def swap_diag(hs, hd):
+    hs = np.array(hs, order="F", copy=True)
+    hd = np.array(hd, copy=True)
+    d = np.diag(hs).copy()
+    np.fill_diagonal(hs, hd)
+    return hs, d
+
+
+def seclr4(hs, hsdiag, el, eu, nume, is_complex):
+    """SECLR4 on the CPU: returns (info, eigval, z, nbelw, hs_out)."""
+    n = hs.shape[0]
+    cdt = np.complex128 if is_complex else np.float64
+    hs = np.array(hs, dtype=cdt, order="F", copy=True)
+    hd = np.array(hsdiag, dtype=cdt, copy=True)
+    ev = np.zeros(n)
+    z = np.zeros((n, n), cdt, order="F")
+    ne, nb = ctypes.c_int(0), ctypes.c_int(0)
+    rc = lib().oracle_seclr4(n, int(is_complex), _dp(hs), n, _dp(hd), float(el), float(eu), int(nume),
+                             ctypes.byref(ne), ctypes.byref(nb), _dp(ev), _dp(z), n)
+    return rc, ev[:ne.value], z[:, :ne.value], nb.value, hs

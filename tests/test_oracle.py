@@ -1,0 +1,184 @@
+"""The CPU oracle against the reference: golden vectors written by the reference
+lapw1 (built from its own sources into oracle/_ref, see tests/golden/
+make_fixtures.py), the MgO example's published eigenvalues, and closed forms
+(scipy special functions, scipy.linalg.eigh)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.special as sps
+
+import oracle_lib as O
+from conftest import Fixture, relerr, CASES
+
+
+# ---------------------------------------------------------------- special functions
+@pytest.mark.parametrize("x", [0.05, 0.1, 0.3, 1.0, 2.5, 7.3, 12.0, 14.9])
+def test_sphbes_matches_scipy(x):
+    n = 12
+    fj = O.sphbes(n, x)
+    ref = sps.spherical_jn(np.arange(n + 1), x)
+    # x <= 0.1 uses the series of sphbes.f:121-141, truncated after x^4: the
+    # first omitted term of j_0 is x^6/5040
+    tol = x ** 6 / 5040 * 1.01 if x <= 0.1 else 1e-13
+    assert np.max(np.abs(fj - ref)) < tol * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("x", [1e-6, 0.2, 3.0, 9.0])
+def test_dvbes1_matches_derivative(x):
+    n = 11
+    fj = O.sphbes(n - 1, x)
+    dj = O.dvbes1(fj, x, 1.0)
+    ref = sps.spherical_jn(np.arange(n), x, derivative=True)
+    if x > 1e-5:
+        assert np.max(np.abs(dj - ref)) < 1e-13
+    else:  # small-argument branch of dvbes1.f:179-183: j_1'(0)=1/3, others 0
+        assert dj[1] == pytest.approx(1 / 3) and dj[0] == 0.0
+
+
+@pytest.mark.parametrize("v", [(0.3, -1.2, 0.7), (0.0, 0.0, 1.0), (1.0, 0.0, 0.0), (-0.2, 0.4, -2.0)])
+def test_ylm_condon_shortley(v):
+    lmax = 6
+    y = O.ylm(v, lmax)
+    r = np.linalg.norm(v)
+    theta = np.arccos(v[2] / r)
+    phi = np.arctan2(v[1], v[0])
+    for l in range(lmax + 1):
+        for m in range(-l, l + 1):
+            ref = sps.sph_harm(m, l, phi, theta)  # scipy includes the Condon-Shortley phase
+            assert abs(y[l * l + l + m] - ref) < 1e-13
+
+
+# ---------------------------------------------------------------- HAMILT / HNS vs reference
+def _full_cases():
+    return CASES
+
+
+@pytest.mark.parametrize("name", _full_cases())
+def test_oracle_albl_matches_reference(name):
+    fx = Fixture(name)
+    run, b, kd = fx.run(), fx.basis(0), fx.k(0)
+    hs, hd, al, bl = O.hamilt(run, b)
+    assert relerr(al, kd["AL"]) < 1e-13
+    assert relerr(bl, kd["BL"]) < 1e-13
+
+
+@pytest.mark.parametrize("name", _full_cases())
+def test_oracle_hamilt_matches_reference(name):
+    fx = Fixture(name)
+    run, b, kd = fx.run(), fx.basis(0), fx.k(0)
+    hs, hd, al, bl = O.hamilt(run, b)
+    S = np.triu(kd["HS_HAMILT"])
+    H = np.tril(kd["HS_HAMILT"], -1)
+    assert relerr(np.triu(hs), S) < 1e-13, "overlap"
+    assert relerr(np.tril(hs, -1), H) < 1e-13, "hamiltonian"
+    assert relerr(hd, kd["HSDIAG_HAMILT"]) < 1e-13, "hsdiag"
+
+
+@pytest.mark.parametrize("name", _full_cases())
+def test_oracle_hns_matches_reference(name):
+    fx = Fixture(name)
+    run, b, kd = fx.run(), fx.basis(0), fx.k(0)
+    hs0, hd0 = O.swap_diag(kd["HS_HAMILT"], kd["HSDIAG_HAMILT"])
+    hs = O.hns(run, b, kd["AL"], kd["BL"], hs0)
+    assert relerr(np.tril(hs), np.tril(kd["HS_HNS"])) < 1e-13
+    assert np.array_equal(np.triu(hs, 1), np.triu(kd["HS_HNS"], 1)), "HNS must not touch the upper triangle"
+
+
+@pytest.mark.parametrize("name", _full_cases())
+def test_oracle_seclr4_matches_reference(name):
+    fx = Fixture(name)
+    kd = fx.k(0)
+    nume = int(kd["NUME"])
+    rc, ev, z, nb, _ = O.seclr4(kd["HS_HNS"], kd["HSDIAG_HNS"], float(kd["EL"]), float(kd["EU"]), nume,
+                                fx.is_complex)
+    assert rc == 0
+    assert len(ev) == int(kd["NE"])
+    assert nb == int(kd["NBELW"])
+    assert np.max(np.abs(ev - kd["EIGVAL"])) < 1e-10
+    # eigenvectors: compare columns of non-degenerate eigenvalues (up to the phase)
+    zr = kd["Z"]
+    gaps = np.minimum(np.r_[np.inf, np.diff(ev)], np.r_[np.diff(ev), np.inf])
+    for j in np.where(gaps > 1e-4)[0]:
+        a, r = z[:, j], zr[:, j]
+        ph = np.vdot(a, r)
+        ph = ph / abs(ph)
+        assert np.max(np.abs(a * ph - r)) < 1e-8 * np.max(np.abs(r)), j
+
+
+@pytest.mark.parametrize("name", ["c1_fcc", "mgo"])
+def test_oracle_full_calkpt_chain_second_kpoint(name):
+    """HAMILT -> swap -> HNS -> SECLR4 from the basis alone, against the
+    reference eigenvalues of a second k-point."""
+    fx = Fixture(name)
+    if fx.nk < 2:
+        pytest.skip("single k-point case")
+    run, b, kd = fx.run(), fx.basis(1), fx.k(1)
+    hs, hd, al, bl = O.hamilt(run, b)
+    hs, hd = O.swap_diag(hs, hd)
+    hs = O.hns(run, b, al, bl, hs)
+    rc, ev, z, nb, _ = O.seclr4(hs, hd, float(kd["EL"]), float(kd["EU"]), int(kd["NUME"]), fx.is_complex)
+    assert rc == 0 and nb == int(kd["NBELW"])
+    assert np.max(np.abs(ev - kd["EIGVAL"])) < 1e-10
+
+
+def test_reference_build_matches_published_mgo():
+    """The flang build of the reference reproduces the MgO eigenvalues that
+    ship with WIEN2k_07 (eos.output1_1, printed to 7 decimals; the published
+    run is from an earlier code version, hence 3e-5 Ry)."""
+    fx = Fixture("mgo")
+    for i in range(fx.nk):
+        pub = fx.raw[f"published/k{i}"]
+        ours = fx.k(i)["EIGVAL"]
+        m = min(len(pub), len(ours))
+        assert m >= 10
+        assert np.max(np.abs(ours[:m] - pub[:m])) < 3e-5
+
+
+# ---------------------------------------------------------------- SECLR4 semantics
+def _random_pencil(n, cpx, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cpx else 0)
+    S = X @ X.conj().T / n + np.eye(n)
+    Y = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cpx else 0)
+    H = (Y + Y.conj().T) / 2
+    # pack as CALKPT leaves it after HNS: S strict upper, H lower (conjugated
+    # for the complex build), diag(H) in HS, diag(S) in HSDIAG
+    hs = np.triu(S, 1) + (np.tril(H, -1).conj() if cpx else np.tril(H, -1)) + np.diag(np.diag(H))
+    return H, S, hs, np.diag(S).copy()
+
+
+@pytest.mark.parametrize("cpx", [False, True])
+@pytest.mark.parametrize("window,nume", [((-0.5, 0.8), 1000), ((-1e9, 1e9), 7), ((1.0, 0.0), 1000)])
+def test_seclr4_window_and_nume(cpx, window, nume):
+    n = 40
+    H, S, hs, hd = _random_pencil(n, cpx, 7 + cpx)
+    # the pencil SECLR4 solves: the conjugated H of the lower triangle is
+    # conjugated back (seclr4.F:159-166) and ZHCGST reduces with U^T
+    # (zhcgst.f), i.e. with the unconjugated S of the upper triangle
+    Hm, Sm = H, S
+    w_all = sla.eigh(Hm, Sm, eigvals_only=True)
+    el, eu = window
+    rc, ev, z, nb, _ = O.seclr4(hs, hd, el, eu, nume, cpx)
+    assert rc == 0
+    if el >= eu:
+        el, eu = -1e70, 1e70
+    sel = w_all[(w_all > el) & (w_all <= eu)][:nume]
+    assert len(ev) == len(sel)
+    assert nb == int(np.sum(w_all <= el))
+    assert np.max(np.abs(ev - sel)) < 1e-11
+    # residual and S-orthonormality of the back-transformed vectors
+    R = Hm @ z - Sm @ z * ev
+    assert np.max(np.abs(R)) < 1e-10
+    G = z.conj().T @ Sm @ z
+    assert np.max(np.abs(G - np.eye(len(ev)))) < 1e-10
+
+
+def test_seclr4_not_positive_definite_reports_info():
+    n = 12
+    H, S, hs, hd = _random_pencil(n, False, 3)
+    hd = hd.copy()
+    hd[5] = -1.0  # S no longer positive definite: POTRF INFO > 0 (seclr4.F:200-205)
+    rc, *_ = O.seclr4(hs, hd, -1e9, 1e9, 100, False)
+    assert rc > 0
