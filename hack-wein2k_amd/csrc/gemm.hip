@@ -1,0 +1,167 @@
+// FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4_f64).
+//
+//   C = alpha * op(A) * op(B) + beta * C        (column-major, Fortran layout)
+//
+// with optional structure: the output restricted to its lower or upper
+// triangle (tiles outside skipped, diagonal tiles masked per element), and
+// op(A) / op(B) triangular (k range clipped per tile, elements outside the
+// triangle read as zero, so the caller's storage there may hold anything).
+//
+// Tile 64x64x16, 256 threads = 4 wave64 in a 2x2 arrangement, each wave a
+// 32x32 block of four 16x16 MFMA accumulators.  Operands staged through LDS
+// k-major so that the MFMA operand reads (lane l -> row l%16, k l/16) are
+// bank-conflict free; LDS double-buffered so the next k-slab's global loads
+// overlap the current slab's MFMAs.
+#include "w2k_internal.h"
+
+namespace w2k {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int PAD = 2;  // LDS row padding (doubles)
+
+struct GemmArgs {
+  int m, n, k;
+  double alpha, beta;
+  const double *A;
+  int lda;
+  const double *B;
+  int ldb;
+  double *C;
+  int ldc;
+  int ta, tb;        // 1 = transposed
+  int tri;           // output: 0 full, 1 lower, 2 upper
+  int atri, btri;    // op(A): 1 upper / 2 lower; op(B): 1 upper / 2 lower
+};
+
+// element (i, kk) of op(A), zero outside the matrix or outside its triangle
+__device__ __forceinline__ double load_opA(const GemmArgs &g, int i, int kk)
+{
+  if (i >= g.m || kk >= g.k) return 0.0;
+  if (g.atri == 1 && kk < i) return 0.0;
+  if (g.atri == 2 && kk > i) return 0.0;
+  return g.ta ? g.A[(size_t)kk + (size_t)g.lda * i] : g.A[(size_t)i + (size_t)g.lda * kk];
+}
+__device__ __forceinline__ double load_opB(const GemmArgs &g, int kk, int j)
+{
+  if (j >= g.n || kk >= g.k) return 0.0;
+  if (g.btri == 1 && kk > j) return 0.0;
+  if (g.btri == 2 && kk < j) return 0.0;
+  return g.tb ? g.B[(size_t)j + (size_t)g.ldb * kk] : g.B[(size_t)kk + (size_t)g.ldb * j];
+}
+
+__global__ __launch_bounds__(256) void k_dgemm(GemmArgs g)
+{
+  __shared__ double As[2][BK][BM + PAD];
+  __shared__ double Bs[2][BK][BN + PAD];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN;
+  // output-triangle tile skip
+  if (g.tri == 1 && i0 + BM - 1 < j0) return;
+  if (g.tri == 2 && j0 + BN - 1 < i0) return;
+  // k range from operand structure
+  int kbeg = 0, kend = g.k;
+  if (g.atri == 1) kbeg = max(kbeg, i0);
+  if (g.atri == 2) kend = min(kend, i0 + BM);
+  if (g.btri == 1) kend = min(kend, j0 + BN);
+  if (g.btri == 2) kbeg = max(kbeg, j0);
+  kbeg = (kbeg / BK) * BK;
+
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+
+  d4 acc[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+
+  // register staging for the global loads of one slab: 4 A + 4 B per thread
+  double ra[4], rb[4];
+  auto gload = [&](int k0) {
+    if (!g.ta) {  // 64 rows contiguous per k column
+      for (int r = 0; r < 4; r++) ra[r] = load_opA(g, i0 + (tid & 63), k0 + (tid >> 6) + 4 * r);
+    } else {      // 16 k contiguous per row
+      for (int r = 0; r < 4; r++) ra[r] = load_opA(g, i0 + (tid >> 4) + 16 * r, k0 + (tid & 15));
+    }
+    if (g.tb) {   // 64 columns contiguous per k row
+      for (int r = 0; r < 4; r++) rb[r] = load_opB(g, k0 + (tid >> 6) + 4 * r, j0 + (tid & 63));
+    } else {      // 16 k contiguous per column
+      for (int r = 0; r < 4; r++) rb[r] = load_opB(g, k0 + (tid & 15), j0 + (tid >> 4) + 16 * r);
+    }
+  };
+  auto sstore = [&](int buf) {
+    if (!g.ta) {
+      for (int r = 0; r < 4; r++) As[buf][(tid >> 6) + 4 * r][tid & 63] = ra[r];
+    } else {
+      for (int r = 0; r < 4; r++) As[buf][tid & 15][(tid >> 4) + 16 * r] = ra[r];
+    }
+    if (g.tb) {
+      for (int r = 0; r < 4; r++) Bs[buf][(tid >> 6) + 4 * r][tid & 63] = rb[r];
+    } else {
+      for (int r = 0; r < 4; r++) Bs[buf][tid & 15][(tid >> 4) + 16 * r] = rb[r];
+    }
+  };
+
+  if (kbeg < kend) {
+    gload(kbeg);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool more = (k0 + BK < kend);
+      if (more) gload(k0 + BK);
+#pragma unroll
+      for (int kq = 0; kq < BK; kq += 4) {
+        double av[2], bv[2];
+        for (int a = 0; a < 2; a++) av[a] = As[buf][kq + lk][wi + 16 * a + li];
+        for (int b = 0; b < 2; b++) bv[b] = Bs[buf][kq + lk][wj + 16 * b + li];
+        for (int a = 0; a < 2; a++)
+          for (int b = 0; b < 2; b++)
+            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+      }
+      if (more) {
+        sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+      }
+    }
+  }
+  // epilogue: lane holds rows lane/16 + 4r (r = 0..3), column lane%16 of each
+  // 16x16 block (v_mfma_f64_16x16x4_f64 result layout, checked on gfx950)
+  for (int a = 0; a < 2; a++) {
+    for (int b = 0; b < 2; b++) {
+      const int j = j0 + wj + 16 * b + li;
+      for (int r = 0; r < 4; r++) {
+        const int i = i0 + wi + 16 * a + lk + 4 * r;
+        if (i >= g.m || j >= g.n) continue;
+        if (g.tri == 1 && i < j) continue;
+        if (g.tri == 2 && i > j) continue;
+        double *cp = g.C + (size_t)i + (size_t)g.ldc * j;
+        double v = g.alpha * acc[a][b][r];
+        if (g.beta != 0.0) v += g.beta * *cp;
+        *cp = v;
+      }
+    }
+  }
+}
+
+int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+          const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri)
+{
+  if (m <= 0 || n <= 0) return 0;
+  GemmArgs g;
+  g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.ta = (ta == 'T' || ta == 't'); g.tb = (tb == 'T' || tb == 't');
+  g.tri = tri; g.atri = atri; g.btri = btri;
+  dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
+  if (k <= 0) {
+    g.k = 0;  // still applies beta
+  }
+  hipLaunchKernelGGL(k_dgemm, grid, dim3(256), 0, s, g);
+  W2K_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace w2k

@@ -1,0 +1,234 @@
+// HNS on the GPU (hns.F:150-488): non-spherical V_LM terms added to the lower
+// triangle of HS (diagonal included).
+//
+// For every atom (all NDF equivalent atoms) the augmentation coefficients
+//   ALM/BLM/CLM(i, L0M0) = 4 pi R^2 sqrt(1/Omega) i^L0 e^{2 pi i G_i.tau}
+//                          conj Y_L0M0(R k_i) x {AL, BL, 0} (PW rows) or
+//                          x {ALO, BLO, CLO} (LO rows of that atom/l/jlo)
+// (hns.F:160-259) are contracted with the per-atom coupling coefficients
+// (Gaunt x radial integrals, hns.F:266-434, precomputed once per run) into
+//   A2 = sum_p TUU conj(ALM_p) + TDU conj(BLM_p) + TUU21 conj(CLM_p), etc.
+// and stored as the two operands of ONE rank-K update over all atoms:
+//   REAL:    HS_low += [A2R,-A2I,B2R,-B2I] . [A1R,A1I,B1R,B1I]^T
+//            (+ [C2R,-C2I] . [C1R,C1I]^T on the LO x LO block)
+//   COMPLEX: HS_low += A2 A1^T + B2 B1^T (+ C2 C1^T), "T" not "H" (hns.F:451),
+//            as two real GEMMs for the real and imaginary parts
+// which replaces the per-atom DSYR2M/ZHER2M calls (dsyr2m.f:66-77) with a
+// single MFMA GEMM of inner dimension 4*sum(LINDEX).
+#include "special.h"
+#include "w2k_internal.h"
+
+namespace w2k {
+
+constexpr double PI_H = 3.141592653589793238462643;
+
+struct HnsArgs {
+  int n, nv, nnsl, lt, ndf;
+  const double *kv, *cs, *al, *bl;
+  const int *atom_of, *lindex, *koff;  // koff[e]: first column of atom e (in units of LINDEX columns)
+  const int *lm0_of;                   // [nat][25]
+  const double *tcoef;                 // [nat][25(k)][25(p)][9][2]
+  const int *lorow;                    // [nat][LOMAXP][NLOAT][2]: first/last 0-based LO row (or -1)
+  const double *loabc;                 // [nat][LOMAXP][NLOAT][3]
+  const double *rotij, *rotloc, *rmt;
+  double vi;
+  int ktot;                            // sum over atoms of LINDEX
+  double *L, *R, *R2;                  // n x (4 ktot), R2 only for the complex build
+  double *LC, *RC, *RC2;               // nlo x (2 ktot)
+  int nlo;
+};
+
+// one thread per (row i, equivalent atom e)
+template <bool CPX>
+__global__ __launch_bounds__(64) void k_hns_prep(HnsArgs g)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = blockIdx.y;
+  if (i >= g.n) return;
+  const int a = g.atom_of[e];
+  const int lidx = g.lindex[a];
+  if (lidx == 0) return;
+  const int n = g.n;
+  const double r = g.rmt[a];
+  const double rhoatm = 4.0 * PI_H * sqrt(g.vi) * r * r;
+  // phase and Y_lm of the rotated k+G, hns.F:171-183
+  const double phr = g.cs[2 * ((size_t)e * n + i)], phi = g.cs[2 * ((size_t)e * n + i) + 1];
+  double v[3] = {g.kv[4 * i], g.kv[4 * i + 1], g.kv[4 * i + 2]}, r1[3], r2[3];
+  rotate3(v, g.rotij + 9 * e, r1);
+  rotate3(r1, g.rotloc + 9 * a, r2);
+  cd yl[W2K_NSL2];
+  ylm(r2, g.nnsl - 1, yl);
+  // ALM/BLM/CLM for this row, hns.F:190-255
+  cd alm[W2K_NSL2], blm[W2K_NSL2], clm[W2K_NSL2];
+  int lm = 0;
+  double cilr = 1.0, cili = 0.0;
+  for (int l0 = 0; l0 < g.nnsl; l0++) {
+    double fa = 0.0, fb = 0.0, fc = 0.0;
+    bool on = false;
+    if (i < g.nv) {
+      fa = g.al[i + (size_t)n * (l0 + (size_t)g.lt * a)];
+      fb = g.bl[i + (size_t)n * (l0 + (size_t)g.lt * a)];
+      on = true;
+    } else if (l0 <= W2K_LOMAX) {
+      for (int jlo = 0; jlo < W2K_NLOAT; jlo++) {
+        const int *rr = g.lorow + 2 * (l0 + W2K_LOMAXP * (jlo + W2K_NLOAT * a));
+        const int qq = i - g.nv;
+        if (rr[0] >= 0 && qq >= rr[0] && qq <= rr[1]) {
+          const double *ab = g.loabc + 3 * (l0 + W2K_LOMAXP * (jlo + W2K_NLOAT * a));
+          fa = ab[0]; fb = ab[1]; fc = ab[2];
+          on = true;
+        }
+      }
+    }
+    for (int m0 = -l0; m0 <= l0; m0++, lm++) {
+      if (!on) { alm[lm] = {0, 0}; blm[lm] = {0, 0}; clm[lm] = {0, 0}; continue; }
+      // (CIL*PHS) * conj(YL)
+      const cd cp = cmul({cilr, cili}, {phr, phi});
+      const cd t = cmul(cp, conj(yl[lm]));
+      const double ar = rhoatm * t.re, ai = rhoatm * t.im;
+      alm[lm] = {fa * ar, fa * ai};
+      blm[lm] = {fb * ar, fb * ai};
+      clm[lm] = {fc * ar, fc * ai};
+    }
+    const double t = cilr;  // CIL = CIL * i
+    cilr = -cili;
+    cili = t;
+  }
+  const bool lorow = (i >= g.nv);
+  const int q = i - g.nv;
+  const int kb = g.koff[e];
+  const double *T = g.tcoef + (size_t)a * W2K_NSL2 * W2K_NSL2 * 18;
+  const size_t ldL = (size_t)n, ldC = (size_t)g.nlo;
+  const int KT = g.ktot;
+  for (int k = 0; k < lidx; k++) {
+    const int lk = g.lm0_of[a * W2K_NSL2 + k];
+    double a2r = 0, a2i = 0, b2r = 0, b2i = 0, c2r = 0, c2i = 0;
+    const int np = g.nnsl * g.nnsl;
+    for (int p = 0; p < np; p++) {
+      const double *t = T + ((size_t)k * W2K_NSL2 + p) * 18;
+      const cd A = conj(alm[p]), B = conj(blm[p]), Cc = conj(clm[p]);
+      // A2 += TUU A + TDU B + TUU21 C ; B2 += TUD A + TDD B + TUD21 C ; C2 += TUU12 A + TDU12 B + TUU22 C
+      a2r += t[0] * A.re - t[1] * A.im + t[2] * B.re - t[3] * B.im + t[4] * Cc.re - t[5] * Cc.im;
+      a2i += t[0] * A.im + t[1] * A.re + t[2] * B.im + t[3] * B.re + t[4] * Cc.im + t[5] * Cc.re;
+      b2r += t[6] * A.re - t[7] * A.im + t[8] * B.re - t[9] * B.im + t[10] * Cc.re - t[11] * Cc.im;
+      b2i += t[6] * A.im + t[7] * A.re + t[8] * B.im + t[9] * B.re + t[10] * Cc.im + t[11] * Cc.re;
+      c2r += t[12] * A.re - t[13] * A.im + t[14] * B.re - t[15] * B.im + t[16] * Cc.re - t[17] * Cc.im;
+      c2i += t[12] * A.im + t[13] * A.re + t[14] * B.im + t[15] * B.re + t[16] * Cc.im + t[17] * Cc.re;
+    }
+    const cd a1 = alm[lk], b1 = blm[lk], c1 = clm[lk];
+    // columns: block 0 A, block 1 A (imag), block 2 B, block 3 B (imag)
+    const size_t c0 = (size_t)(kb + k);
+    g.L[i + ldL * (c0 + 0 * (size_t)KT)] = a2r;
+    g.L[i + ldL * (c0 + 1 * (size_t)KT)] = -a2i;
+    g.L[i + ldL * (c0 + 2 * (size_t)KT)] = b2r;
+    g.L[i + ldL * (c0 + 3 * (size_t)KT)] = -b2i;
+    g.R[i + ldL * (c0 + 0 * (size_t)KT)] = a1.re;
+    g.R[i + ldL * (c0 + 1 * (size_t)KT)] = a1.im;
+    g.R[i + ldL * (c0 + 2 * (size_t)KT)] = b1.re;
+    g.R[i + ldL * (c0 + 3 * (size_t)KT)] = b1.im;
+    if (CPX) {
+      g.R2[i + ldL * (c0 + 0 * (size_t)KT)] = a1.im;
+      g.R2[i + ldL * (c0 + 1 * (size_t)KT)] = -a1.re;
+      g.R2[i + ldL * (c0 + 2 * (size_t)KT)] = b1.im;
+      g.R2[i + ldL * (c0 + 3 * (size_t)KT)] = -b1.re;
+    }
+    if (lorow) {
+      g.LC[q + ldC * (c0 + 0 * (size_t)KT)] = c2r;
+      g.LC[q + ldC * (c0 + 1 * (size_t)KT)] = -c2i;
+      g.RC[q + ldC * (c0 + 0 * (size_t)KT)] = c1.re;
+      g.RC[q + ldC * (c0 + 1 * (size_t)KT)] = c1.im;
+      if (CPX) {
+        g.RC2[q + ldC * (c0 + 0 * (size_t)KT)] = c1.im;
+        g.RC2[q + ldC * (c0 + 1 * (size_t)KT)] = -c1.re;
+      }
+    }
+  }
+}
+
+// HS (complex interleaved, lower incl. diagonal) += Re + i Im from planar
+// temporaries (only the lower triangle of the temporaries is valid)
+__global__ void k_add_cplx_lower(int n, int off, int ld, const double *__restrict__ re, const double *__restrict__ im,
+                                 int ldt, double *__restrict__ hs)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n || i < j) return;
+  const size_t o = (size_t)(i + off) + (size_t)ld * (j + off);
+  hs[2 * o] += re[i + (size_t)ldt * j];
+  hs[2 * o + 1] += im[i + (size_t)ldt * j];
+}
+
+int hns(Ctx &c)
+{
+  RunDev &R = c.run;
+  if (!c.have_hamilt) return W2K_ESTATE;
+  const int n = c.n, nlo = c.nlo, ndf = R.ndf;
+  const int cpx = R.is_complex;
+  hipStream_t s = c.stream;
+  int ktot = 0;
+  std::vector<int> koff(ndf);
+  for (int e = 0; e < ndf; e++) {
+    koff[e] = ktot;
+    ktot += R.lindex[R.atom_of[e]];
+  }
+  if (ktot == 0) return 0;
+  int rc;
+  const size_t kcols = (size_t)4 * ktot;
+  if ((rc = c.d_hnsL.alloc((size_t)n * kcols))) return rc;
+  if ((rc = c.d_hnsR.alloc((size_t)n * kcols * (cpx ? 2 : 1)))) return rc;
+  const size_t lcsz = (size_t)(nlo > 0 ? nlo : 1) * 2 * ktot;
+  if ((rc = c.w_small.alloc(lcsz * (cpx ? 3 : 2) + (size_t)ndf))) return rc;
+  int *d_koff;
+  if ((rc = c.w_int.alloc(ndf))) return rc;
+  d_koff = c.w_int.p;
+  W2K_HIP(hipMemcpyAsync(d_koff, koff.data(), sizeof(int) * ndf, hipMemcpyHostToDevice, s));
+  HnsArgs g;
+  g.n = n; g.nv = c.nv; g.nnsl = R.nnsl; g.lt = R.ltop; g.ndf = ndf;
+  g.kv = c.d_kvec.p; g.cs = c.d_cs.p; g.al = c.d_al.p; g.bl = c.d_bl.p;
+  g.atom_of = R.d_atom_of.p; g.lindex = R.d_lindex.p; g.koff = d_koff; g.lm0_of = R.d_lm0_of.p;
+  g.tcoef = R.d_tcoef.p; g.lorow = R.d_lorow.p; g.loabc = R.d_loabc.p;
+  g.rotij = R.d_rotij.p; g.rotloc = R.d_rotloc.p; g.rmt = R.d_rmt.p; g.vi = R.vi;
+  g.ktot = ktot;
+  g.L = c.d_hnsL.p;
+  g.R = c.d_hnsR.p;
+  g.R2 = cpx ? c.d_hnsR.p + (size_t)n * kcols : nullptr;
+  g.LC = c.w_small.p;
+  g.RC = c.w_small.p + lcsz;
+  g.RC2 = cpx ? c.w_small.p + 2 * lcsz : nullptr;
+  g.nlo = nlo;
+  // atoms without non-spherical terms leave their columns untouched: clear
+  W2K_HIP(hipMemsetAsync(c.d_hnsL.p, 0, sizeof(double) * (size_t)n * kcols, s));
+  W2K_HIP(hipMemsetAsync(c.d_hnsR.p, 0, sizeof(double) * (size_t)n * kcols * (cpx ? 2 : 1), s));
+  W2K_HIP(hipMemsetAsync(c.w_small.p, 0, sizeof(double) * lcsz * (cpx ? 3 : 2), s));
+  dim3 grid((n + 63) / 64, ndf);
+  if (cpx) hipLaunchKernelGGL(k_hns_prep<true>, grid, dim3(64), 0, s, g);
+  else hipLaunchKernelGGL(k_hns_prep<false>, grid, dim3(64), 0, s, g);
+  W2K_HIP(hipGetLastError());
+  const int K4 = (int)kcols, K2 = 2 * ktot;
+  if (!cpx) {
+    if ((rc = dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, g.R, n, 1.0, c.d_hs.p, n, 1))) return rc;
+    if (nlo > 0) {
+      double *hsl = c.d_hs.p + (size_t)c.nv + (size_t)n * c.nv;
+      if ((rc = dgemm(s, 'N', 'T', nlo, nlo, K2, 1.0, g.LC, nlo, g.RC, nlo, 1.0, hsl, n, 1))) return rc;
+    }
+  } else {
+    // planar temporaries for the real and imaginary parts
+    if ((rc = c.w_a.alloc((size_t)n * n))) return rc;
+    if ((rc = c.w_b.alloc((size_t)n * n))) return rc;
+    if ((rc = dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, g.R, n, 0.0, c.w_a.p, n, 1))) return rc;
+    if ((rc = dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, g.R2, n, 0.0, c.w_b.p, n, 1))) return rc;
+    hipLaunchKernelGGL(k_add_cplx_lower, dim3((n + 255) / 256, n), dim3(256), 0, s, n, 0, n, c.w_a.p, c.w_b.p, n,
+                       c.d_hs.p);
+    W2K_HIP(hipGetLastError());
+    if (nlo > 0) {
+      if ((rc = dgemm(s, 'N', 'T', nlo, nlo, K2, 1.0, g.LC, nlo, g.RC, nlo, 0.0, c.w_a.p, nlo, 1))) return rc;
+      if ((rc = dgemm(s, 'N', 'T', nlo, nlo, K2, 1.0, g.LC, nlo, g.RC2, nlo, 0.0, c.w_b.p, nlo, 1))) return rc;
+      hipLaunchKernelGGL(k_add_cplx_lower, dim3((nlo + 255) / 256, nlo), dim3(256), 0, s, nlo, c.nv, n, c.w_a.p,
+                         c.w_b.p, nlo, c.d_hs.p);
+      W2K_HIP(hipGetLastError());
+    }
+  }
+  return 0;
+}
+
+}  // namespace w2k

@@ -1,0 +1,619 @@
+// Context, per-run tables and the extern "C" entry points of libw2klapw1.so.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "w2k_internal.h"
+#include "../../include/w2k_lapw1_test.h"
+
+namespace w2k {
+
+int w2k_hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+  if (getenv("W2K_VERBOSE")) fprintf(stderr, "w2k: %s failed (%s) at %s:%d\n", what, hipGetErrorString(e), file, line);
+  return e == hipErrorOutOfMemory ? W2K_ENOMEM : W2K_EDEVICE;
+}
+
+template <class T>
+int DBuf<T>::alloc(size_t count)
+{
+  if (count <= n && p) return 0;
+  release();
+  if (count == 0) count = 1;
+  if (hipMalloc(&p, sizeof(T) * count) != hipSuccess) {
+    p = nullptr;
+    n = 0;
+    return W2K_ENOMEM;
+  }
+  n = count;
+  return 0;
+}
+template <class T>
+void DBuf<T>::release()
+{
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+}
+template struct DBuf<double>;
+template struct DBuf<int>;
+
+template <class T>
+static int upload(DBuf<T> &b, const T *src, size_t count, hipStream_t s)
+{
+  int rc = b.alloc(count);
+  if (rc) return rc;
+  if (count) W2K_HIP(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+// ---------------------------------------------------------------- per-run upload
+struct GEntry {
+  int l0, m0, lp, mp, lmx;
+  double gr, gi;
+};
+
+// LMSORT (lmsort.f): stable order by (L0,M0,LP,MP,LMX), L0=LQNS(3)-1,
+// M0=LQNS(6), LP=LQNS(1)-1, MP=LQNS(5), LMX=LQNS(4); duplicates keep the first.
+static std::vector<GEntry> lmsort(const w2k_run_desc *r, int a)
+{
+  std::vector<GEntry> v;
+  const int nq = r->lqind[a];
+  for (int q = 0; q < nq; q++) {
+    const int *lq = r->lqns + 6 * (q + (size_t)r->ngaud * a);
+    GEntry g{lq[2] - 1, lq[5], lq[0] - 1, lq[4], lq[3], r->gfac[2 * (q + (size_t)r->ngaud * a)],
+             r->gfac[2 * (q + (size_t)r->ngaud * a) + 1]};
+    v.push_back(g);
+  }
+  auto key = [](const GEntry &x) { return std::make_tuple(x.l0, x.m0, x.lp, x.mp, x.lmx); };
+  std::stable_sort(v.begin(), v.end(), [&](const GEntry &x, const GEntry &y) { return key(x) < key(y); });
+  std::vector<GEntry> u;
+  for (auto &g : v)
+    if (u.empty() || key(u.back()) != key(g)) u.push_back(g);
+  return u;
+}
+
+int run_upload(Ctx &c, const w2k_run_desc *r)
+{
+  RunDev &R = c.run;
+  hipStream_t s = c.stream;
+  if (!r || r->nat < 1 || r->ndf < r->nat || r->ltop < 3 || r->ltop > W2K_LMAX || r->lmaxd != W2K_LMAX ||
+      r->nnsl < 1 || r->nnsl > W2K_NSLMAX)
+    return W2K_EINVAL;
+  R.is_complex = r->is_complex ? 1 : 0;
+  R.nat = r->nat;
+  R.ndf = r->ndf;
+  R.ltop = r->ltop;
+  R.nnsl = r->nnsl;
+  R.ins = r->ins;
+  R.nlo = r->nlo;
+  R.vi = r->vi;
+  const int nat = r->nat, ndf = r->ndf;
+  R.mult.assign(r->mult, r->mult + nat);
+  R.first_eq.resize(nat);
+  R.atom_of.resize(ndf);
+  int e = 0;
+  for (int a = 0; a < nat; a++) {
+    R.first_eq[a] = e;
+    for (int m = 0; m < R.mult[a]; m++) {
+      if (e >= ndf) return W2K_EINVAL;
+      R.atom_of[e++] = a;
+    }
+  }
+  if (e != ndf) return W2K_EINVAL;
+  int rc;
+  if ((rc = upload(R.d_pos, r->pos, (size_t)3 * ndf, s))) return rc;
+  if ((rc = upload(R.d_rmt, r->rmt, (size_t)nat, s))) return rc;
+  if ((rc = upload(R.d_v, r->v, (size_t)nat, s))) return rc;
+  if ((rc = upload(R.d_rotloc, r->rotloc, (size_t)9 * nat, s))) return rc;
+  if ((rc = upload(R.d_rotij, r->rotij, (size_t)9 * ndf, s))) return rc;
+  if ((rc = upload(R.d_mult, R.mult.data(), (size_t)nat, s))) return rc;
+  if ((rc = upload(R.d_atom_of, R.atom_of.data(), (size_t)ndf, s))) return rc;
+  if ((rc = upload(R.d_first_eq, R.first_eq.data(), (size_t)nat, s))) return rc;
+  // radial parameters [a][6][LMAX]
+  std::vector<double> rad((size_t)nat * 6 * W2K_LMAX);
+  const double *src[6] = {r->u, r->up, r->du, r->dup, r->chn, r->e};
+  for (int a = 0; a < nat; a++)
+    for (int k = 0; k < 6; k++)
+      for (int l = 0; l < W2K_LMAX; l++) rad[((size_t)a * 6 + k) * W2K_LMAX + l] = src[k][l + W2K_LMAX * a];
+  if ((rc = upload(R.d_rad, rad.data(), rad.size(), s))) return rc;
+  if ((rc = upload(R.d_lapw, r->lapw, (size_t)nat * W2K_LMAX, s))) return rc;
+  // warped interstitial potential
+  if (r->ins == 0 && r->warp) {
+    R.warp1 = r->warp1; R.warp2 = r->warp2; R.warp3 = r->warp3;
+    const size_t nw = (size_t)(2 * R.warp1 + 1) * (2 * R.warp2 + 1) * (2 * R.warp3 + 1);
+    if ((rc = upload(R.d_warp, r->warp, 2 * nw, s))) return rc;
+  } else {
+    R.warp1 = R.warp2 = R.warp3 = 0;
+    double z[2] = {0.0, 0.0};
+    if ((rc = upload(R.d_warp, z, 2, s))) return rc;
+  }
+  // step function: radius groups, 3 V_a, Theta(0) (hamilt.F:322-329)
+  {
+    std::vector<int> order(nat);
+    for (int a = 0; a < nat; a++) order[a] = a;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return r->rmt[x] < r->rmt[y]; });
+    std::vector<int> gend;
+    std::vector<double> gr;
+    for (int k = 0; k < nat; k++) {
+      if (k > 0 && r->rmt[order[k]] != r->rmt[order[k - 1]]) gend.push_back(k);
+      if (gr.empty() || r->rmt[order[k]] != gr.back()) gr.push_back(r->rmt[order[k]]);
+    }
+    gend.push_back(nat);
+    R.ngrp = (int)gend.size();
+    std::vector<double> v3(nat);
+    double th0 = 0.0;
+    for (int a = 0; a < nat; a++) {
+      v3[a] = 3.0 * r->v[a];
+      th0 = th0 + 1.0 / (double)nat - r->v[a] * (double)r->mult[a];
+    }
+    R.theta0 = th0;
+    if ((rc = upload(R.d_aorder, order.data(), order.size(), s))) return rc;
+    if ((rc = upload(R.d_grp_end, gend.data(), gend.size(), s))) return rc;
+    if ((rc = upload(R.d_grp_r, gr.data(), gr.size(), s))) return rc;
+    if ((rc = upload(R.d_v3, v3.data(), v3.size(), s))) return rc;
+  }
+  // local orbitals
+  {
+    const double PI = 4.0 * atan(1.0);
+    auto LO = [&](const double *arr, int l, int jlo, int a) { return arr[l + W2K_LOMAXP * (jlo + W2K_NLOAT * a)]; };
+    auto LO2 = [&](const double *arr, int l, int a) { return arr[l + W2K_LOMAXP * a]; };
+    auto RAD = [&](const double *arr, int l, int a) { return arr[l + W2K_LMAX * a]; };
+    std::vector<double> loabc((size_t)nat * W2K_LOMAXP * W2K_NLOAT * 3, 0.0);
+    std::vector<int> lorow((size_t)nat * W2K_LOMAXP * W2K_NLOAT * 2, -1);
+    std::vector<int> locol;
+    std::vector<double> lopar;
+    int nnlo = 0, cnt = 0;
+    for (int a = 0; a < nat; a++) {
+      const double rr = r->rmt[a];
+      const double pi4r2v = 4.0 * PI * sqrt(r->vi) * rr * rr;
+      for (int l = 0; l <= W2K_LOMAX; l++) {
+        const int nil = r->ilo[l + W2K_LOMAXP * a];
+        for (int jlo = 0; jlo < nil; jlo++) {
+          const double alo = LO(r->alo, l, jlo, a), blo = LO(r->blo, l, jlo, a), clo = LO(r->clo, l, jlo, a);
+          const double elo = LO(r->elo, l, jlo, a);
+          double *ab = &loabc[3 * (l + W2K_LOMAXP * (jlo + W2K_NLOAT * a))];
+          ab[0] = alo; ab[1] = blo; ab[2] = clo;
+          // LO rows of (a, l, jlo): the LOHNS range (lohns.f), 0-based relative to NV
+          int *rw = &lorow[2 * (l + W2K_LOMAXP * (jlo + W2K_NLOAT * a))];
+          rw[0] = cnt;
+          rw[1] = cnt + (2 * l + 1) * R.mult[a] - 1;
+          cnt += (2 * l + 1) * R.mult[a];
+          const double pi12 = LO2(r->pi12lo, l, a), pe12 = LO2(r->pe12lo, l, a), dplo = LO2(r->dplo, l, a);
+          const double el = RAD(r->e, l, a), chn = RAD(r->chn, l, a);
+          const double HALF = 0.5;
+          // hamilt.F:520-548
+          const double c1r = alo + pi12 * clo;
+          const double c2r = blo * chn + pe12 * clo;
+          const double c11r = alo * el + HALF * (blo + clo * pi12 * (elo + el));
+          const double c12r = blo * chn * el + HALF * (alo + clo * pi12 + clo * pe12 * (elo + el));
+          const double c3r = clo + pe12 * blo + alo * pi12;
+          const double c13r = clo * elo + HALF * (blo * pi12 + (elo + el) * (alo * pi12 + pe12 * blo));
+          const double akinlor = HALF * (rr * rr) * (alo * RAD(r->du, l, a) + blo * RAD(r->dup, l, a) + clo * dplo);
+          for (int jeqo = 1; jeqo <= R.mult[a]; jeqo++) {
+            for (int mo1 = -l; mo1 <= l; mo1++) {
+              nnlo++;
+              int cc[8] = {a, l, jlo + 1, jeqo, mo1, nnlo, R.first_eq[a], R.mult[a]};
+              locol.insert(locol.end(), cc, cc + 8);
+              double pp[8] = {c1r, c2r, c3r, c11r, c12r, c13r, akinlor, pi4r2v};
+              lopar.insert(lopar.end(), pp, pp + 8);
+            }
+          }
+        }
+      }
+    }
+    R.nlo_cols = nnlo;
+    R.locol = locol;
+    if ((rc = upload(R.d_loabc, loabc.data(), loabc.size(), s))) return rc;
+    if ((rc = upload(R.d_lorow, lorow.data(), lorow.size(), s))) return rc;
+    if ((rc = upload(R.d_locol, locol.data(), locol.size(), s))) return rc;
+    if ((rc = upload(R.d_lopar, lopar.data(), lopar.size(), s))) return rc;
+  }
+  // non-spherical coupling coefficients (hns.F:266-380), per atom type
+  {
+    R.lindex.assign(nat, 0);
+    R.lm0_of.assign((size_t)nat * W2K_NSL2, 0);
+    std::vector<double> T((size_t)nat * W2K_NSL2 * W2K_NSL2 * 18, 0.0);
+    const int LOP2 = W2K_LOMAX + 2;  // leading dimension of VNS*LO
+    auto VNS = [&](const double *arr, int x, int lmx, int y, int a) {
+      return arr[x + W2K_NSLMAX * (lmx + (size_t)r->lmmxd * (y + W2K_NSLMAX * (size_t)a))];
+    };
+    auto VNSLO = [&](const double *arr, int x, int lmx, int y, int a) {
+      return arr[x + LOP2 * (lmx + (size_t)r->lmmxd * (y + W2K_NSLMAX * (size_t)a))];
+    };
+    for (int a = 0; a < nat; a++) {
+      std::vector<GEntry> gl = lmsort(r, a);
+      int q = 0, k = 0;
+      while (q < (int)gl.size()) {
+        const int L0 = gl[q].l0, M0 = gl[q].m0;
+        if (k >= W2K_NSL2 || L0 >= r->nnsl) return W2K_EINVAL;
+        R.lm0_of[a * W2K_NSL2 + k] = L0 * (L0 + 1) + M0;
+        while (q < (int)gl.size() && gl[q].l0 == L0 && gl[q].m0 == M0) {
+          const int LP = gl[q].lp, MP = gl[q].mp;
+          const int p = LP * (LP + 1) + MP;
+          double t[18] = {0};
+          // t: TUU, TDU, TUU21, TUD, TDD, TUD21, TUU12, TDU12, TUU22 (re, im)
+          while (q < (int)gl.size() && gl[q].l0 == L0 && gl[q].m0 == M0 && gl[q].lp == LP && gl[q].mp == MP) {
+            const int lmx = gl[q].lmx - 1;
+            const double gr = gl[q].gr, gi = gl[q].gi;
+            double v;
+            v = VNS(r->vns1, LP, lmx, L0, a); t[0] += gr * v; t[1] += gi * v;   // TUU
+            v = VNS(r->vns3, L0, lmx, LP, a); t[2] += gr * v; t[3] += gi * v;   // TDU
+            v = VNS(r->vns3, LP, lmx, L0, a); t[6] += gr * v; t[7] += gi * v;   // TUD
+            v = VNS(r->vns2, LP, lmx, L0, a); t[8] += gr * v; t[9] += gi * v;   // TDD
+            int loorl0 = 0;
+            if (L0 <= W2K_LOMAX) {
+              loorl0 = r->loor[L0 + W2K_LOMAXP * a];
+              if (loorl0) {
+                v = VNSLO(r->vns1lo, L0, lmx, LP, a); t[12] += gr * v; t[13] += gi * v;  // TUU12
+                v = VNSLO(r->vns2lo, L0, lmx, LP, a); t[14] += gr * v; t[15] += gi * v;  // TDU12
+              }
+            }
+            if (LP <= W2K_LOMAX && r->loor[LP + W2K_LOMAXP * a]) {
+              if (loorl0) { v = VNSLO(r->vns3lo, LP, lmx, L0, a); t[16] += gr * v; t[17] += gi * v; }  // TUU22
+              v = VNSLO(r->vns1lo, LP, lmx, L0, a); t[4] += gr * v; t[5] += gi * v;    // TUU21
+              v = VNSLO(r->vns2lo, LP, lmx, L0, a); t[10] += gr * v; t[11] += gi * v;  // TUD21
+            }
+            q++;
+          }
+          double *dst = &T[(((size_t)a * W2K_NSL2 + k) * W2K_NSL2 + p) * 18];
+          for (int z = 0; z < 18; z++) dst[z] = t[z];
+        }
+        k++;
+      }
+      R.lindex[a] = k;
+    }
+    if ((rc = upload(R.d_tcoef, T.data(), T.size(), s))) return rc;
+    if ((rc = upload(R.d_lm0_of, R.lm0_of.data(), R.lm0_of.size(), s))) return rc;
+    if ((rc = upload(R.d_lindex, R.lindex.data(), R.lindex.size(), s))) return rc;
+  }
+  W2K_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+__global__ void k_swap_diag(int n, int cpx, double *hs, double *hsdiag)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t o = (size_t)i * (n + 1);
+  if (cpx) {
+    double r = hs[2 * o], im = hs[2 * o + 1];
+    hs[2 * o] = hsdiag[2 * i];
+    hs[2 * o + 1] = hsdiag[2 * i + 1];
+    hsdiag[2 * i] = r;
+    hsdiag[2 * i + 1] = im;
+  } else {
+    double t = hs[o];
+    hs[o] = hsdiag[i];
+    hsdiag[i] = t;
+  }
+}
+
+__global__ void k_diag_copy(int n, int cpx, int dir, double *hs, double *buf)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t o = (size_t)i * (n + 1);
+  const int w = cpx ? 2 : 1;
+  for (int k = 0; k < w; k++) {
+    if (dir == 0) buf[w * i + k] = hs[w * o + k];
+    else hs[w * o + k] = buf[w * i + k];
+  }
+}
+
+static int timed_begin(Ctx &c, int slot) { W2K_HIP(hipEventRecord(c.ev[2 * slot], c.stream)); return 0; }
+static int timed_end(Ctx &c, int slot)
+{
+  W2K_HIP(hipEventRecord(c.ev[2 * slot + 1], c.stream));
+  return 0;
+}
+
+}  // namespace w2k
+
+using namespace w2k;
+
+extern "C" {
+
+const char *w2k_version(void) { return "w2k-mi355x 0.1 (gfx950)"; }
+
+int w2k_create(w2k_ctx **out, int device, const w2k_run_desc *run)
+{
+  if (!out || !run) return W2K_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return W2K_ENODEV;
+  if (hipSetDevice(device) != hipSuccess) return W2K_ENODEV;
+  w2k_ctx *x = new (std::nothrow) w2k_ctx;
+  if (!x) return W2K_ENOMEM;
+  Ctx &c = x->c;
+  c.device = device;
+  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete x;
+    return W2K_EDEVICE;
+  }
+  for (int k = 0; k < 16; k++) (void)hipEventCreate(&c.ev[k]);
+  int rc = run_upload(c, run);
+  if (rc) {
+    w2k_destroy(x);
+    return rc;
+  }
+  *out = x;
+  return 0;
+}
+
+int w2k_set_run(w2k_ctx *x, const w2k_run_desc *run)
+{
+  if (!x) return W2K_EINVAL;
+  if (hipSetDevice(x->c.device) != hipSuccess) return W2K_ENODEV;
+  x->c.have_hamilt = 0;
+  return run_upload(x->c, run);
+}
+
+void w2k_destroy(w2k_ctx *x)
+{
+  if (!x) return;
+  (void)hipSetDevice(x->c.device);
+  if (x->c.stream) (void)hipStreamSynchronize(x->c.stream);
+  for (int k = 0; k < 16; k++) (void)hipEventDestroy(x->c.ev[k]);
+  if (x->c.stream) (void)hipStreamDestroy(x->c.stream);
+  x->c.stream = nullptr;
+  delete x;
+}
+
+int w2k_set_basis(w2k_ctx *x, int nv, int nlo, const int *kzz, const double *xk, const double *yk, const double *zk,
+                  const double *rk)
+{
+  if (!x || nv < 0 || nlo < 0 || nv + nlo < 1 || !kzz || !xk || !yk || !zk || !rk) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (nlo != 0 && nlo != c.run.nlo_cols) return W2K_EINVAL;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  const int n = nv + nlo;
+  c.nv = nv;
+  c.nlo = nlo;
+  c.n = n;
+  c.have_hamilt = 0;
+  c.ne = 0;
+  std::vector<double> kv((size_t)4 * n);
+  for (int i = 0; i < n; i++) {
+    kv[4 * i] = xk[i];
+    kv[4 * i + 1] = yk[i];
+    kv[4 * i + 2] = zk[i];
+    kv[4 * i + 3] = rk[i];
+  }
+  int rc;
+  if ((rc = upload(c.d_kzz, kzz, (size_t)3 * n, c.stream))) return rc;
+  if ((rc = upload(c.d_kvec, kv.data(), kv.size(), c.stream))) return rc;
+  const int w = c.run.is_complex ? 2 : 1;
+  if ((rc = c.d_hs.alloc((size_t)w * n * n))) return rc;
+  if ((rc = c.d_hsdiag.alloc((size_t)w * n))) return rc;
+  W2K_HIP(hipMemsetAsync(c.d_hs.p, 0, sizeof(double) * w * (size_t)n * n, c.stream));
+  W2K_HIP(hipMemsetAsync(c.d_hsdiag.p, 0, sizeof(double) * w * (size_t)n, c.stream));
+  W2K_HIP(hipStreamSynchronize(c.stream));
+  c.have_basis = 1;
+  return 0;
+}
+
+int w2k_hamilt(w2k_ctx *x)
+{
+  if (!x) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  int rc = timed_begin(c, 0);
+  if (!rc) rc = w2k::hamilt(c);
+  if (!rc) rc = timed_end(c, 0);
+  if (!rc) {
+    W2K_HIP(hipStreamSynchronize(c.stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c.ev[0], c.ev[1]);
+    c.last_ms[0] = ms;
+  }
+  return rc;
+}
+
+int w2k_swap_diag(w2k_ctx *x)
+{
+  if (!x) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  hipLaunchKernelGGL(k_swap_diag, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run.is_complex, c.d_hs.p,
+                     c.d_hsdiag.p);
+  W2K_HIP(hipGetLastError());
+  return 0;
+}
+
+int w2k_hns(w2k_ctx *x)
+{
+  if (!x) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  int rc = timed_begin(c, 1);
+  if (!rc) rc = w2k::hns(c);
+  if (!rc) rc = timed_end(c, 1);
+  if (!rc) {
+    W2K_HIP(hipStreamSynchronize(c.stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c.ev[2], c.ev[3]);
+    c.last_ms[1] = ms;
+  }
+  return rc;
+}
+
+int w2k_seclr4(w2k_ctx *x, double el, double eu, int nume, int *ne, int *nbelw)
+{
+  if (!x || !ne || !nbelw || nume < 0) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  int rc = timed_begin(c, 2);
+  if (!rc) rc = w2k::seclr4(c, el, eu, nume, ne, nbelw);
+  int rc2 = timed_end(c, 2);
+  if (!rc) rc = rc2;
+  W2K_HIP(hipStreamSynchronize(c.stream));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, c.ev[4], c.ev[5]);
+  c.last_ms[2] = ms;
+  return rc;
+}
+
+int w2k_calkpt(w2k_ctx *x, int nohns, double el, double eu, int nume, int *ne, int *nbelw)
+{
+  int rc = w2k_hamilt(x);
+  if (rc) return rc;
+  if ((rc = w2k_swap_diag(x))) return rc;
+  if (!nohns && (rc = w2k_hns(x))) return rc;
+  return w2k_seclr4(x, el, eu, nume, ne, nbelw);
+}
+
+int w2k_get_hs(w2k_ctx *x, double *hs, int ld, double *hsdiag)
+{
+  if (!x) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis || ld < c.n) return W2K_EINVAL;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  const int w = c.run.is_complex ? 2 : 1;
+  if (hs)
+    W2K_HIP(hipMemcpy2DAsync(hs, sizeof(double) * w * (size_t)ld, c.d_hs.p, sizeof(double) * w * (size_t)c.n,
+                             sizeof(double) * w * (size_t)c.n, c.n, hipMemcpyDeviceToHost, c.stream));
+  if (hsdiag)
+    W2K_HIP(hipMemcpyAsync(hsdiag, c.d_hsdiag.p, sizeof(double) * w * (size_t)c.n, hipMemcpyDeviceToHost, c.stream));
+  W2K_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int w2k_put_hs(w2k_ctx *x, const double *hs, int ld, const double *hsdiag)
+{
+  if (!x) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis || ld < c.n) return W2K_EINVAL;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  const int w = c.run.is_complex ? 2 : 1;
+  if (hs)
+    W2K_HIP(hipMemcpy2DAsync(c.d_hs.p, sizeof(double) * w * (size_t)c.n, hs, sizeof(double) * w * (size_t)ld,
+                             sizeof(double) * w * (size_t)c.n, c.n, hipMemcpyHostToDevice, c.stream));
+  if (hsdiag)
+    W2K_HIP(hipMemcpyAsync(c.d_hsdiag.p, hsdiag, sizeof(double) * w * (size_t)c.n, hipMemcpyHostToDevice, c.stream));
+  W2K_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int w2k_get_diag(w2k_ctx *x, double *hs_diag, double *hsdiag)
+{
+  if (!x) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  const int w = c.run.is_complex ? 2 : 1;
+  int rc;
+  if ((rc = c.w_v.alloc((size_t)w * c.n))) return rc;
+  if (hs_diag) {
+    hipLaunchKernelGGL(k_diag_copy, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run.is_complex, 0,
+                       c.d_hs.p, c.w_v.p);
+    W2K_HIP(hipMemcpyAsync(hs_diag, c.w_v.p, sizeof(double) * w * c.n, hipMemcpyDeviceToHost, c.stream));
+  }
+  if (hsdiag) W2K_HIP(hipMemcpyAsync(hsdiag, c.d_hsdiag.p, sizeof(double) * w * c.n, hipMemcpyDeviceToHost, c.stream));
+  W2K_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int w2k_put_diag(w2k_ctx *x, const double *hs_diag, const double *hsdiag)
+{
+  if (!x) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  const int w = c.run.is_complex ? 2 : 1;
+  int rc;
+  if ((rc = c.w_v.alloc((size_t)w * c.n))) return rc;
+  if (hs_diag) {
+    W2K_HIP(hipMemcpyAsync(c.w_v.p, hs_diag, sizeof(double) * w * c.n, hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_diag_copy, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run.is_complex, 1,
+                       c.d_hs.p, c.w_v.p);
+  }
+  if (hsdiag) W2K_HIP(hipMemcpyAsync(c.d_hsdiag.p, hsdiag, sizeof(double) * w * c.n, hipMemcpyHostToDevice, c.stream));
+  W2K_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int w2k_get_albl(w2k_ctx *x, double *al, double *bl)
+{
+  if (!x) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_hamilt) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  const size_t cnt = (size_t)c.n * c.run.ltop * c.run.nat;
+  if (al) W2K_HIP(hipMemcpyAsync(al, c.d_al.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c.stream));
+  if (bl) W2K_HIP(hipMemcpyAsync(bl, c.d_bl.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c.stream));
+  W2K_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int w2k_get_eig(w2k_ctx *x, double *eigval, double *z, int ldz)
+{
+  if (!x) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (ldz < c.n) return W2K_EINVAL;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  const int w = c.run.is_complex ? 2 : 1;
+  if (c.ne > 0) {
+    if (eigval) W2K_HIP(hipMemcpyAsync(eigval, c.d_eig.p, sizeof(double) * c.ne, hipMemcpyDeviceToHost, c.stream));
+    if (z)
+      W2K_HIP(hipMemcpy2DAsync(z, sizeof(double) * w * (size_t)ldz, c.d_z.p, sizeof(double) * w * (size_t)c.n,
+                               sizeof(double) * w * (size_t)c.n, c.ne, hipMemcpyDeviceToHost, c.stream));
+  }
+  W2K_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int w2k_device_buffers(w2k_ctx *x, void **hs, void **hsdiag, void **eigval, void **z)
+{
+  if (!x) return W2K_EINVAL;
+  if (hs) *hs = x->c.d_hs.p;
+  if (hsdiag) *hsdiag = x->c.d_hsdiag.p;
+  if (eigval) *eigval = x->c.d_eig.p;
+  if (z) *z = x->c.d_z.p;
+  return 0;
+}
+
+void *w2k_stream(w2k_ctx *x) { return x ? (void *)x->c.stream : nullptr; }
+
+int w2k_sync(w2k_ctx *x)
+{
+  if (!x) return W2K_EINVAL;
+  if (hipSetDevice(x->c.device) != hipSuccess) return W2K_ENODEV;
+  W2K_HIP(hipStreamSynchronize(x->c.stream));
+  return 0;
+}
+
+int w2k_last_timings(w2k_ctx *x, double *ms, int n)
+{
+  if (!x || !ms) return W2K_EINVAL;
+  for (int k = 0; k < n && k < 8; k++) ms[k] = x->c.last_ms[k];
+  return 0;
+}
+
+}  // extern "C"
+
+extern "C" int w2k_test_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+                              const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri)
+{
+  if (hipSetDevice(0) != hipSuccess) return W2K_ENODEV;
+  const size_t na = (size_t)lda * ((ta == 'N' || ta == 'n') ? k : m);
+  const size_t nb = (size_t)ldb * ((tb == 'N' || tb == 'n') ? n : k);
+  const size_t nc = (size_t)ldc * n;
+  DBuf<double> dA, dB, dC;
+  int rc;
+  if ((rc = dA.alloc(na)) || (rc = dB.alloc(nb)) || (rc = dC.alloc(nc))) return rc;
+  W2K_HIP(hipMemcpy(dA.p, A, sizeof(double) * na, hipMemcpyHostToDevice));
+  W2K_HIP(hipMemcpy(dB.p, B, sizeof(double) * nb, hipMemcpyHostToDevice));
+  W2K_HIP(hipMemcpy(dC.p, C, sizeof(double) * nc, hipMemcpyHostToDevice));
+  rc = dgemm(nullptr, ta, tb, m, n, k, alpha, dA.p, lda, dB.p, ldb, beta, dC.p, ldc, tri, atri, btri);
+  if (rc) return rc;
+  W2K_HIP(hipDeviceSynchronize());
+  W2K_HIP(hipMemcpy(C, dC.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
+  return 0;
+}

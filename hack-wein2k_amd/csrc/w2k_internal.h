@@ -1,0 +1,127 @@
+// Internal declarations of the MI355X lapw1 library (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/w2k_lapw1.h"
+
+#define W2K_LOMAXP (W2K_LOMAX + 1)
+#define W2K_NSL2 (W2K_NSLMAX * W2K_NSLMAX)
+
+namespace w2k {
+
+// ---------------------------------------------------------------- errors
+struct Status {
+  int code = 0;
+};
+
+#define W2K_HIP(call)                                       \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return w2k_hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+int w2k_hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+// ---------------------------------------------------------------- device buffers
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count);
+  void release();
+  ~DBuf() { release(); }
+};
+
+// ---------------------------------------------------------------- per-run data (device)
+struct RunDev {
+  int is_complex = 0, nat = 0, ndf = 0, ltop = 0, nnsl = 0, ins = 0, nlo = 0;
+  double vi = 0;
+  std::vector<int> mult, atom_of;  // atom_of[e] = inequivalent index of equivalent atom e
+  std::vector<int> first_eq;       // first equivalent-atom index of each type
+  // host copies (small) used to derive device tables
+  std::vector<double> pos, rmt, v, rotloc, rotij;
+  std::vector<double> u, up, du, dup, chn, e;  // [a][l] (LMAX stride)
+  std::vector<int> lapw, ilo, loor;
+  std::vector<double> alo, blo, clo, elo, pi12lo, pe12lo, dplo;
+  int warp1 = 0, warp2 = 0, warp3 = 0;
+  // device copies
+  DBuf<double> d_pos, d_rmt, d_v, d_rotloc, d_rotij;
+  DBuf<int> d_mult, d_atom_of, d_first_eq;
+  DBuf<double> d_rad;     // [a][6][LMAX]: u, up, du, dup, chn, e
+  DBuf<int> d_lapw;       // [a][LMAX]
+  DBuf<double> d_warp;    // complex box, interleaved
+  // HNS: per inequivalent atom, the sorted unique (L0,M0) list and the
+  // coupling coefficients T[k][p][9] (complex), k < lindex[a], p < 25
+  std::vector<int> lindex;          // LINDEX per atom type
+  std::vector<int> lm0_of;          // [a][25]: L0*(L0+1)+M0 of the k-th unique (L0,M0)
+  DBuf<int> d_lm0_of;
+  DBuf<double> d_tcoef;             // [a][25][25][9][2]
+  DBuf<int> d_lindex;
+  // LO columns in HAMILT order (hamilt.F:495-552): a, l, jlo, jeqo, mo1, nnlo, first_eq, mult
+  int nlo_cols = 0;
+  std::vector<int> locol;
+  DBuf<int> d_locol;                // [nlo][8]
+  DBuf<double> d_lopar;             // [nlo][8]: c1r,c2r,c3r,c11r,c12r,c13r,akinlor,pi4r2v
+  DBuf<double> d_loabc;             // [nat][LOMAXP][NLOAT][3]: alo, blo, clo
+  DBuf<int> d_lorow;                // [nat][LOMAXP][NLOAT][2]: LO rows (0-based, relative to NV) of (a,l,jlo)
+  // atom types ordered by sphere radius (groups share one sincos per pair)
+  int ngrp = 0;
+  DBuf<int> d_aorder, d_grp_end;
+  DBuf<double> d_grp_r, d_v3;
+  double theta0 = 0;
+};
+
+// ---------------------------------------------------------------- context
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  RunDev run;
+  // basis
+  int nv = 0, nlo = 0, n = 0;
+  int have_basis = 0, have_hamilt = 0;
+  DBuf<int> d_kzz;          // [n][3]
+  DBuf<double> d_kvec;      // [n][4]: xk, yk, zk, rk
+  DBuf<double> d_hs;        // n*n (complex: 2*n*n)
+  DBuf<double> d_hsdiag;    // n (complex: 2n)
+  DBuf<double> d_al, d_bl;  // [n][ltop][nat] Fortran order (n fastest)
+  DBuf<double> d_cs;        // [ndf][n][2]: cos, sin of 2 pi KZZ.tau
+  DBuf<double> d_theta;     // step-function box (complex interleaved)
+  int tb1 = 0, tb2 = 0, tb3 = 0;
+  DBuf<double> d_colcoef;   // HAMILT per-column MT coefficients
+  // HNS work
+  DBuf<double> d_hnsL, d_hnsR;  // rank-update operands
+  // eigen results
+  int ne = 0, nbelw = 0;
+  DBuf<double> d_eig, d_z;
+  // SECLR4 work (grown on demand)
+  DBuf<double> w_a, w_b, w_c, w_v;
+  DBuf<double> w_small;
+  DBuf<int> w_int;
+  // timing
+  hipEvent_t ev[16];
+  double last_ms[8] = {0};
+};
+
+// kernels / stages (implemented in the .hip files)
+int run_upload(Ctx &c, const w2k_run_desc *r);
+int hamilt(Ctx &c);
+int hns(Ctx &c);
+int seclr4(Ctx &c, double el, double eu, int nume, int *ne, int *nbelw);
+
+// generic FP64 GEMM on the device: C = alpha * op(A) * op(B) + beta * C
+// tri: 0 full, 1 lower (i >= j) only, 2 upper (i <= j) only (tiles outside skipped)
+// atri/btri: 0 dense, 1 A (resp. B) upper triangular, 2 lower triangular (zero
+// blocks skipped; the caller guarantees the structure)
+int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+          const double *B, int ldb, double beta, double *C, int ldc, int tri = 0, int atri = 0, int btri = 0);
+
+}  // namespace w2k
+
+struct w2k_ctx {
+  w2k::Ctx c;
+};

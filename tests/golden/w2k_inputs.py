@@ -446,10 +446,15 @@ def case_rocksalt(n: int, rkmax: float, name: str, seed: int) -> Case:
                 ops_inversion(), rkmax=rkmax, kpoints=[(0, 0, 0, 1)], window=(-9.0, 0.5), seed=seed)
 
 
+def _with_window(c: Case, w) -> Case:
+    c.window = w
+    return c
+
+
 CASES = {
     "c1_fcc": case_c1_fcc,
     "lo_mult2": lambda: case_lo_mult2(False),
     "lo_cplx": lambda: case_lo_mult2(True),
     "rs8": lambda: case_rocksalt(1, 5.0, "rs8", SEED_BASE + 21),
-    "c2_rs64": lambda: case_rocksalt(2, 7.0, "c2_rs64", SEED_BASE + 1),
+    "c2_rs64": lambda: _with_window(case_rocksalt(2, 7.0, "c2_rs64", SEED_BASE + 1), (-9.0, 12.0)),
 }

@@ -1,0 +1,125 @@
+"""HAMILT / HNS on the MI355X against the reference golden vectors (and the
+FP64 MFMA GEMM against numpy).  Bar: H and S elementwise to 1e-12 relative
+(|diff| / max|M| per matrix)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import Fixture, relerr, CASES, PKG
+
+pytestmark = pytest.mark.gpu
+L = PKG.lapw1
+TOL = 1e-12
+
+
+def _dgemm(ta, tb, m, n, k, alpha, A, B, beta, C, tri=0, atri=0, btri=0):
+    lib = L.load_library()
+    fn = lib.w2k_test_dgemm
+    dp = ctypes.POINTER(ctypes.c_double)
+    fn.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                   dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    A = np.asfortranarray(A); B = np.asfortranarray(B); C = np.asfortranarray(C.copy())
+    rc = fn(ta.encode(), tb.encode(), m, n, k, alpha, A.ctypes.data_as(dp), A.shape[0], B.ctypes.data_as(dp),
+            B.shape[0], beta, C.ctypes.data_as(dp), C.shape[0], tri, atri, btri)
+    assert rc == 0
+    return C
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 5), (64, 64, 16), (130, 70, 129), (200, 3, 47)])
+def test_dgemm_matches_numpy(ta, tb, m, n, k):
+    rng = np.random.default_rng(m * 1000 + n * 10 + k)
+    A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+    C = rng.standard_normal((m, n))
+    opA = A if ta == "N" else A.T
+    opB = B if tb == "N" else B.T
+    out = _dgemm(ta, tb, m, n, k, 0.7, A, B, -1.3, C)
+    ref = 0.7 * opA @ opB - 1.3 * C
+    assert np.max(np.abs(out - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("tri", [1, 2])
+def test_dgemm_triangular_output(tri):
+    rng = np.random.default_rng(5)
+    n, k = 150, 40
+    A = rng.standard_normal((n, k)); B = rng.standard_normal((n, k)); C = rng.standard_normal((n, n))
+    out = _dgemm("N", "T", n, n, k, 1.0, A, B, 1.0, C, tri=tri)
+    full = C + A @ B.T
+    mask = np.tril(np.ones((n, n), bool)) if tri == 1 else np.triu(np.ones((n, n), bool))
+    assert np.max(np.abs(out[mask] - full[mask])) < 1e-12
+    assert np.array_equal(out[~mask], C[~mask])
+
+
+@pytest.mark.parametrize("atri,btri", [(1, 0), (2, 0), (0, 1), (0, 2)])
+def test_dgemm_triangular_operands(atri, btri):
+    rng = np.random.default_rng(9)
+    m = n = k = 131
+    A = rng.standard_normal((m, k)); B = rng.standard_normal((k, n)); C = np.zeros((m, n))
+    Am = np.triu(A) if atri == 1 else (np.tril(A) if atri == 2 else A)
+    Bm = np.triu(B) if btri == 1 else (np.tril(B) if btri == 2 else B)
+    out = _dgemm("N", "N", m, n, k, 1.0, A, B, 0.0, C, atri=atri, btri=btri)
+    assert np.max(np.abs(out - Am @ Bm)) < 1e-11
+
+
+def _ctx(fx):
+    ctx = L.Lapw1(fx.run())
+    return ctx
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_hamilt_matches_reference(name):
+    fx = Fixture(name)
+    ctx = _ctx(fx)
+    for ik in range(fx.nk):
+        if not fx.has_full(ik):
+            continue
+        kd = fx.k(ik)
+        ctx.set_basis(fx.basis(ik))
+        ctx.hamilt()
+        hs, hd = ctx.get_hs()
+        ref = kd["HS_HAMILT"]
+        assert relerr(np.triu(hs), np.triu(ref)) < TOL, f"S k{ik}"
+        assert relerr(np.tril(hs, -1), np.tril(ref, -1)) < TOL, f"H k{ik}"
+        assert relerr(hd, kd["HSDIAG_HAMILT"]) < TOL, f"HSDIAG k{ik}"
+        al, bl = ctx.get_albl()
+        assert relerr(al, kd["AL"]) < TOL and relerr(bl, kd["BL"]) < TOL
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_hns_matches_reference(name):
+    fx = Fixture(name)
+    ctx = _ctx(fx)
+    kd = fx.k(0)
+    ctx.set_basis(fx.basis(0))
+    ctx.hamilt()
+    ctx.swap_diag()
+    ctx.hns()
+    hs, hd = ctx.get_hs()
+    ref = kd["HS_HNS"]
+    assert relerr(np.tril(hs), np.tril(ref)) < TOL, "H after HNS"
+    assert relerr(np.triu(hs, 1), np.triu(ref, 1)) < TOL, "S untouched"
+    assert relerr(hd, kd["HSDIAG_HNS"]) < TOL
+    ctx.close()
+
+
+def test_hns_alone_on_reference_input():
+    """HNS applied to the reference's own post-HAMILT matrix (put_hs)."""
+    fx = Fixture("lo_mult2")
+    kd = fx.k(0)
+    ctx = _ctx(fx)
+    ctx.set_basis(fx.basis(0))
+    ctx.hamilt()  # AL/BL tables
+    hs0 = np.array(kd["HS_HAMILT"], order="F")
+    d = np.diag(hs0).copy()
+    np.fill_diagonal(hs0, kd["HSDIAG_HAMILT"])
+    ctx.put_hs(hs0, d)
+    ctx.hns()
+    hs, hd = ctx.get_hs()
+    assert relerr(np.tril(hs), np.tril(kd["HS_HNS"])) < TOL
+    ctx.close()
