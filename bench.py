@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Seconds per k-point of lapw1's hot path (HAMILT + HNS + SECLR4) on one MI355X.
+
+Workload: the 64-atom rocksalt supercell (2x2x2, Mg/O, RMT 2.0/1.6, RKmax 7,
+real/inversion-symmetric, one k-point at Gamma, N = 5951 = 5695 plane waves +
+256 local orbitals, lowest ~20% of the spectrum = 1184 eigenpairs), built from
+the seeded WIEN2k input deck whose reference outputs are stored in
+tests/golden/c2_rs64.npz.  One step is one k-point: HAMILT, the diagonal swap,
+HNS and SECLR4 through the library's C-ABI with the per-run tables and the
+basis already resident on the device (uploaded before the timed region).
+
+The CPU baseline is the reference lapw1 itself (compiled from its sources by
+oracle/Makefile.ref into oracle/_ref/), run on this host on the same deck for
+the same k-point, timed by its own phase clocks; it runs before the GPU is
+touched.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import resource
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(REPO, "tests", "golden", "c2_rs64.npz")
+REF_EXE = os.path.join(REPO, "oracle", "_ref", "lapw1_real")
+
+METRIC = "s per k-point (H/S setup+eigensolve), split-prec vs FP64 at ΔE≤1e-8 Ry"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def load_package():
+    name = "hack_wein2k_amd"
+    spec = importlib.util.spec_from_file_location(name, os.path.join(REPO, "hack-wein2k_amd", "__init__.py"),
+                                                  submodule_search_locations=[os.path.join(REPO, "hack-wein2k_amd")])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def load_workload(L):
+    raw = np.load(FIXTURE)
+    run = L.RunData({k[4:]: raw[k] for k in raw.files if k.startswith("run/")}, bool(int(raw["meta/complex"])))
+    kd = {k[3:]: raw[k] for k in raw.files if k.startswith("k0/")}
+    basis = L.Basis(int(kd["NV"]), int(kd["NLO"]), kd["KZZ"], kd["XK"], kd["YK"], kd["ZK"], kd["RK"])
+    return run, basis, kd
+
+
+def cpu_baseline(kd, threads: int):
+    """The reference lapw1 on the same deck (same seeded inputs, same window)."""
+    if not os.path.exists(REF_EXE):
+        return None
+    sys.path.insert(0, os.path.join(REPO, "tests", "golden"))
+    import w2k_inputs as W
+    case = W.CASES["c2_rs64"]()
+    case.window = (float(kd["EL"]), float(kd["EU"]))
+    with tempfile.TemporaryDirectory(prefix="w2k_ref_") as d:
+        W.write_case(case, d)
+        env = dict(os.environ, OPENBLAS_NUM_THREADS=str(threads), OMP_NUM_THREADS=str(threads))
+
+        def unlimited_stack():
+            resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+        t0 = time.time()
+        r = subprocess.run([REF_EXE, "nodump"], cwd=d, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           preexec_fn=unlimited_stack, timeout=900)
+        wall = time.time() - t0
+        tf = os.path.join(d, "ref_times.txt")
+        if r.returncode != 0 or not os.path.exists(tf):
+            sys.stderr.write(r.stdout.decode(errors="replace")[-2000:])
+            return None
+        t = {k: float(v) for k, v in (l.split() for l in open(tf))}
+    per_k = t["hamilt_wall_s"] + t["hns_wall_s"] + t["seclr4_wall_s"]
+    return {
+        "value": round(per_k, 3),
+        "unit": "s/k-point",
+        "cores": threads,
+        "kind": "reference",
+        "sample": (f"the whole workload: one k-point of the 64-atom rocksalt deck through the reference lapw1 "
+                   f"(flang build of SRC_lapw1); HAMILT {t['hamilt_wall_s']:.2f} s and HNS {t['hns_wall_s']:.2f} s "
+                   f"on 1 thread (serial in the reference), SECLR4 {t['seclr4_wall_s']:.2f} s with OpenBLAS on "
+                   f"{threads} threads; process wall {wall:.1f} s"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(k-points shard across GPUs with no exchange; run one process per GPU)", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = load_package()
+    L = pkg.lapw1
+    run, basis, kd = load_workload(L)
+    el, eu, nume = float(kd["EL"]), float(kd["EU"]), int(kd["NUME"])
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(kd, args.cpu_threads)
+
+    ctx = L.Lapw1(run)
+    ctx.set_basis(basis)
+    for _ in range(args.warmup):
+        ctx.calkpt(el, eu, nume)
+    ctx.sync()
+    names = L.Lapw1.TIMING_NAMES
+    acc = np.zeros(len(names))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ne, nb = ctx.calkpt(el, eu, nume)
+        acc += ctx.timings()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    dev = acc / args.steps
+    ev, _ = ctx.get_eig()
+    ctx.close()
+
+    ok = ne == int(kd["NE"]) and nb == int(kd["NBELW"])
+    dmax = float(np.max(np.abs(ev - kd["EIGVAL"]))) if ok else float("nan")
+
+    nv, nlo, n = basis.nv, basis.nlo, basis.n
+    nat, lt = run.nat, run.ltop
+    sum_lindex = _sum_lindex(run)
+    f_hamilt = 8.0 * nv * nv * nat * lt
+    f_hns = 4.0 * n * n * sum_lindex
+    t = dict(zip(names, dev))
+    cands = {
+        "k_hamilt_pw": (f_hamilt, t["hamilt_pw_kernel"]),
+        "k_dgemm (HNS rank-K)": (f_hns, t["hns_gemm"]),
+    }
+    kname, (flops, kms) = max(cands.items(), key=lambda kv: kv[1][1])
+    achieved = flops / (kms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": round(ms_step / 1e3, 4),
+        "unit": "s/k-point",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded WIEN2k input deck (tests/golden/w2k_inputs.py), plain FP64 path",
+        "config": {"workload": f"64-atom rocksalt 2x2x2 (Mg/O), real, Gamma, RKmax 7, N={n} (NV {nv} + NLO {nlo}), "
+                               f"NE={ne}", "nat": nat, "lt": lt, "sum_lindex": sum_lindex},
+        "roofline": {"kernel": kname, "bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "flops_per_launch": flops, "ms_per_launch": round(kms, 3)},
+        "cpu_baseline": cpu,
+        "phases_ms": {k: round(v, 2) for k, v in t.items()},
+        "check": {"ne": ne, "nbelw": nb, "ne_ref": int(kd["NE"]), "max_abs_eig_dev_ry": dmax},
+    }
+    print(json.dumps(out))
+
+
+def _sum_lindex(run) -> int:
+    """sum over all equivalent atoms of the number of distinct (L0,M0) of HNS."""
+    lqns = run.arr["lqns"]
+    lqind = run.arr["lqind"]
+    mult = run.arr["mult"]
+    tot = 0
+    for a in range(run.nat):
+        seen = set()
+        for q in range(int(lqind[a])):
+            seen.add((int(lqns[2, q, a]), int(lqns[5, q, a])))
+        tot += len(seen) * int(mult[a])
+    return tot
+
+
+if __name__ == "__main__":
+    main()

@@ -160,7 +160,7 @@ int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
     g.k = 0;  // still applies beta
   }
   hipLaunchKernelGGL(k_dgemm, grid, dim3(256), 0, s, g);
-  W2K_HIP(hipGetLastError());
+  W2K_STAGE(s, "k_dgemm");
   return 0;
 }
 

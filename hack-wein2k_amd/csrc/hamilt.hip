@@ -448,11 +448,14 @@ int hamilt(Ctx &c)
   if ((rc = c.d_bl.alloc((size_t)n * lt * nat))) return rc;
   if ((rc = c.d_colcoef.alloc((size_t)4 * nat * lt * (nv > 0 ? nv : 1)))) return rc;
   hipLaunchKernelGGL(k_cs, dim3((n + 255) / 256, ndf), dim3(256), 0, s, n, c.d_kzz.p, R.d_pos.p, c.d_cs.p);
+  W2K_STAGE(s, "k_cs");
   hipLaunchKernelGGL(k_albl, dim3((n + 127) / 128, nat), dim3(128), 0, s, n, nv, lt, c.d_kvec.p, R.d_rmt.p,
                      R.d_rad.p, R.d_lapw.p, c.d_al.p, c.d_bl.p);
+  W2K_STAGE(s, "k_albl");
   if (nv > 0) {
     hipLaunchKernelGGL(k_colcoef, dim3((nv + 127) / 128, nat), dim3(128), 0, s, n, nv, lt, R.vi, R.d_rmt.p,
                        R.d_rad.p, R.d_lapw.p, c.d_al.p, c.d_bl.p, c.d_colcoef.p);
+    W2K_STAGE(s, "k_colcoef");
     PwArgs a;
     a.n = n; a.nv = nv; a.nat = nat; a.lt = lt;
     a.kzz = c.d_kzz.p; a.kv = c.d_kvec.p; a.cs = c.d_cs.p; a.al = c.d_al.p; a.bl = c.d_bl.p;
@@ -461,9 +464,11 @@ int hamilt(Ctx &c)
     a.mult = R.d_mult.p; a.first_eq = R.d_first_eq.p; a.v3 = R.d_v3.p; a.theta0 = R.theta0;
     a.warp = R.d_warp.p; a.w1 = R.warp1; a.w2 = R.warp2; a.w3 = R.warp3;
     a.hs = c.d_hs.p; a.hsdiag = c.d_hsdiag.p;
+    W2K_HIP(hipEventRecord(c.ev[12], s));
     rc = cpx ? dispatch_pw<true>(a, s) : dispatch_pw<false>(a, s);
     if (rc) return rc;
-    W2K_HIP(hipGetLastError());
+    W2K_STAGE(s, "k_hamilt_pw");
+    W2K_HIP(hipEventRecord(c.ev[13], s));
   }
   if (c.nlo > 0) {
     if (c.nlo != R.nlo_cols) return W2K_EINVAL;
@@ -475,7 +480,7 @@ int hamilt(Ctx &c)
     dim3 grid((n + 127) / 128, c.nlo);
     if (cpx) hipLaunchKernelGGL(k_hamilt_lo<true>, grid, dim3(128), 0, s, b);
     else hipLaunchKernelGGL(k_hamilt_lo<false>, grid, dim3(128), 0, s, b);
-    W2K_HIP(hipGetLastError());
+    W2K_STAGE(s, "k_hamilt_lo");
   }
   c.have_hamilt = 1;
   return 0;

@@ -177,10 +177,10 @@ int hns(Ctx &c)
   if ((rc = c.d_hnsL.alloc((size_t)n * kcols))) return rc;
   if ((rc = c.d_hnsR.alloc((size_t)n * kcols * (cpx ? 2 : 1)))) return rc;
   const size_t lcsz = (size_t)(nlo > 0 ? nlo : 1) * 2 * ktot;
-  if ((rc = c.w_small.alloc(lcsz * (cpx ? 3 : 2) + (size_t)ndf))) return rc;
+  if ((rc = c.h_small.alloc(lcsz * (cpx ? 3 : 2)))) return rc;
   int *d_koff;
-  if ((rc = c.w_int.alloc(ndf))) return rc;
-  d_koff = c.w_int.p;
+  if ((rc = c.h_int.alloc(ndf))) return rc;
+  d_koff = c.h_int.p;
   W2K_HIP(hipMemcpyAsync(d_koff, koff.data(), sizeof(int) * ndf, hipMemcpyHostToDevice, s));
   HnsArgs g;
   g.n = n; g.nv = c.nv; g.nnsl = R.nnsl; g.lt = R.ltop; g.ndf = ndf;
@@ -192,40 +192,44 @@ int hns(Ctx &c)
   g.L = c.d_hnsL.p;
   g.R = c.d_hnsR.p;
   g.R2 = cpx ? c.d_hnsR.p + (size_t)n * kcols : nullptr;
-  g.LC = c.w_small.p;
-  g.RC = c.w_small.p + lcsz;
-  g.RC2 = cpx ? c.w_small.p + 2 * lcsz : nullptr;
+  g.LC = c.h_small.p;
+  g.RC = c.h_small.p + lcsz;
+  g.RC2 = cpx ? c.h_small.p + 2 * lcsz : nullptr;
   g.nlo = nlo;
   // atoms without non-spherical terms leave their columns untouched: clear
   W2K_HIP(hipMemsetAsync(c.d_hnsL.p, 0, sizeof(double) * (size_t)n * kcols, s));
   W2K_HIP(hipMemsetAsync(c.d_hnsR.p, 0, sizeof(double) * (size_t)n * kcols * (cpx ? 2 : 1), s));
-  W2K_HIP(hipMemsetAsync(c.w_small.p, 0, sizeof(double) * lcsz * (cpx ? 3 : 2), s));
+  W2K_HIP(hipMemsetAsync(c.h_small.p, 0, sizeof(double) * lcsz * (cpx ? 3 : 2), s));
   dim3 grid((n + 63) / 64, ndf);
   if (cpx) hipLaunchKernelGGL(k_hns_prep<true>, grid, dim3(64), 0, s, g);
   else hipLaunchKernelGGL(k_hns_prep<false>, grid, dim3(64), 0, s, g);
-  W2K_HIP(hipGetLastError());
+  W2K_STAGE(s, "k_hns_prep");
   const int K4 = (int)kcols, K2 = 2 * ktot;
   if (!cpx) {
+    W2K_HIP(hipEventRecord(c.ev[14], s));
     if ((rc = dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, g.R, n, 1.0, c.d_hs.p, n, 1))) return rc;
+    W2K_HIP(hipEventRecord(c.ev[15], s));
     if (nlo > 0) {
       double *hsl = c.d_hs.p + (size_t)c.nv + (size_t)n * c.nv;
       if ((rc = dgemm(s, 'N', 'T', nlo, nlo, K2, 1.0, g.LC, nlo, g.RC, nlo, 1.0, hsl, n, 1))) return rc;
     }
   } else {
     // planar temporaries for the real and imaginary parts
-    if ((rc = c.w_a.alloc((size_t)n * n))) return rc;
-    if ((rc = c.w_b.alloc((size_t)n * n))) return rc;
-    if ((rc = dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, g.R, n, 0.0, c.w_a.p, n, 1))) return rc;
-    if ((rc = dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, g.R2, n, 0.0, c.w_b.p, n, 1))) return rc;
-    hipLaunchKernelGGL(k_add_cplx_lower, dim3((n + 255) / 256, n), dim3(256), 0, s, n, 0, n, c.w_a.p, c.w_b.p, n,
+    if ((rc = c.h_re.alloc((size_t)n * n))) return rc;
+    if ((rc = c.h_im.alloc((size_t)n * n))) return rc;
+    W2K_HIP(hipEventRecord(c.ev[14], s));
+    if ((rc = dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, g.R, n, 0.0, c.h_re.p, n, 1))) return rc;
+    if ((rc = dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, g.R2, n, 0.0, c.h_im.p, n, 1))) return rc;
+    W2K_HIP(hipEventRecord(c.ev[15], s));
+    hipLaunchKernelGGL(k_add_cplx_lower, dim3((n + 255) / 256, n), dim3(256), 0, s, n, 0, n, c.h_re.p, c.h_im.p, n,
                        c.d_hs.p);
-    W2K_HIP(hipGetLastError());
+    W2K_STAGE(s, "k_add_cplx_lower");
     if (nlo > 0) {
-      if ((rc = dgemm(s, 'N', 'T', nlo, nlo, K2, 1.0, g.LC, nlo, g.RC, nlo, 0.0, c.w_a.p, nlo, 1))) return rc;
-      if ((rc = dgemm(s, 'N', 'T', nlo, nlo, K2, 1.0, g.LC, nlo, g.RC2, nlo, 0.0, c.w_b.p, nlo, 1))) return rc;
-      hipLaunchKernelGGL(k_add_cplx_lower, dim3((nlo + 255) / 256, nlo), dim3(256), 0, s, nlo, c.nv, n, c.w_a.p,
-                         c.w_b.p, nlo, c.d_hs.p);
-      W2K_HIP(hipGetLastError());
+      if ((rc = dgemm(s, 'N', 'T', nlo, nlo, K2, 1.0, g.LC, nlo, g.RC, nlo, 0.0, c.h_re.p, nlo, 1))) return rc;
+      if ((rc = dgemm(s, 'N', 'T', nlo, nlo, K2, 1.0, g.LC, nlo, g.RC2, nlo, 0.0, c.h_im.p, nlo, 1))) return rc;
+      hipLaunchKernelGGL(k_add_cplx_lower, dim3((nlo + 255) / 256, nlo), dim3(256), 0, s, nlo, c.nv, n, c.h_re.p,
+                         c.h_im.p, nlo, c.d_hs.p);
+      W2K_STAGE(s, "k_add_cplx_lower");
     }
   }
   return 0;

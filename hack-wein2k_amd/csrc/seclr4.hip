@@ -1,12 +1,875 @@
-// SECLR4 on the GPU -- placeholder until the eigensolver kernels land.
+// SECLR4 on the GPU (seclr4.F:149-358): the generalized eigenproblem
+// H z = e S z for the eigenvalues in (EL, EU], at most NUME of them.
+//
+// Input: HS/HSDIAG as CALKPT leaves them after HNS -- H in the lower triangle
+// (diagonal included), S in the strict upper triangle, diag(S) in HSDIAG.
+//
+// Stages (all on the context stream):
+//   1. Cholesky S = U^T U (POTRF 'U', seclr4.F:196): blocked right-looking,
+//      64x64 diagonal blocks factored in LDS, panel TRSM, trailing SYRK as an
+//      MFMA GEMM on the upper triangle.
+//   2. Reduction to standard form C = U^-T H U^-1 (DSCGST, dscgst.f:99-142):
+//      W = U^-1 by blocked triangular inversion, then X = H W and C = W^T X as
+//      MFMA GEMMs (lower triangle of C, mirrored).
+//   3. Householder tridiagonalisation (DSYTRD2, dsytrd2.f, lower; DLATRD
+//      panels of 32 columns, trailing rank-64 update on MFMA), DLARFG signs.
+//   4. Bisection (DSTEBZ2, RANGE='V', ORDER='B', ABSTOL=0, dstebz2.f:301-652):
+//      split points, per-block Gershgorin intervals, NWL (eigenvalues <= EL),
+//      block-ordered list capped at NUME; one thread per eigenvalue.
+//   5. Inverse iteration (DSTEIN): one thread per eigenvalue, close
+//      eigenvalues perturbed as DSTEIN does, then vectors of one cluster
+//      (gap <= 1e-3 |T|_1) orthogonalised in order (block classical
+//      Gram-Schmidt, twice, on MFMA) and normalised with the largest
+//      component positive.
+//   6. Z = Q Zt with the reflectors in compact WY form (DORMTR), then the
+//      selection sort of dsyevx2.f:381-400, then Z = U^-1 Z (DTRSM,
+//      seclr4.F:354) as a triangular MFMA GEMM with W = U^-1.
+// The arithmetic is real FP64 throughout (the complex build is rejected with
+// W2K_EINVAL here; HAMILT/HNS support it).
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
 #include "w2k_internal.h"
 
 namespace w2k {
-int seclr4(Ctx &c, double el, double eu, int nume, int *ne, int *nbelw)
+
+constexpr int NB = 64;    // Cholesky / inversion block
+constexpr int TNB = 32;   // tridiagonalisation panel
+constexpr int SYMV_ROWS = 256, SYMV_COLS = 256;
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ double block_sum(double v, double *red)
 {
-  (void)c; (void)el; (void)eu; (void)nume;
-  *ne = 0;
-  *nbelw = 0;
-  return W2K_EINVAL;
+  // sum over the block (blockDim.x <= 1024, multiple of 64)
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  const int nw = blockDim.x >> 6;
+  if (threadIdx.x < 64) {
+    t = (threadIdx.x < nw) ? red[threadIdx.x] : 0.0;
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+    if (threadIdx.x == 0) red[32] = t;
+  }
+  __syncthreads();
+  return red[32];
 }
+
+// 1. Sw (upper incl. diagonal) and Hf (full symmetric) from HS/HSDIAG
+__global__ void k_prep(int n, const double *__restrict__ hs, const double *__restrict__ hsdiag,
+                       double *__restrict__ sw, double *__restrict__ hf)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n) return;
+  const size_t o = (size_t)i + (size_t)n * j;
+  const double v = hs[o];
+  if (i < j) {
+    sw[o] = v;
+    hf[o] = hs[(size_t)j + (size_t)n * i];
+  } else if (i == j) {
+    sw[o] = hsdiag[i];
+    hf[o] = v;
+  } else {
+    sw[o] = 0.0;
+    hf[o] = v;
+  }
+}
+
+// unblocked Cholesky of the kb x kb diagonal block at (k0,k0), upper
+__global__ __launch_bounds__(256) void k_potf2(int n, int k0, int kb, double *__restrict__ a, int *__restrict__ info)
+{
+  __shared__ double s[NB][NB + 1];
+  __shared__ double piv;
+  const int t = threadIdx.x;
+  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    s[r][c] = (r <= c) ? a[(size_t)(k0 + r) + (size_t)n * (k0 + c)] : 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < kb; j++) {
+    if (t == 0) {
+      double ajj = s[j][j];
+      if (!(ajj > 0.0)) {
+        if (*info == 0) *info = k0 + j + 1;
+        ajj = 1.0;  // keep the arithmetic finite; the caller reports INFO
+      }
+      piv = sqrt(ajj);
+      s[j][j] = piv;
+    }
+    __syncthreads();
+    for (int c = j + 1 + t; c < kb; c += blockDim.x) s[j][c] /= piv;
+    __syncthreads();
+    // trailing update of the block (upper): s[r][c] -= s[j][r] s[j][c], j < r <= c
+    const int m = kb - j - 1;
+    for (int idx = t; idx < m * m; idx += blockDim.x) {
+      const int r = j + 1 + idx % m, c = j + 1 + idx / m;
+      if (r <= c) s[r][c] -= s[j][r] * s[j][c];
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    if (r <= c) a[(size_t)(k0 + r) + (size_t)n * (k0 + c)] = s[r][c];
+  }
+}
+
+// row panel U(k0:k0+kb, c) = U_kk^-T S(k0:k0+kb, c) for columns c >= k0+kb
+__global__ __launch_bounds__(256) void k_trsm_row(int n, int k0, int kb, double *__restrict__ a)
+{
+  __shared__ double u[NB][NB + 1];
+  const int t = threadIdx.x;
+  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    u[r][c] = (r <= c) ? a[(size_t)(k0 + r) + (size_t)n * (k0 + c)] : 0.0;
+  }
+  __syncthreads();
+  const int c = k0 + kb + blockIdx.x * blockDim.x + t;
+  if (c >= n) return;
+  double x[NB];
+  double *col = a + (size_t)n * c + k0;
+  for (int r = 0; r < kb; r++) x[r] = col[r];
+  // solve U^T y = x (U^T lower): forward substitution
+  for (int r = 0; r < kb; r++) {
+    double s = x[r];
+    for (int q = 0; q < r; q++) s -= u[q][r] * x[q];
+    x[r] = s / u[r][r];
+  }
+  for (int r = 0; r < kb; r++) col[r] = x[r];
+}
+
+// inverse of the kb x kb upper block of U at (j0,j0) into W(j0,j0)
+__global__ __launch_bounds__(256) void k_trtri_diag(int n, int j0, int kb, const double *__restrict__ u,
+                                                    double *__restrict__ w)
+{
+  __shared__ double s[NB][NB + 1];
+  __shared__ double r_[NB][NB + 1];
+  const int t = threadIdx.x;
+  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    s[r][c] = (r <= c) ? u[(size_t)(j0 + r) + (size_t)n * (j0 + c)] : 0.0;
+    r_[r][c] = 0.0;
+  }
+  __syncthreads();
+  // column c of the inverse: solve U x = e_c by back substitution (thread per column)
+  if (t < kb) {
+    const int c = t;
+    for (int r = c; r >= 0; r--) {
+      double s2 = (r == c) ? 1.0 : 0.0;
+      for (int q = r + 1; q <= c; q++) s2 -= s[r][q] * r_[q][c];
+      r_[r][c] = s2 / s[r][r];
+    }
+  }
+  __syncthreads();
+  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    w[(size_t)(j0 + r) + (size_t)n * (j0 + c)] = (r <= c) ? r_[r][c] : 0.0;
+  }
+}
+
+__global__ void k_fill(size_t cnt, double *p, double v)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) p[i] = v;
+}
+
+// C(j,i) = C(i,j), i > j
+__global__ void k_mirror_lower(int n, double *c)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n || i <= j) return;
+  c[(size_t)j + (size_t)n * i] = c[(size_t)i + (size_t)n * j];
+}
+
+// ------------------------------------------------------------------ tridiagonalisation
+// partial products y_part[chunk][r] = sum_{c in chunk} A(r,c) v(c), r,c >= i0
+__global__ __launch_bounds__(SYMV_ROWS) void k_symv_part(int n, int i0, const double *__restrict__ a,
+                                                         const double *__restrict__ v, double *__restrict__ part)
+{
+  const int r = i0 + blockIdx.x * SYMV_ROWS + threadIdx.x;
+  const int c0 = i0 + blockIdx.y * SYMV_COLS;
+  __shared__ double vs[SYMV_COLS];
+  const int cc = c0 + threadIdx.x;
+  vs[threadIdx.x] = (cc < n) ? v[cc] : 0.0;
+  __syncthreads();
+  if (r >= n) return;
+  const int c1 = min(n, c0 + SYMV_COLS);
+  double s = 0.0;
+  const double *ap = a + (size_t)r;
+  for (int c = c0; c < c1; c++) s += ap[(size_t)n * c] * vs[c - c0];
+  part[(size_t)blockIdx.y * n + r] = s;
+}
+
+struct TriArgs {
+  int n, p0;            // matrix order, first column of the panel
+  double *a;            // n x n symmetric (both triangles), reflectors stored below
+  double *vp, *wp;      // n x TNB panel matrices (explicit, zero above)
+  double *d, *e, *tau;  // tridiagonal and reflector scalars
+  const double *part;   // symv partials (nchunk x n)
+  int nchunk;
+};
+
+// finish w for panel column `fin` (if fin >= 0), then update column `col`
+// with the panel so far and generate its reflector (if col >= 0)
+__global__ __launch_bounds__(1024) void k_tri_step(TriArgs g, int fin, int col)
+{
+  __shared__ double red[40];
+  __shared__ double vw[2 * TNB];
+  const int n = g.n, t = threadIdx.x, nt = blockDim.x;
+  if (fin >= 0) {
+    const int ii = fin - g.p0;
+    const int r0 = fin + 1;
+    double *w = g.wp + (size_t)n * ii;
+    const double *v = g.vp + (size_t)n * ii;
+    // y = sum of partials; corrections -W (V^T v) - V (W^T v) over earlier panel columns
+    for (int k = 0; k < ii; k++) {
+      double s1 = 0.0, s2 = 0.0;
+      for (int r = r0 + t; r < n; r += nt) {
+        s1 += g.vp[(size_t)n * k + r] * v[r];
+        s2 += g.wp[(size_t)n * k + r] * v[r];
+      }
+      s1 = block_sum(s1, red);
+      s2 = block_sum(s2, red);
+      if (t == 0) { vw[k] = s1; vw[TNB + k] = s2; }
+    }
+    __syncthreads();
+    const double tau = g.tau[fin];
+    double dot = 0.0;
+    for (int r = r0 + t; r < n; r += nt) {
+      double y = 0.0;
+      for (int c = 0; c < g.nchunk; c++) y += g.part[(size_t)c * n + r];
+      for (int k = 0; k < ii; k++) y -= g.wp[(size_t)n * k + r] * vw[k] + g.vp[(size_t)n * k + r] * vw[TNB + k];
+      y *= tau;
+      w[r] = y;
+      dot += y * v[r];
+    }
+    dot = block_sum(dot, red);
+    const double alpha2 = -0.5 * tau * dot;
+    for (int r = r0 + t; r < n; r += nt) w[r] += alpha2 * v[r];
+    for (int r = t; r < r0; r += nt) w[r] = 0.0;
+    __syncthreads();
+  }
+  if (col < 0) return;
+  const int ii = col - g.p0;
+  double *ac = g.a + (size_t)n * col;
+  // column update with the earlier panel columns: A(col:n, col) -= V W(col,:)^T + W V(col,:)^T
+  if (ii > 0) {
+    if (t < ii) { vw[t] = g.wp[(size_t)n * t + col]; vw[TNB + t] = g.vp[(size_t)n * t + col]; }
+    __syncthreads();
+    for (int r = col + t; r < n; r += nt) {
+      double s = ac[r];
+      for (int k = 0; k < ii; k++) s -= g.vp[(size_t)n * k + r] * vw[k] + g.wp[(size_t)n * k + r] * vw[TNB + k];
+      ac[r] = s;
+    }
+    __syncthreads();
+  }
+  if (t == 0) g.d[col] = ac[col];
+  double *v = g.vp + (size_t)n * ii;
+  if (col >= n - 1) {
+    for (int r = t; r < n; r += nt) v[r] = 0.0;
+    if (t == 0) g.tau[col] = 0.0;
+    return;
+  }
+  // reflector annihilating A(col+2:n, col) (DLARFG)
+  double xs = 0.0;
+  for (int r = col + 2 + t; r < n; r += nt) xs += ac[r] * ac[r];
+  xs = block_sum(xs, red);
+  const double xnorm = sqrt(xs);
+  const double alpha = ac[col + 1];
+  double tau = 0.0, beta = alpha, scal = 0.0;
+  if (xnorm != 0.0) {
+    beta = -copysign(sqrt(alpha * alpha + xnorm * xnorm), alpha);
+    tau = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+  }
+  __syncthreads();
+  for (int r = t; r < n; r += nt) {
+    double vr = 0.0;
+    if (r == col + 1) vr = 1.0;
+    else if (r > col + 1) {
+      vr = (xnorm != 0.0) ? ac[r] * scal : 0.0;
+      ac[r] = vr;  // reflector kept below the subdiagonal for the back-transform
+    }
+    v[r] = vr;
+  }
+  if (t == 0) {
+    g.e[col] = beta;
+    g.tau[col] = tau;
+    ac[col + 1] = beta;
+  }
+}
+
+// ------------------------------------------------------------------ bisection
+struct BisArgs {
+  int n;
+  const double *d, *e;
+  double *e2;          // squared off-diagonals, 0 at split points
+  double el, eu;
+  int nume;
+  // outputs
+  int *cnt;            // [0] = M, [1] = NWL, [2] = number of blocks
+  int *tblk;           // task: block index
+  int *tidx;           // task: index within block (or -1 for a 1x1 block)
+  int *bstart, *bend;  // block ranges
+  double *bgl, *bgu, *batol, *bnrm;  // per block interval, tolerance, one-norm
+  double *pivmin;
+};
+
+__device__ int sturm_count(int n, const double *d, const double *e2, double x, double pivmin)
+{
+  double t = d[0] - x;
+  if (fabs(t) < pivmin) t = -pivmin;
+  int c = (t <= 0.0);
+  for (int j = 1; j < n; j++) {
+    t = d[j] - e2[j - 1] / t - x;
+    if (fabs(t) < pivmin) t = -pivmin;
+    if (t <= 0.0) c++;
+  }
+  return c;
+}
+
+// single thread: split points, blocks, counts, task list (dstebz2.f:301-540)
+__global__ void k_bisect_prep(BisArgs g)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int n = g.n;
+  const double safemn = DBL_MIN, ulp = DBL_EPSILON, fudge = 2.1;
+  double pivmin = 1.0;
+  int nb = 0;
+  int start = 0;
+  for (int j = 1; j < n; j++) {
+    const double t = g.e[j - 1] * g.e[j - 1];
+    if (fabs(g.d[j] * g.d[j - 1]) * ulp * ulp + safemn > t) {
+      g.e2[j - 1] = 0.0;
+      g.bstart[nb] = start;
+      g.bend[nb] = j;
+      nb++;
+      start = j;
+    } else {
+      g.e2[j - 1] = t;
+      pivmin = fmax(pivmin, t);
+    }
+  }
+  g.bstart[nb] = start;
+  g.bend[nb] = n;
+  nb++;
+  pivmin *= safemn;
+  *g.pivmin = pivmin;
+  int m = 0, nwl = 0;
+  for (int b = 0; b < nb; b++) {
+    const int ib = g.bstart[b], ie = g.bend[b], in = ie - ib;
+    double onenrm = fabs(g.d[ib]) + (in > 1 ? fabs(g.e[ib]) : 0.0);
+    for (int i = ib + 1; i < ie; i++)
+      onenrm = fmax(onenrm, fabs(g.d[i]) + fabs(g.e[i - 1]) + (i < ie - 1 ? fabs(g.e[i]) : 0.0));
+    g.bnrm[b] = onenrm;
+    if (in == 1) {
+      if (g.el >= g.d[ib] - pivmin) nwl++;
+      if (g.el < g.d[ib] - pivmin && g.eu >= g.d[ib] - pivmin) {
+        g.tblk[m] = b;
+        g.tidx[m] = -1;
+        m++;
+      }
+      continue;
+    }
+    double gu = g.d[ib], gl = g.d[ib], t1 = 0.0;
+    for (int j = ib; j < ie - 1; j++) {
+      const double t2 = fabs(g.e[j]);
+      gu = fmax(gu, g.d[j] + t1 + t2);
+      gl = fmin(gl, g.d[j] - t1 - t2);
+      t1 = t2;
+    }
+    gu = fmax(gu, g.d[ie - 1] + t1);
+    gl = fmin(gl, g.d[ie - 1] - t1);
+    const double bnorm = fmax(fabs(gl), fabs(gu));
+    gl = gl - fudge * bnorm * ulp * in - fudge * pivmin;
+    gu = gu + fudge * bnorm * ulp * in + fudge * pivmin;
+    g.batol[b] = ulp * fmax(fabs(gl), fabs(gu));
+    if (gu < g.el) {
+      nwl += in;
+      continue;
+    }
+    gl = fmax(gl, g.el);
+    gu = fmin(gu, g.eu);
+    g.bgl[b] = gl;
+    g.bgu[b] = gu;
+    if (gl >= gu) continue;
+    const int cl = sturm_count(in, g.d + ib, g.e2 + ib, gl, pivmin);
+    const int cu = sturm_count(in, g.d + ib, g.e2 + ib, gu, pivmin);
+    nwl += cl;
+    for (int k = cl; k < cu; k++) {
+      if (m < n) {
+        g.tblk[m] = b;
+        g.tidx[m] = k;
+      }
+      m++;
+    }
+  }
+  g.cnt[0] = m < g.nume ? m : g.nume;
+  g.cnt[1] = nwl;
+  g.cnt[2] = nb;
+}
+
+__global__ void k_bisect(BisArgs g, int m, double *__restrict__ w)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const int b = g.tblk[t], k = g.tidx[t];
+  const int ib = g.bstart[b], in = g.bend[b] - ib;
+  if (k < 0) {
+    w[t] = g.d[ib];
+    return;
+  }
+  const double pivmin = *g.pivmin, atoli = g.batol[b], rtoli = 2.0 * DBL_EPSILON;
+  double lo = g.bgl[b], hi = g.bgu[b];
+  for (int it = 0; it < 200; it++) {
+    const double w1 = fabs(hi - lo), w2 = fmax(fabs(hi), fabs(lo));
+    if (w1 < fmax(atoli, fmax(pivmin, rtoli * w2))) break;
+    const double mid = 0.5 * (lo + hi);
+    if (sturm_count(in, g.d + ib, g.e2 + ib, mid, pivmin) > k) hi = mid;
+    else lo = mid;
+  }
+  w[t] = 0.5 * (lo + hi);
+}
+
+// ------------------------------------------------------------------ inverse iteration
+// single thread: DSTEIN shifts (close eigenvalues separated by 10 ulp) and clusters
+__global__ void k_stein_prep(BisArgs g, int m, const double *__restrict__ w, double *__restrict__ xs,
+                             int *__restrict__ cstart)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int prevb = -1, gp = 0;
+  double xjm = 0.0;
+  for (int t = 0; t < m; t++) {
+    const int b = g.tblk[t];
+    double xj = w[t];
+    if (b != prevb) {
+      gp = t;
+    } else {
+      const double pertol = 10.0 * fabs(DBL_EPSILON * xj);
+      if (xj - xjm < pertol) xj = xjm + pertol;
+      if (xj - xjm > 1.0e-3 * g.bnrm[b]) gp = t;
+    }
+    xs[t] = xj;
+    cstart[t] = gp;
+    xjm = xj;
+    prevb = b;
+  }
+}
+
+// tridiagonal LU with partial pivoting (DLAGTF) and solves (DLAGTS), one
+// thread per eigenvalue; factors stored interleaved [row][task]
+__global__ void k_stein(BisArgs g, int m, const double *__restrict__ xs, double *__restrict__ z, int ldz,
+                        double *__restrict__ f, int *__restrict__ piv)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  const int b = g.tblk[t];
+  const int ib = g.bstart[b], ie = g.bend[b], bs = ie - ib;
+  double *zc = z + (size_t)ldz * t;
+  if (bs == 1) {
+    zc[ib] = 1.0;
+    return;
+  }
+  const double lam = xs[t];
+  const double *d = g.d + ib, *e = g.e + ib;
+  const size_t M = (size_t)m;
+  double *u0 = f, *u1 = f + (size_t)g.n * M, *u2 = f + 2 * (size_t)g.n * M, *mul = f + 3 * (size_t)g.n * M;
+  const double onenrm = g.bnrm[b];
+  const double tol = fmax(fabs(lam), onenrm) * DBL_EPSILON;
+  double a0 = d[0] - lam, a1 = e[0];
+  for (int k = 0; k < bs - 1; k++) {
+    const double sub = e[k];
+    const double dn = d[k + 1] - lam, en = (k + 1 < bs - 1) ? e[k + 1] : 0.0;
+    const size_t o = (size_t)k * M + t;
+    if (fabs(a0) >= fabs(sub)) {
+      piv[o] = 0; u0[o] = a0; u1[o] = a1; u2[o] = 0.0;
+      const double mm = (a0 != 0.0) ? sub / a0 : 0.0;
+      mul[o] = mm;
+      a0 = dn - mm * a1;
+      a1 = en;
+    } else {
+      piv[o] = 1; u0[o] = sub; u1[o] = dn; u2[o] = en;
+      const double mm = a0 / sub;
+      mul[o] = mm;
+      const double na0 = a1 - mm * dn, na1 = -mm * en;
+      a0 = na0;
+      a1 = na1;
+    }
+  }
+  {
+    const size_t o = (size_t)(bs - 1) * M + t;
+    u0[o] = a0; u1[o] = 0.0; u2[o] = 0.0;
+  }
+  for (int k = 0; k < bs; k++) {
+    const size_t o = (size_t)k * M + t;
+    if (fabs(u0[o]) < tol) u0[o] = (u0[o] < 0 ? -tol : tol);
+  }
+  // deterministic start vector, then three solves on an interleaved work
+  // vector (plane 4 of the factor storage), copied to the column at the end
+  double *x = f + 4 * (size_t)g.n * M;
+  unsigned long long s = 0x9E3779B97F4A7C15ull ^ ((unsigned long long)(t + 1) * 0xD1B54A32D192ED03ull);
+  for (int k = 0; k < bs; k++) {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    x[(size_t)k * M + t] = ((double)(s >> 11) / 9007199254740992.0) * 2.0 - 1.0;
+  }
+  for (int it = 0; it < 3; it++) {
+    double mx = 0.0;
+    for (int k = 0; k < bs; k++) mx = fmax(mx, fabs(x[(size_t)k * M + t]));
+    const double sc = 1.0 / mx;
+    for (int k = 0; k < bs; k++) x[(size_t)k * M + t] *= sc;
+    for (int k = 0; k < bs - 1; k++) {
+      const size_t o = (size_t)k * M + t;
+      double xk = x[o], xk1 = x[o + M];
+      if (piv[o]) { const double tmp = xk; xk = xk1; xk1 = tmp; }
+      x[o] = xk;
+      x[o + M] = xk1 - mul[o] * xk;
+    }
+    for (int k = bs - 1; k >= 0; k--) {
+      const size_t o = (size_t)k * M + t;
+      double v = x[o];
+      if (k + 1 < bs) v -= u1[o] * x[o + M];
+      if (k + 2 < bs) v -= u2[o] * x[o + 2 * M];
+      x[o] = v / u0[o];
+    }
+  }
+  for (int k = 0; k < bs; k++) zc[ib + k] = x[(size_t)k * M + t];
+}
+
+// orthonormalise columns c0..c0+cw-1 of z in order (MGS), one work-group
+__global__ __launch_bounds__(256) void k_mgs_chunk(int n, double *z, int ldz, int c0, int cw)
+{
+  __shared__ double red[40];
+  for (int j = 0; j < cw; j++) {
+    double *zj = z + (size_t)ldz * (c0 + j);
+    for (int k = 0; k < j; k++) {
+      const double *zk = z + (size_t)ldz * (c0 + k);
+      double s = 0.0;
+      for (int r = threadIdx.x; r < n; r += blockDim.x) s += zk[r] * zj[r];
+      s = block_sum(s, red);
+      for (int r = threadIdx.x; r < n; r += blockDim.x) zj[r] -= s * zk[r];
+      __syncthreads();
+    }
+    double s = 0.0;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) s += zj[r] * zj[r];
+    s = block_sum(s, red);
+    const double sc = 1.0 / sqrt(s);
+    for (int r = threadIdx.x; r < n; r += blockDim.x) zj[r] *= sc;
+    __syncthreads();
+  }
+}
+
+// unit 2-norm, largest |component| positive (first maximum), per column
+__global__ __launch_bounds__(256) void k_normsign(int n, double *z, int ldz)
+{
+  __shared__ double red[40];
+  __shared__ double bestv[256];
+  __shared__ int besti[256];
+  double *zc = z + (size_t)ldz * blockIdx.x;
+  double s = 0.0, bv = -1.0;
+  int bi = 0x7fffffff;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const double v = zc[r];
+    s += v * v;
+    if (fabs(v) > bv) { bv = fabs(v); bi = r; }
+  }
+  bestv[threadIdx.x] = bv;
+  besti[threadIdx.x] = bi;
+  s = block_sum(s, red);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = -1.0;
+    int idx = 0;
+    for (int k = 0; k < (int)blockDim.x; k++)
+      if (bestv[k] > b || (bestv[k] == b && besti[k] < idx)) { b = bestv[k]; idx = besti[k]; }
+    red[34] = zc[idx];
+  }
+  __syncthreads();
+  double sc = 1.0 / sqrt(s);
+  if (red[34] < 0.0) sc = -sc;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) zc[r] *= sc;
+}
+
+// ------------------------------------------------------------------ back-transforms
+// explicit panel of reflectors p0..p0+nb-1 (v(i+1)=1, v(i+2:n) from A) into vp
+__global__ void k_vpanel(int n, int p0, int nb, const double *__restrict__ a, double *__restrict__ vp)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (r >= n || k >= nb) return;
+  const int i = p0 + k;
+  double v = 0.0;
+  if (r == i + 1) v = 1.0;
+  else if (r > i + 1) v = a[(size_t)r + (size_t)n * i];
+  vp[(size_t)r + (size_t)n * k] = v;
+}
+
+// T (nb x nb upper) of the compact WY form I - V T V^T from G = V^T V (DLARFT 'F','C')
+__global__ void k_larft(int nb, const double *__restrict__ g, const double *__restrict__ tau, double *__restrict__ tm)
+{
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < nb; i++) {
+    for (int r = 0; r < nb; r++) if (r > i) tm[r + nb * i] = 0.0;
+    const double ti = tau[i];
+    tm[i + nb * i] = ti;
+    for (int r = 0; r < i; r++) {
+      double s = 0.0;
+      for (int q = r; q < i; q++) s += tm[r + nb * q] * g[q + nb * i];
+      tm[r + nb * i] = -ti * s;
+    }
+  }
+}
+
+// selection sort of the eigenpairs (dsyevx2.f:381-400): permutation only
+__global__ void k_sortperm(int m, double *w, int *perm)
+{
+  if (threadIdx.x != 0) return;
+  for (int j = 0; j < m; j++) perm[j] = j;
+  for (int j = 0; j < m - 1; j++) {
+    int ii = -1;
+    double t = w[j];
+    for (int jj = j + 1; jj < m; jj++)
+      if (w[jj] < t) { ii = jj; t = w[jj]; }
+    if (ii >= 0) {
+      w[ii] = w[j];
+      w[j] = t;
+      const int p = perm[ii];
+      perm[ii] = perm[j];
+      perm[j] = p;
+    }
+  }
+}
+
+__global__ void k_gather_cols(int n, int m, const int *__restrict__ perm, const double *__restrict__ src,
+                              double *__restrict__ dst)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= n || c >= m) return;
+  dst[(size_t)r + (size_t)n * c] = src[(size_t)r + (size_t)n * perm[c]];
+}
+
+// ------------------------------------------------------------------ driver
+
+int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
+{
+  *ne_out = 0;
+  *nbelw_out = 0;
+  if (c.run.is_complex) return W2K_EINVAL;
+  const int n = c.n;
+  hipStream_t s = c.stream;
+  int rc;
+  if (el >= eu) { el = -1.0e70; eu = 1.0e70; }  // seclr4.F:149-152
+  if (nume <= 0) nume = n;
+  const size_t nn = (size_t)n * n;
+  // workspaces: w_a = S/U, w_b = H full then C, w_c = W = U^-1 then X
+  if ((rc = c.w_a.alloc(nn)) || (rc = c.w_b.alloc(nn)) || (rc = c.w_c.alloc(nn))) return rc;
+  double *U = c.w_a.p, *Hf = c.w_b.p, *W = c.w_c.p;
+  DBuf<double> &X = c.w_x;
+  if ((rc = X.alloc(nn))) return rc;
+  // small buffers: info, d, e, e2, tau, panel V/W, partials, task arrays
+  const int nchunk = (n + SYMV_COLS - 1) / SYMV_COLS;
+  const size_t small = (size_t)8 * n + (size_t)2 * n * TNB + (size_t)nchunk * n + 6 * (size_t)n + 64;
+  if ((rc = c.w_small.alloc(small))) return rc;
+  double *d = c.w_small.p, *e = d + n, *e2 = e + n, *tau = e2 + n, *wv = tau + n;
+  double *vp = wv + n, *wp = vp + (size_t)n * TNB, *part = wp + (size_t)n * TNB;
+  double *bgl = part + (size_t)nchunk * n, *bgu = bgl + n, *batol = bgu + n, *bnrm = batol + n;
+  double *xs = bnrm + n, *pivmin = xs + n;
+  if ((rc = c.w_int.alloc((size_t)6 * n + 16))) return rc;
+  int *info = c.w_int.p, *cnt = info + 4, *tblk = cnt + 8, *tidx = tblk + n, *bstart = tidx + n, *bend = bstart + n;
+  int *cstart = bend + n, *perm = cstart + n;
+  W2K_HIP(hipMemsetAsync(c.w_int.p, 0, sizeof(int) * 16, s));
+
+  W2K_HIP(hipEventRecord(c.ev[6], s));
+  // ---- 1. Cholesky
+  {
+    dim3 g2((n + 255) / 256, n);
+    hipLaunchKernelGGL(k_prep, g2, dim3(256), 0, s, n, c.d_hs.p, c.d_hsdiag.p, U, Hf);
+    W2K_STAGE(s, "k_prep");
+    for (int k0 = 0; k0 < n; k0 += NB) {
+      const int kb = std::min(NB, n - k0);
+      hipLaunchKernelGGL(k_potf2, dim3(1), dim3(256), 0, s, n, k0, kb, U, info);
+      W2K_STAGE(s, "k_potf2");
+      const int rest = n - k0 - kb;
+      if (rest > 0) {
+        hipLaunchKernelGGL(k_trsm_row, dim3((rest + 255) / 256), dim3(256), 0, s, n, k0, kb, U);
+        W2K_STAGE(s, "k_trsm_row");
+        double *X0 = U + (size_t)k0 + (size_t)n * (k0 + kb);
+        double *C0 = U + (size_t)(k0 + kb) + (size_t)n * (k0 + kb);
+        if ((rc = dgemm(s, 'T', 'N', rest, rest, kb, -1.0, X0, n, X0, n, 1.0, C0, n, 2))) return rc;
+      }
+    }
+  }
+  W2K_HIP(hipEventRecord(c.ev[7], s));
+  int hinfo[4];
+  W2K_HIP(hipMemcpyAsync(hinfo, info, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+  W2K_HIP(hipStreamSynchronize(s));
+  if (hinfo[0] != 0) return hinfo[0];  // POTRF INFO, seclr4.F:200-205
+  // ---- 2. reduction: W = U^-1, X = H W, C = W^T X (lower), mirrored
+  {
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nn, W, 0.0);
+    W2K_STAGE(s, "k_fill");
+    for (int j0 = 0; j0 < n; j0 += NB) {
+      const int kb = std::min(NB, n - j0);
+      hipLaunchKernelGGL(k_trtri_diag, dim3(1), dim3(256), 0, s, n, j0, kb, U, W);
+      W2K_STAGE(s, "k_trtri_diag");
+      if (j0 > 0) {
+        // T = W(0:j0,0:j0) U(0:j0, j0:j0+kb); W(0:j0, j0:) = -T W_jj
+        double *T = X.p;
+        if ((rc = dgemm(s, 'N', 'N', j0, kb, j0, 1.0, W, n, U + (size_t)n * j0, n, 0.0, T, n, 0, 1, 0))) return rc;
+        if ((rc = dgemm(s, 'N', 'N', j0, kb, kb, -1.0, T, n, W + (size_t)j0 + (size_t)n * j0, n, 0.0,
+                        W + (size_t)n * j0, n, 0, 0, 1)))
+          return rc;
+      }
+    }
+    if ((rc = dgemm(s, 'N', 'N', n, n, n, 1.0, Hf, n, W, n, 0.0, X.p, n, 0, 0, 1))) return rc;
+    if ((rc = dgemm(s, 'T', 'N', n, n, n, 1.0, W, n, X.p, n, 0.0, Hf, n, 1, 2, 0))) return rc;
+    dim3 g2((n + 255) / 256, n);
+    hipLaunchKernelGGL(k_mirror_lower, g2, dim3(256), 0, s, n, Hf);
+    W2K_STAGE(s, "k_mirror_lower");
+  }
+  W2K_HIP(hipEventRecord(c.ev[8], s));
+  // ---- 3. tridiagonalisation of C (in Hf)
+  double *A = Hf;
+  {
+    TriArgs g;
+    g.n = n; g.a = A; g.vp = vp; g.wp = wp; g.d = d; g.e = e; g.tau = tau; g.part = part;
+    for (int p0 = 0; p0 < n; p0 += TNB) {
+      const int nb = std::min(TNB, n - p0);
+      g.p0 = p0;
+      g.nchunk = 0;
+      hipLaunchKernelGGL(k_tri_step, dim3(1), dim3(1024), 0, s, g, -1, p0);
+      W2K_STAGE(s, "k_tri_step");
+      for (int k = 0; k < nb; k++) {
+        const int i = p0 + k;
+        if (i >= n - 1) break;
+        const int r0 = i + 1;
+        const int rows = n - r0;
+        const int nch = (rows + SYMV_COLS - 1) / SYMV_COLS;
+        hipLaunchKernelGGL(k_symv_part, dim3((rows + SYMV_ROWS - 1) / SYMV_ROWS, nch), dim3(SYMV_ROWS), 0, s, n, r0,
+                           A, vp + (size_t)n * k, part);
+        W2K_STAGE(s, "k_symv_part");
+        // partial sums are indexed from chunk 0 at column r0
+        g.nchunk = nch;
+        const int next = (k + 1 < nb) ? i + 1 : -1;
+        hipLaunchKernelGGL(k_tri_step, dim3(1), dim3(1024), 0, s, g, i, next);
+        W2K_STAGE(s, "k_tri_step");
+      }
+      const int p1 = p0 + nb;
+      if (p1 < n) {
+        const int r = n - p1;
+        double *A1 = A + (size_t)p1 + (size_t)n * p1;
+        if ((rc = dgemm(s, 'N', 'T', r, r, nb, -1.0, vp + p1, n, wp + p1, n, 1.0, A1, n))) return rc;
+        if ((rc = dgemm(s, 'N', 'T', r, r, nb, -1.0, wp + p1, n, vp + p1, n, 1.0, A1, n))) return rc;
+      }
+    }
+  }
+  W2K_HIP(hipEventRecord(c.ev[9], s));
+  // ---- 4. bisection
+  BisArgs b;
+  b.n = n; b.d = d; b.e = e; b.e2 = e2; b.el = el; b.eu = eu; b.nume = nume;
+  b.cnt = cnt; b.tblk = tblk; b.tidx = tidx; b.bstart = bstart; b.bend = bend;
+  b.bgl = bgl; b.bgu = bgu; b.batol = batol; b.bnrm = bnrm; b.pivmin = pivmin;
+  hipLaunchKernelGGL(k_bisect_prep, dim3(1), dim3(64), 0, s, b);
+  W2K_STAGE(s, "k_bisect_prep");
+  int hc[4];
+  W2K_HIP(hipMemcpyAsync(hc, cnt, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+  W2K_HIP(hipStreamSynchronize(s));
+  const int m = hc[0];
+  *nbelw_out = hc[1];
+  c.nbelw = hc[1];
+  if ((rc = c.d_eig.alloc(m > 0 ? m : 1)) || (rc = c.d_z.alloc((size_t)n * (m > 0 ? m : 1)))) return rc;
+  if (m == 0) {
+    c.ne = 0;
+    W2K_HIP(hipEventRecord(c.ev[10], s));
+    W2K_HIP(hipEventRecord(c.ev[11], s));
+    return 0;
+  }
+  double *w = wv;
+  hipLaunchKernelGGL(k_bisect, dim3((m + 63) / 64), dim3(64), 0, s, b, m, w);
+  W2K_STAGE(s, "k_bisect");
+  // ---- 5. inverse iteration + cluster orthogonalisation
+  hipLaunchKernelGGL(k_stein_prep, dim3(1), dim3(64), 0, s, b, m, w, xs, cstart);
+  W2K_STAGE(s, "k_stein_prep");
+  double *Zt = X.p;  // n x m
+  {
+    const size_t zc = (size_t)n * m;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((zc + 255) / 256)), dim3(256), 0, s, zc, Zt, 0.0);
+    W2K_STAGE(s, "k_fill");
+    if ((rc = c.w_fac.alloc((size_t)5 * n * m)) || (rc = c.w_piv.alloc((size_t)n * m))) return rc;
+    hipLaunchKernelGGL(k_stein, dim3((m + 63) / 64), dim3(64), 0, s, b, m, xs, Zt, n, c.w_fac.p, c.w_piv.p);
+    W2K_STAGE(s, "k_stein");
+    std::vector<int> cs(m);
+    W2K_HIP(hipMemcpyAsync(cs.data(), cstart, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+    W2K_HIP(hipStreamSynchronize(s));
+    // clusters [t0, t1)
+    if ((rc = c.w_bt.alloc((size_t)(n + 2 * m + 2 * TNB) * TNB))) return rc;
+    double *ybuf = c.w_bt.p;
+    int t0 = 0;
+    while (t0 < m) {
+      int t1 = t0 + 1;
+      while (t1 < m && cs[t1] == t0) t1++;
+      for (int c0 = t0; c0 < t1; c0 += 32) {
+        const int cw = std::min(32, t1 - c0);
+        for (int pass = 0; pass < 2 && c0 > t0; pass++) {
+          const int np = c0 - t0;
+          double *Q = Zt + (size_t)n * t0, *Zc = Zt + (size_t)n * c0;
+          if ((rc = dgemm(s, 'T', 'N', np, cw, n, 1.0, Q, n, Zc, n, 0.0, ybuf, np))) return rc;
+          if ((rc = dgemm(s, 'N', 'N', n, cw, np, -1.0, Q, n, ybuf, np, 1.0, Zc, n))) return rc;
+        }
+        if (t1 - t0 > 1) {
+          hipLaunchKernelGGL(k_mgs_chunk, dim3(1), dim3(256), 0, s, n, Zt, n, c0, cw);
+          W2K_STAGE(s, "k_mgs_chunk");
+        }
+      }
+      t0 = t1;
+    }
+    hipLaunchKernelGGL(k_normsign, dim3(m), dim3(256), 0, s, n, Zt, n);
+    W2K_STAGE(s, "k_normsign");
+    W2K_HIP(hipStreamSynchronize(s));
+  }
+  W2K_HIP(hipEventRecord(c.ev[10], s));
+  // ---- 6. Z = Q Zt (reflectors of step 3, last panel first), sort, Z = W Z
+  {
+    // w_bt: V panel (n x TNB), V^T V and T (TNB x TNB each), Y1 and Y2 (TNB x m each)
+    double *vb = c.w_bt.p, *gm = vb + (size_t)n * TNB, *tm = gm + TNB * TNB, *y1 = tm + TNB * TNB;
+    double *y2 = y1 + (size_t)TNB * m;
+    const int nref = n - 1;  // reflectors 0..n-2
+    const int last = ((nref - 1) / TNB) * TNB;
+    for (int p0 = last; p0 >= 0; p0 -= TNB) {
+      const int nb = std::min(TNB, nref - p0);
+      if (nb <= 0) continue;
+      hipLaunchKernelGGL(k_vpanel, dim3((n + 255) / 256, nb), dim3(256), 0, s, n, p0, nb, A, vb);
+      W2K_STAGE(s, "k_vpanel");
+      if ((rc = dgemm(s, 'T', 'N', nb, nb, n, 1.0, vb, n, vb, n, 0.0, gm, nb))) return rc;
+      hipLaunchKernelGGL(k_larft, dim3(1), dim3(64), 0, s, nb, gm, tau + p0, tm);
+      W2K_STAGE(s, "k_larft");
+      if ((rc = dgemm(s, 'T', 'N', nb, m, n, 1.0, vb, n, Zt, n, 0.0, y1, nb))) return rc;
+      if ((rc = dgemm(s, 'N', 'N', nb, m, nb, 1.0, tm, nb, y1, nb, 0.0, y2, nb, 0, 1, 0))) return rc;
+      if ((rc = dgemm(s, 'N', 'N', n, m, nb, -1.0, vb, n, y2, nb, 1.0, Zt, n))) return rc;
+    }
+    hipLaunchKernelGGL(k_sortperm, dim3(1), dim3(64), 0, s, m, w, perm);
+    W2K_STAGE(s, "k_sortperm");
+    double *Zs = U;  // U is not needed after the inversion
+    hipLaunchKernelGGL(k_gather_cols, dim3((n + 255) / 256, m), dim3(256), 0, s, n, m, perm, Zt, Zs);
+    W2K_STAGE(s, "k_gather_cols");
+    if ((rc = dgemm(s, 'N', 'N', n, m, n, 1.0, W, n, Zs, n, 0.0, c.d_z.p, n, 0, 1, 0))) return rc;
+    W2K_HIP(hipMemcpyAsync(c.d_eig.p, w, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+    W2K_HIP(hipStreamSynchronize(s));
+  }
+  W2K_HIP(hipEventRecord(c.ev[11], s));
+  W2K_HIP(hipStreamSynchronize(s));
+  take_ms(c, 3, 6, 7);
+  take_ms(c, 4, 7, 8);
+  take_ms(c, 5, 8, 9);
+  take_ms(c, 6, 9, 10);
+  take_ms(c, 7, 10, 11);
+  c.ne = m;
+  *ne_out = m;
+  return 0;
+}
+
 }  // namespace w2k

@@ -10,10 +10,34 @@
 
 namespace w2k {
 
+static thread_local char g_last_error[512];
+
 int w2k_hip_fail(hipError_t e, const char *what, const char *file, int line)
 {
-  if (getenv("W2K_VERBOSE")) fprintf(stderr, "w2k: %s failed (%s) at %s:%d\n", what, hipGetErrorString(e), file, line);
+  const char *base = strrchr(file, '/');
+  snprintf(g_last_error, sizeof g_last_error, "%s failed (%s) at %s:%d", what, hipGetErrorString(e),
+           base ? base + 1 : file, line);
+  if (getenv("W2K_VERBOSE")) fprintf(stderr, "w2k: %s\n", g_last_error);
   return e == hipErrorOutOfMemory ? W2K_ENOMEM : W2K_EDEVICE;
+}
+
+// W2K_SYNC_CHECK=1: synchronise after every kernel so a device fault is
+// attributed to the kernel that caused it (slow; for diagnosis only)
+static int sync_checks()
+{
+  static const int on = getenv("W2K_SYNC_CHECK") != nullptr && atoi(getenv("W2K_SYNC_CHECK")) != 0;
+  return on;
+}
+
+int stage_check(hipStream_t s, const char *name, const char *file, int line)
+{
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return w2k_hip_fail(e, name, file, line);
+  if (sync_checks()) {
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return w2k_hip_fail(e, name, file, line);
+  }
+  return 0;
 }
 
 template <class T>
@@ -303,6 +327,18 @@ __global__ void k_diag_copy(int n, int cpx, int dir, double *hs, double *buf)
   }
 }
 
+// elapsed time between two recorded events into last_ms[slot]; a failed
+// query leaves the slot at 0 and does not leave an error behind
+void take_ms(Ctx &c, int slot, int e0, int e1)
+{
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c.ev[e0], c.ev[e1]) != hipSuccess) {
+    (void)hipGetLastError();
+    ms = 0.f;
+  }
+  c.last_ms[slot] = ms;
+}
+
 static int timed_begin(Ctx &c, int slot) { W2K_HIP(hipEventRecord(c.ev[2 * slot], c.stream)); return 0; }
 static int timed_end(Ctx &c, int slot)
 {
@@ -317,6 +353,8 @@ using namespace w2k;
 extern "C" {
 
 const char *w2k_version(void) { return "w2k-mi355x 0.1 (gfx950)"; }
+
+const char *w2k_last_error(void) { return g_last_error; }
 
 int w2k_create(w2k_ctx **out, int device, const w2k_run_desc *run)
 {
@@ -333,7 +371,7 @@ int w2k_create(w2k_ctx **out, int device, const w2k_run_desc *run)
     delete x;
     return W2K_EDEVICE;
   }
-  for (int k = 0; k < 16; k++) (void)hipEventCreate(&c.ev[k]);
+  for (int k = 0; k < W2K_NEVENTS; k++) (void)hipEventCreate(&c.ev[k]);
   int rc = run_upload(c, run);
   if (rc) {
     w2k_destroy(x);
@@ -356,7 +394,7 @@ void w2k_destroy(w2k_ctx *x)
   if (!x) return;
   (void)hipSetDevice(x->c.device);
   if (x->c.stream) (void)hipStreamSynchronize(x->c.stream);
-  for (int k = 0; k < 16; k++) (void)hipEventDestroy(x->c.ev[k]);
+  for (int k = 0; k < W2K_NEVENTS; k++) (void)hipEventDestroy(x->c.ev[k]);
   if (x->c.stream) (void)hipStreamDestroy(x->c.stream);
   x->c.stream = nullptr;
   delete x;
@@ -406,9 +444,8 @@ int w2k_hamilt(w2k_ctx *x)
   if (!rc) rc = timed_end(c, 0);
   if (!rc) {
     W2K_HIP(hipStreamSynchronize(c.stream));
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, c.ev[0], c.ev[1]);
-    c.last_ms[0] = ms;
+    take_ms(c, 0, 0, 1);
+    take_ms(c, 8, 12, 13);
   }
   return rc;
 }
@@ -421,7 +458,7 @@ int w2k_swap_diag(w2k_ctx *x)
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
   hipLaunchKernelGGL(k_swap_diag, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run.is_complex, c.d_hs.p,
                      c.d_hsdiag.p);
-  W2K_HIP(hipGetLastError());
+  W2K_STAGE(c.stream, "k_swap_diag");
   return 0;
 }
 
@@ -436,9 +473,8 @@ int w2k_hns(w2k_ctx *x)
   if (!rc) rc = timed_end(c, 1);
   if (!rc) {
     W2K_HIP(hipStreamSynchronize(c.stream));
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, c.ev[2], c.ev[3]);
-    c.last_ms[1] = ms;
+    take_ms(c, 1, 2, 3);
+    take_ms(c, 9, 14, 15);
   }
   return rc;
 }
@@ -454,9 +490,7 @@ int w2k_seclr4(w2k_ctx *x, double el, double eu, int nume, int *ne, int *nbelw)
   int rc2 = timed_end(c, 2);
   if (!rc) rc = rc2;
   W2K_HIP(hipStreamSynchronize(c.stream));
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, c.ev[4], c.ev[5]);
-  c.last_ms[2] = ms;
+  take_ms(c, 2, 4, 5);
   return rc;
 }
 
@@ -513,6 +547,7 @@ int w2k_get_diag(w2k_ctx *x, double *hs_diag, double *hsdiag)
   if (hs_diag) {
     hipLaunchKernelGGL(k_diag_copy, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run.is_complex, 0,
                        c.d_hs.p, c.w_v.p);
+    W2K_STAGE(c.stream, "k_diag_copy");
     W2K_HIP(hipMemcpyAsync(hs_diag, c.w_v.p, sizeof(double) * w * c.n, hipMemcpyDeviceToHost, c.stream));
   }
   if (hsdiag) W2K_HIP(hipMemcpyAsync(hsdiag, c.d_hsdiag.p, sizeof(double) * w * c.n, hipMemcpyDeviceToHost, c.stream));
@@ -533,6 +568,7 @@ int w2k_put_diag(w2k_ctx *x, const double *hs_diag, const double *hsdiag)
     W2K_HIP(hipMemcpyAsync(c.w_v.p, hs_diag, sizeof(double) * w * c.n, hipMemcpyHostToDevice, c.stream));
     hipLaunchKernelGGL(k_diag_copy, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run.is_complex, 1,
                        c.d_hs.p, c.w_v.p);
+    W2K_STAGE(c.stream, "k_diag_copy");
   }
   if (hsdiag) W2K_HIP(hipMemcpyAsync(c.d_hsdiag.p, hsdiag, sizeof(double) * w * c.n, hipMemcpyHostToDevice, c.stream));
   W2K_HIP(hipStreamSynchronize(c.stream));
@@ -592,7 +628,7 @@ int w2k_sync(w2k_ctx *x)
 int w2k_last_timings(w2k_ctx *x, double *ms, int n)
 {
   if (!x || !ms) return W2K_EINVAL;
-  for (int k = 0; k < n && k < 8; k++) ms[k] = x->c.last_ms[k];
+  for (int k = 0; k < n && k < W2K_NTIMINGS; k++) ms[k] = x->c.last_ms[k];
   return 0;
 }
 

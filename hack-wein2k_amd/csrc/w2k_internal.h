@@ -11,6 +11,8 @@
 
 #define W2K_LOMAXP (W2K_LOMAX + 1)
 #define W2K_NSL2 (W2K_NSLMAX * W2K_NSLMAX)
+#define W2K_NEVENTS 16
+#define W2K_NTIMINGS 10
 
 namespace w2k {
 
@@ -26,6 +28,15 @@ struct Status {
   } while (0)
 
 int w2k_hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+// launch check after kernel `name` (and a stream synchronisation when the
+// environment sets W2K_SYNC_CHECK=1)
+int stage_check(hipStream_t s, const char *name, const char *file, int line);
+#define W2K_STAGE(s, name)                                       \
+  do {                                                           \
+    int rc_ = stage_check((s), (name), __FILE__, __LINE__);      \
+    if (rc_) return rc_;                                         \
+  } while (0)
 
 // ---------------------------------------------------------------- device buffers
 template <class T>
@@ -95,16 +106,22 @@ struct Ctx {
   DBuf<double> d_colcoef;   // HAMILT per-column MT coefficients
   // HNS work
   DBuf<double> d_hnsL, d_hnsR;  // rank-update operands
+  DBuf<double> h_small;         // LO-block operands
+  DBuf<int> h_int;              // per-atom column offsets
+  DBuf<double> h_re, h_im;      // complex build: planar products
   // eigen results
   int ne = 0, nbelw = 0;
   DBuf<double> d_eig, d_z;
   // SECLR4 work (grown on demand)
-  DBuf<double> w_a, w_b, w_c, w_v;
+  DBuf<double> w_a, w_b, w_c, w_v, w_x;
   DBuf<double> w_small;
   DBuf<int> w_int;
+  DBuf<double> w_fac;   // inverse-iteration factors, [plane][row][eigenvalue]
+  DBuf<int> w_piv;      // inverse-iteration pivots
+  DBuf<double> w_bt;    // back-transform panel work
   // timing
-  hipEvent_t ev[16];
-  double last_ms[8] = {0};
+  hipEvent_t ev[W2K_NEVENTS];
+  double last_ms[W2K_NTIMINGS] = {0};
 };
 
 // kernels / stages (implemented in the .hip files)
@@ -112,6 +129,8 @@ int run_upload(Ctx &c, const w2k_run_desc *r);
 int hamilt(Ctx &c);
 int hns(Ctx &c);
 int seclr4(Ctx &c, double el, double eu, int nume, int *ne, int *nbelw);
+// elapsed milliseconds between events e0 and e1 into last_ms[slot]
+void take_ms(Ctx &c, int slot, int e0, int e1);
 
 // generic FP64 GEMM on the device: C = alpha * op(A) * op(B) + beta * C
 // tri: 0 full, 1 lower (i >= j) only, 2 upper (i <= j) only (tiles outside skipped)

@@ -159,9 +159,13 @@ class Basis:
 
 
 class Lapw1Error(RuntimeError):
-    def __init__(self, where: str, code: int):
-        super().__init__(f"{where} failed with status {code}")
+    def __init__(self, where: str, code: int, detail: str = ""):
+        msg = f"{where} failed with status {code}"
+        if detail:
+            msg += f" ({detail})"
+        super().__init__(msg)
         self.code = code
+        self.detail = detail
 
 
 _LIB = None
@@ -194,6 +198,7 @@ def load_library(path: Optional[str] = None):
     lib.w2k_stream.restype = V
     lib.w2k_last_timings.argtypes = [V, _c_dbl_p, ctypes.c_int]
     lib.w2k_version.restype = ctypes.c_char_p
+    lib.w2k_last_error.restype = ctypes.c_char_p
     if path is None:
         _LIB = lib
     return lib
@@ -229,7 +234,8 @@ class Lapw1:
 
     def _chk(self, where, rc):
         if rc != 0:
-            raise Lapw1Error(where, rc)
+            detail = self.lib.w2k_last_error().decode() if rc < 0 else ""
+            raise Lapw1Error(where, rc, detail)
 
     @property
     def cdt(self):
@@ -291,9 +297,13 @@ class Lapw1:
         self._chk("w2k_get_eig", self.lib.w2k_get_eig(self.ctx, _dp(ev), _dp(z), n))
         return ev[:ne], z[:, :ne]
 
+    TIMING_NAMES = ("hamilt", "hns", "seclr4", "cholesky", "reduction", "tridiag", "bisect_invit",
+                    "backtransform", "hamilt_pw_kernel", "hns_gemm")
+
     def timings(self) -> np.ndarray:
-        t = np.zeros(8, np.float64)
-        self.lib.w2k_last_timings(self.ctx, _dp(t), 8)
+        """Device milliseconds of the last calls (order of TIMING_NAMES)."""
+        t = np.zeros(len(self.TIMING_NAMES), np.float64)
+        self.lib.w2k_last_timings(self.ctx, _dp(t), len(t))
         return t
 
     def sync(self):

@@ -150,11 +150,19 @@ int w2k_sync(w2k_ctx *ctx);
 /* Kernel timing of the most recent calls, in milliseconds, measured with HIP
  * events on the context stream: [0] HAMILT, [1] HNS, [2] SECLR4 total,
  * [3] Cholesky, [4] reduction to standard form, [5] tridiagonalisation,
- * [6] bisection + inverse iteration, [7] back-transforms. */
+ * [6] bisection + inverse iteration, [7] back-transforms, [8] the HAMILT
+ * plane-wave pair kernel alone, [9] the HNS rank-K GEMM alone.  At most
+ * n values are written. */
 int w2k_last_timings(w2k_ctx *ctx, double *ms, int n);
 
 /* Library version string. */
 const char *w2k_version(void);
+
+/* Text of the last failed device call on this thread: the call or kernel,
+ * the runtime's error string and the source position (empty when none).
+ * Setting W2K_SYNC_CHECK=1 in the environment synchronises after every kernel
+ * so that a device fault is attributed to the kernel that caused it. */
+const char *w2k_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -15,18 +15,9 @@ GOLDEN = os.path.join(REPO, "tests", "golden")
 sys.path.insert(0, os.path.join(REPO, "tests"))
 
 
-def _load_pkg():
-    if "lapw1_pkg" in sys.modules:
-        return sys.modules["lapw1_pkg"]
-    spec = importlib.util.spec_from_file_location("lapw1_pkg", os.path.join(PKG_DIR, "__init__.py"),
-                                                  submodule_search_locations=[PKG_DIR])
-    mod = importlib.util.module_from_spec(spec)
-    sys.modules["lapw1_pkg"] = mod
-    spec.loader.exec_module(mod)
-    return mod
+from oracle_lib import load_package  # noqa: E402
 
-
-PKG = _load_pkg()
+PKG = load_package()
 
 
 def pytest_configure(config):
@@ -45,7 +36,7 @@ class Fixture:
         self.run_arrays = {k[4:]: v for k, v in self.raw.items() if k.startswith("run/")}
 
     def run(self):
-        from lapw1_pkg.lapw1 import RunData
+        RunData = PKG.lapw1.RunData
         return RunData(self.run_arrays, self.is_complex)
 
     def k(self, i: int) -> dict:
@@ -53,7 +44,7 @@ class Fixture:
         return {k[len(p):]: v for k, v in self.raw.items() if k.startswith(p)}
 
     def basis(self, i: int):
-        from lapw1_pkg.lapw1 import Basis
+        Basis = PKG.lapw1.Basis
         kd = self.k(i)
         return Basis(int(kd["NV"]), int(kd["NLO"]), kd["KZZ"], kd["XK"], kd["YK"], kd["ZK"], kd["RK"])
 

@@ -6,7 +6,9 @@ itself pinned against the reference golden vectors (tests/golden/*.npz).
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
+import sys
 from typing import Tuple
 
 import numpy as np
@@ -19,10 +21,27 @@ _c_dbl_p = ctypes.POINTER(ctypes.c_double)
 _lib = None
 
 
+PKG_NAME = "hack_wein2k_amd"
+
+
+def load_package():
+    """The package directory hack-wein2k_amd/ imported as `hack_wein2k_amd`
+    (one module object per process, shared by tests, bench and smoke)."""
+    if PKG_NAME in sys.modules:
+        return sys.modules[PKG_NAME]
+    pkg_dir = os.path.join(REPO, "hack-wein2k_amd")
+    spec = importlib.util.spec_from_file_location(PKG_NAME, os.path.join(pkg_dir, "__init__.py"),
+                                                  submodule_search_locations=[pkg_dir])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[PKG_NAME] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
 def lib():
     global _lib
     if _lib is None:
-        from lapw1_pkg import lapw1 as L
+        L = load_package().lapw1
         l = ctypes.CDLL(ORACLE_SO)
         RD = ctypes.POINTER(L.RunDesc)
         l.oracle_sphbes.argtypes = [ctypes.c_int, ctypes.c_double, _c_dbl_p]
