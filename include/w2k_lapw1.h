@@ -46,7 +46,7 @@ extern "C" {
 /*
  * Per-run (k-independent) data: everything HAMILT and HNS read from their
  * argument lists and from the modules albl/loabc/lolog/atspdt/loint/rotmat/
- * struk/out (SURVEY boundary table; hamilt.F:5-27, hns.F:5-17).  All arrays
+ * struk/out (hamilt.F:5-27, hns.F:5-17).  All arrays
  * keep the reference's Fortran layouts and leading dimensions.  The pointers
  * are read during w2k_create / w2k_set_run only (the data is copied).
  */

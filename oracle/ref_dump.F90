@@ -2,7 +2,8 @@
 ! oracle/_ref/ and is linked with the reference objects compiled there).
 !
 ! Runs the program sequence of lapw1.F:33-84 with CALKPT (calkpt.F:143-197)
-! unrolled, so that the module state can be written out
+! unrolled, so that the module state can be written out (argument 'nodump':
+! timings only; 'eigonly': per k-point NE, NBELW and EIGVAL only)
 !   - once per run after INILPW/INIKPT            -> run.bin
 !   - per k-point at CALKPT entry (basis), after HAMILT, after HNS and after
 !     SECLR4                                       -> k<NNNN>.bin
@@ -86,7 +87,7 @@ program ref_dump
   use out, only : WARP, WARP1, WARP2, WARP3
   implicit none
   include 'param.inc'
-  logical :: nmat_only, dodump
+  logical :: nmat_only, dodump, eigdump
   integer :: numwk, info, kpoint, u, nn, i, j
   integer, allocatable :: ltmp(:,:)
   character(len=80) :: deffn, fname
@@ -97,9 +98,14 @@ program ref_dump
   cut = .false.
   deffn = 'lapw1.def'
   dodump = .true.
+  eigdump = .false.
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg)
     if (arg(1:6) == 'nodump') dodump = .false.
+    if (arg(1:7) == 'eigonly') then
+      dodump = .false.
+      eigdump = .true.
+    end if
   end if
   call init_parallel
   call init_all_timer
@@ -239,6 +245,17 @@ program ref_dump
 !_REAL    call wd(u, 'Z', Z, [nn, max(NE,1)])
 !_COMPLEX    call wz(u, 'Z', Z, [nn, max(NE,1)])
     close(u)
+    end if
+    if (eigdump) then
+      write(fname, '(a,i4.4,a)') 'k', kpoint, '.bin'
+      open(u, file=fname, access='stream', form='unformatted', status='replace')
+      call ws(u, 'NV', NV)
+      call ws(u, 'NLO', NLO)
+      call ws(u, 'HSROWS', HSROWS)
+      call ws(u, 'NE', NE)
+      call ws(u, 'NBELW', NBELW)
+      call wd(u, 'EIGVAL', EIGVAL, [max(NE,1)])
+      close(u)
     end if
     call prtkpt(kpoint, info)
     if (info /= 0) stop 'ref_dump: PRTKPT failed'
