@@ -12,6 +12,8 @@
 // k-major so that the MFMA operand reads (lane l -> row l%16, k l/16) are
 // bank-conflict free; LDS double-buffered so the next k-slab's global loads
 // overlap the current slab's MFMAs.
+#include <algorithm>
+
 #include "w2k_internal.h"
 
 namespace w2k {
@@ -23,6 +25,8 @@ constexpr int PAD = 2;  // LDS row padding (doubles)
 
 struct GemmArgs {
   int m, n, k;
+  int kslice;        // split-K: K range of one z-slice (0: no split)
+  double *ws;        // split-K: m x n partial products per slice
   double alpha, beta;
   const double *A;
   int lda;
@@ -51,6 +55,7 @@ __device__ __forceinline__ double load_opB(const GemmArgs &g, int kk, int j)
   return g.tb ? g.B[(size_t)j + (size_t)g.ldb * kk] : g.B[(size_t)kk + (size_t)g.ldb * j];
 }
 
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_dgemm(GemmArgs g)
 {
   __shared__ double As[2][BK][BM + PAD];
@@ -66,6 +71,10 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmArgs g)
   if (g.atri == 2) kend = min(kend, i0 + BM);
   if (g.btri == 1) kend = min(kend, j0 + BN);
   if (g.btri == 2) kbeg = max(kbeg, j0);
+  if (SPLIT) {
+    kbeg = max(kbeg, (int)blockIdx.z * g.kslice);
+    kend = min(kend, ((int)blockIdx.z + 1) * g.kslice);
+  }
   kbeg = (kbeg / BK) * BK;
 
   const int lane = tid & 63, wave = tid >> 6;
@@ -76,18 +85,21 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmArgs g)
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
+  // with a K split, elements past this slice's end read as zero
+  GemmArgs gl = g;
+  if (SPLIT) gl.k = min(g.k, kend);
   // register staging for the global loads of one slab: 4 A + 4 B per thread
   double ra[4], rb[4];
   auto gload = [&](int k0) {
     if (!g.ta) {  // 64 rows contiguous per k column
-      for (int r = 0; r < 4; r++) ra[r] = load_opA(g, i0 + (tid & 63), k0 + (tid >> 6) + 4 * r);
+      for (int r = 0; r < 4; r++) ra[r] = load_opA(gl, i0 + (tid & 63), k0 + (tid >> 6) + 4 * r);
     } else {      // 16 k contiguous per row
-      for (int r = 0; r < 4; r++) ra[r] = load_opA(g, i0 + (tid >> 4) + 16 * r, k0 + (tid & 15));
+      for (int r = 0; r < 4; r++) ra[r] = load_opA(gl, i0 + (tid >> 4) + 16 * r, k0 + (tid & 15));
     }
     if (g.tb) {   // 64 columns contiguous per k row
-      for (int r = 0; r < 4; r++) rb[r] = load_opB(g, k0 + (tid >> 6) + 4 * r, j0 + (tid & 63));
+      for (int r = 0; r < 4; r++) rb[r] = load_opB(gl, k0 + (tid >> 6) + 4 * r, j0 + (tid & 63));
     } else {      // 16 k contiguous per column
-      for (int r = 0; r < 4; r++) rb[r] = load_opB(g, k0 + (tid & 15), j0 + (tid >> 4) + 16 * r);
+      for (int r = 0; r < 4; r++) rb[r] = load_opB(gl, k0 + (tid & 15), j0 + (tid >> 4) + 16 * r);
     }
   };
   auto sstore = [&](int buf) {
@@ -137,6 +149,10 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmArgs g)
         if (i >= g.m || j >= g.n) continue;
         if (g.tri == 1 && i < j) continue;
         if (g.tri == 2 && i > j) continue;
+        if (SPLIT) {
+          g.ws[((size_t)blockIdx.z * g.n + j) * g.m + i] = acc[a][b][r];
+          continue;
+        }
         double *cp = g.C + (size_t)i + (size_t)g.ldc * j;
         double v = g.alpha * acc[a][b][r];
         if (g.beta != 0.0) v += g.beta * *cp;
@@ -146,8 +162,24 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmArgs g)
   }
 }
 
+// C = alpha * sum_z ws[z] + beta * C, fixed summation order
+__global__ void k_splitk_reduce(int m, int n, int nz, const double *__restrict__ ws, double alpha, double beta,
+                                double *__restrict__ C, int ldc)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= m || j >= n) return;
+  double v = 0.0;
+  for (int z = 0; z < nz; z++) v += ws[((size_t)z * n + j) * m + i];
+  double *cp = C + (size_t)i + (size_t)ldc * j;
+  v *= alpha;
+  if (beta != 0.0) v += beta * *cp;
+  *cp = v;
+}
+
 int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
-          const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri)
+          const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri, double *ws,
+          size_t wsn)
 {
   if (m <= 0 || n <= 0) return 0;
   GemmArgs g;
@@ -155,11 +187,31 @@ int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
   g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
   g.ta = (ta == 'T' || ta == 't'); g.tb = (tb == 'T' || tb == 't');
   g.tri = tri; g.atri = atri; g.btri = btri;
+  g.kslice = 0; g.ws = nullptr;
   dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
   if (k <= 0) {
     g.k = 0;  // still applies beta
   }
-  hipLaunchKernelGGL(k_dgemm, grid, dim3(256), 0, s, g);
+  // few output tiles and a long K: split K over z-slices so that the whole
+  // chip works, then reduce the slices in a fixed order
+  const long tiles = (long)grid.x * grid.y;
+  if (ws && tri == 0 && atri == 0 && btri == 0 && tiles < 256 && k >= 4 * 256) {
+    int nz = (int)std::min<long>((k + 255) / 256, std::max<long>(1, 1024 / tiles));
+    nz = (int)std::min<size_t>((size_t)nz, wsn / ((size_t)m * n));
+    if (nz >= 2) {
+      g.kslice = ((k + nz - 1) / nz + BK - 1) / BK * BK;
+      nz = (k + g.kslice - 1) / g.kslice;
+      g.ws = ws;
+      grid.z = nz;
+      hipLaunchKernelGGL(k_dgemm<true>, grid, dim3(256), 0, s, g);
+      W2K_STAGE(s, "k_dgemm<split>");
+      hipLaunchKernelGGL(k_splitk_reduce, dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, nz, ws, alpha, beta, C,
+                         ldc);
+      W2K_STAGE(s, "k_splitk_reduce");
+      return 0;
+    }
+  }
+  hipLaunchKernelGGL(k_dgemm<false>, grid, dim3(256), 0, s, g);
   W2K_STAGE(s, "k_dgemm");
   return 0;
 }

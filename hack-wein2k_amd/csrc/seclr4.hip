@@ -28,6 +28,8 @@
 // W2K_EINVAL here; HAMILT/HNS support it).
 #include <cfloat>
 #include <cmath>
+#include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "w2k_internal.h"
@@ -186,98 +188,187 @@ __global__ void k_mirror_lower(int n, double *c)
 }
 
 // ------------------------------------------------------------------ tridiagonalisation
-// partial products y_part[chunk][r] = sum_{c in chunk} A(r,c) v(c), r,c >= i0
-__global__ __launch_bounds__(SYMV_ROWS) void k_symv_part(int n, int i0, const double *__restrict__ a,
-                                                         const double *__restrict__ v, double *__restrict__ part)
-{
-  const int r = i0 + blockIdx.x * SYMV_ROWS + threadIdx.x;
-  const int c0 = i0 + blockIdx.y * SYMV_COLS;
-  __shared__ double vs[SYMV_COLS];
-  const int cc = c0 + threadIdx.x;
-  vs[threadIdx.x] = (cc < n) ? v[cc] : 0.0;
-  __syncthreads();
-  if (r >= n) return;
-  const int c1 = min(n, c0 + SYMV_COLS);
-  double s = 0.0;
-  const double *ap = a + (size_t)r;
-  for (int c = c0; c < c1; c++) s += ap[(size_t)n * c] * vs[c - c0];
-  part[(size_t)blockIdx.y * n + r] = s;
-}
-
-struct TriArgs {
-  int n, p0;            // matrix order, first column of the panel
-  double *a;            // n x n symmetric (both triangles), reflectors stored below
-  double *vp, *wp;      // n x TNB panel matrices (explicit, zero above)
-  double *d, *e, *tau;  // tridiagonal and reflector scalars
-  const double *part;   // symv partials (nchunk x n)
-  int nchunk;
+// Panel of TNB columns (DLATRD, lower): the panel buffer P holds, for panel
+// column k, the reflector V(:,k) at column k, W(:,k) at column nb+k and a
+// second copy of V(:,k) at column 2nb+k, so that the trailing update
+// A -= V W^T + W V^T is one GEMM with operands P(:,0:2nb) and P(:,nb:3nb).
+// Per column i of the panel two multi-workgroup kernels run:
+//   k_tri_symv : y = A(i+1:n, i+1:n) v as partial sums per column chunk, the
+//                dot products V^T v, W^T v and v^T y as per-tile partials,
+//                and V(:,k) written out (v = A(:,i) * scal, v(i+1) = 1);
+//   k_tri_col  : w = tau (y - W V^T v - V W^T v) - (tau/2)(w^T v) v written
+//                to W(:,k); the next column updated with the panel so far;
+//                the norm of its sub-subdiagonal part reduced over the grid
+//                (last work-group finishes DLARFG: beta, tau, scal).
+struct TriWork {
+  int n, p0, nb;
+  double *a;          // n x n, both triangles
+  double *P;          // n x 3 TNB panel buffer
+  double *d, *e, *tau, *scal;
+  double *part;       // nchunk x n SYMV partial sums
+  double *dots;       // tiles x 2 TNB partial V^T v, W^T v
+  double *vy;         // tiles x chunks partial v^T y
+  double *norm;       // blocks: partial squared norms
+  int *counter;       // last-block ticket (zero between uses)
 };
 
-// finish w for panel column `fin` (if fin >= 0), then update column `col`
-// with the panel so far and generate its reflector (if col >= 0)
-__global__ __launch_bounds__(1024) void k_tri_step(TriArgs g, int fin, int col)
+__device__ __forceinline__ double wave_sum(double v)
 {
-  __shared__ double red[40];
-  __shared__ double vw[2 * TNB];
-  const int n = g.n, t = threadIdx.x, nt = blockDim.x;
-  if (fin >= 0) {
-    const int ii = fin - g.p0;
-    const int r0 = fin + 1;
-    double *w = g.wp + (size_t)n * ii;
-    const double *v = g.vp + (size_t)n * ii;
-    // y = sum of partials; corrections -W (V^T v) - V (W^T v) over earlier panel columns
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// y partials over column chunk blockIdx.y for the 256 rows of tile blockIdx.x
+__global__ __launch_bounds__(SYMV_ROWS) void k_tri_symv(TriWork g, int i, int nch)
+{
+  const int n = g.n, r0 = i + 1, ii = i - g.p0, nb = g.nb;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int r = r0 + blockIdx.x * SYMV_ROWS + t;
+  const int c0 = r0 + blockIdx.y * SYMV_COLS;
+  const double sc = g.scal[i];
+  const double *ai = g.a + (size_t)n * i;
+  __shared__ double vs[SYMV_COLS];
+  __shared__ double red[2 * TNB + 1][4];
+  {
+    const int c = c0 + t;
+    vs[t] = (c < n) ? (c == r0 ? 1.0 : ai[c] * sc) : 0.0;
+  }
+  __syncthreads();
+  const bool live = r < n;
+  double y = 0.0;
+  if (live) {
+    const int c1 = min(n, c0 + SYMV_COLS);
+    const double *ap = g.a + (size_t)r + (size_t)n * c0;
+    const int m = c1 - c0;
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+    int c = 0;
+    for (; c + 4 <= m; c += 4) {
+      y0 += ap[(size_t)n * c] * vs[c];
+      y1 += ap[(size_t)n * (c + 1)] * vs[c + 1];
+      y2 += ap[(size_t)n * (c + 2)] * vs[c + 2];
+      y3 += ap[(size_t)n * (c + 3)] * vs[c + 3];
+    }
+    for (; c < m; c++) y0 += ap[(size_t)n * c] * vs[c];
+    y = (y0 + y1) + (y2 + y3);
+    g.part[(size_t)blockIdx.y * n + r] = y;
+  }
+  const double vr = live ? (r == r0 ? 1.0 : ai[r] * sc) : 0.0;
+  // v^T y partial of this block
+  {
+    const double s = wave_sum(vr * y);
+    if (lane == 0) red[2 * TNB][wv] = s;
+  }
+  if (blockIdx.y == 0) {
+    double *V = g.P + (size_t)n * ii, *V2 = g.P + (size_t)n * (2 * nb + ii);
+    if (live) { V[r] = vr; V2[r] = vr; }
+    for (int q = blockIdx.x * SYMV_ROWS + t; q < r0; q += gridDim.x * SYMV_ROWS) { V[q] = 0.0; V2[q] = 0.0; }
     for (int k = 0; k < ii; k++) {
-      double s1 = 0.0, s2 = 0.0;
-      for (int r = r0 + t; r < n; r += nt) {
-        s1 += g.vp[(size_t)n * k + r] * v[r];
-        s2 += g.wp[(size_t)n * k + r] * v[r];
-      }
-      s1 = block_sum(s1, red);
-      s2 = block_sum(s2, red);
-      if (t == 0) { vw[k] = s1; vw[TNB + k] = s2; }
+      const double pv = live ? g.P[(size_t)n * k + r] : 0.0;
+      const double pw = live ? g.P[(size_t)n * (nb + k) + r] : 0.0;
+      const double sa = wave_sum(pv * vr), sb = wave_sum(pw * vr);
+      if (lane == 0) { red[k][wv] = sa; red[TNB + k][wv] = sb; }
+    }
+  }
+  __syncthreads();
+  if (t == 0) g.vy[(size_t)blockIdx.x * gridDim.y + blockIdx.y] = (red[2 * TNB][0] + red[2 * TNB][1]) + (red[2 * TNB][2] + red[2 * TNB][3]);
+  if (blockIdx.y == 0 && t < ii) {
+    double *dd = g.dots + (size_t)blockIdx.x * 2 * TNB;
+    dd[t] = (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
+    dd[TNB + t] = (red[TNB + t][0] + red[TNB + t][1]) + (red[TNB + t][2] + red[TNB + t][3]);
+  }
+}
+
+// finish W(:, fin - p0) (fin >= 0) and update + reflect column col (col >= 0)
+__global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, int ntile, int nch)
+{
+  const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x;
+  const int rbase = fin >= 0 ? fin + 1 : col;
+  const int r = rbase + blockIdx.x * 256 + t;
+  const bool live = r < n;
+  __shared__ double ab[2 * TNB];  // a_k = V_k^T v, b_k = W_k^T v
+  __shared__ double wc[TNB], vc[TNB];
+  __shared__ double sh[4];
+  __shared__ double red[4];
+  __shared__ int last;
+  const int ii = fin >= 0 ? fin - p0 : -1;
+  double wr = 0.0;
+  double tauf = 0.0, alpha2 = 0.0;
+  if (fin >= 0) {
+    tauf = g.tau[fin];
+    // redundant fixed-order reduction of the SYMV tile partials
+    for (int q = t; q < 2 * TNB; q += blockDim.x) {
+      double s = 0.0;
+      if ((q < TNB && q < ii) || (q >= TNB && q - TNB < ii))
+        for (int b = 0; b < ntile; b++) s += g.dots[(size_t)b * 2 * TNB + q];
+      ab[q] = s;
+    }
+    if (t == 0) {
+      double s = 0.0;
+      for (int b = 0; b < ntile * nch; b++) s += g.vy[b];
+      sh[0] = s;
     }
     __syncthreads();
-    const double tau = g.tau[fin];
-    double dot = 0.0;
-    for (int r = r0 + t; r < n; r += nt) {
+    double ssum = 0.0;
+    for (int k = 0; k < ii; k++) ssum += ab[k] * ab[TNB + k];
+    alpha2 = -0.5 * tauf * tauf * (sh[0] - 2.0 * ssum);
+    const double *V = g.P, *W = g.P + (size_t)n * nb;
+    double *Wi = g.P + (size_t)n * (nb + ii);
+    if (live) {
       double y = 0.0;
-      for (int c = 0; c < g.nchunk; c++) y += g.part[(size_t)c * n + r];
-      for (int k = 0; k < ii; k++) y -= g.wp[(size_t)n * k + r] * vw[k] + g.vp[(size_t)n * k + r] * vw[TNB + k];
-      y *= tau;
-      w[r] = y;
-      dot += y * v[r];
+      for (int c = 0; c < nch; c++) y += g.part[(size_t)c * n + r];
+      for (int k = 0; k < ii; k++) y -= W[(size_t)n * k + r] * ab[k] + V[(size_t)n * k + r] * ab[TNB + k];
+      wr = tauf * y + alpha2 * V[(size_t)n * ii + r];
+      Wi[r] = wr;
     }
-    dot = block_sum(dot, red);
-    const double alpha2 = -0.5 * tau * dot;
-    for (int r = r0 + t; r < n; r += nt) w[r] += alpha2 * v[r];
-    for (int r = t; r < r0; r += nt) w[r] = 0.0;
-    __syncthreads();
+    for (int q = blockIdx.x * 256 + t; q < rbase; q += gridDim.x * 256) Wi[q] = 0.0;
   }
   if (col < 0) return;
-  const int ii = col - g.p0;
-  double *ac = g.a + (size_t)n * col;
-  // column update with the earlier panel columns: A(col:n, col) -= V W(col,:)^T + W V(col,:)^T
-  if (ii > 0) {
-    if (t < ii) { vw[t] = g.wp[(size_t)n * t + col]; vw[TNB + t] = g.vp[(size_t)n * t + col]; }
-    __syncthreads();
-    for (int r = col + t; r < n; r += nt) {
-      double s = ac[r];
-      for (int k = 0; k < ii; k++) s -= g.vp[(size_t)n * k + r] * vw[k] + g.wp[(size_t)n * k + r] * vw[TNB + k];
-      ac[r] = s;
-    }
-    __syncthreads();
+  const int jj = col - p0;  // columns 0..jj-1 of the panel update this column
+  const double *V = g.P, *W = g.P + (size_t)n * nb;
+  if (t < jj) {
+    vc[t] = V[(size_t)n * t + col];
+    if (t < ii || fin < 0) wc[t] = W[(size_t)n * t + col];
   }
-  if (t == 0) g.d[col] = ac[col];
-  double *v = g.vp + (size_t)n * ii;
+  if (fin >= 0 && t == 0) {
+    // W(col, ii): every block recomputes the value of row col = fin + 1
+    double y = 0.0;
+    for (int c = 0; c < nch; c++) y += g.part[(size_t)c * n + col];
+    for (int k = 0; k < ii; k++) y -= W[(size_t)n * k + col] * ab[k] + V[(size_t)n * k + col] * ab[TNB + k];
+    wc[ii] = tauf * y + alpha2 * 1.0;
+  }
+  __syncthreads();
+  double *ac = g.a + (size_t)n * col;
+  double nrm = 0.0;
+  if (live && r >= col) {
+    double a = ac[r];
+    for (int k = 0; k < jj; k++) {
+      const double wk = (k == ii) ? wr : W[(size_t)n * k + r];
+      a -= V[(size_t)n * k + r] * wc[k] + wk * vc[k];
+    }
+    ac[r] = a;
+    if (r >= col + 2) nrm = a * a;
+  }
+  nrm = wave_sum(nrm);
+  if ((t & 63) == 0) red[t >> 6] = nrm;
+  __syncthreads();
+  if (t == 0) {
+    g.norm[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    __threadfence();
+    last = (atomicAdd(g.counter, 1) == (int)gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last || t != 0) return;
+  __threadfence();
+  *g.counter = 0;
+  double xs = 0.0;
+  for (int b = 0; b < (int)gridDim.x; b++) xs += g.norm[b];
+  g.d[col] = ac[col];
   if (col >= n - 1) {
-    for (int r = t; r < n; r += nt) v[r] = 0.0;
-    if (t == 0) g.tau[col] = 0.0;
+    g.tau[col] = 0.0;
+    g.scal[col] = 0.0;
     return;
   }
-  // reflector annihilating A(col+2:n, col) (DLARFG)
-  double xs = 0.0;
-  for (int r = col + 2 + t; r < n; r += nt) xs += ac[r] * ac[r];
-  xs = block_sum(xs, red);
+  // DLARFG on (A(col+1,col), A(col+2:n,col))
   const double xnorm = sqrt(xs);
   const double alpha = ac[col + 1];
   double tau = 0.0, beta = alpha, scal = 0.0;
@@ -286,21 +377,20 @@ __global__ __launch_bounds__(1024) void k_tri_step(TriArgs g, int fin, int col)
     tau = (beta - alpha) / beta;
     scal = 1.0 / (alpha - beta);
   }
-  __syncthreads();
-  for (int r = t; r < n; r += nt) {
-    double vr = 0.0;
-    if (r == col + 1) vr = 1.0;
-    else if (r > col + 1) {
-      vr = (xnorm != 0.0) ? ac[r] * scal : 0.0;
-      ac[r] = vr;  // reflector kept below the subdiagonal for the back-transform
-    }
-    v[r] = vr;
-  }
-  if (t == 0) {
-    g.e[col] = beta;
-    g.tau[col] = tau;
-    ac[col + 1] = beta;
-  }
+  g.e[col] = beta;
+  g.tau[col] = tau;
+  g.scal[col] = scal;
+}
+
+// reflectors of panel columns into A below the subdiagonal (read by the back-transform)
+__global__ void k_tri_store(TriWork g, int ncols)
+{
+  const int n = g.n;
+  const int k = blockIdx.y;
+  const int i = g.p0 + k;
+  const int r = i + 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncols || r >= n) return;
+  g.a[(size_t)r + (size_t)n * i] = g.P[(size_t)n * k + r];
 }
 
 // ------------------------------------------------------------------ bisection
@@ -607,40 +697,85 @@ __global__ void k_vpanel(int n, int p0, int nb, const double *__restrict__ a, do
   vp[(size_t)r + (size_t)n * k] = v;
 }
 
-// T (nb x nb upper) of the compact WY form I - V T V^T from G = V^T V (DLARFT 'F','C')
-__global__ void k_larft(int nb, const double *__restrict__ g, const double *__restrict__ tau, double *__restrict__ tm)
+// T (nb x nb upper) of the compact WY form I - V T V^T from G = V^T V
+// (DLARFT 'F','C'): T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i), column by column
+__global__ __launch_bounds__(64) void k_larft(int nb, const double *__restrict__ g, const double *__restrict__ tau,
+                                              double *__restrict__ tm)
 {
-  if (threadIdx.x != 0) return;
+  __shared__ double gs[TNB][TNB + 1], ts[TNB][TNB + 1];
+  const int t = threadIdx.x;
+  for (int idx = t; idx < nb * nb; idx += blockDim.x) {
+    const int r = idx % nb, c = idx / nb;
+    gs[r][c] = g[r + nb * c];
+    ts[r][c] = 0.0;
+  }
+  __syncthreads();
   for (int i = 0; i < nb; i++) {
-    for (int r = 0; r < nb; r++) if (r > i) tm[r + nb * i] = 0.0;
     const double ti = tau[i];
-    tm[i + nb * i] = ti;
-    for (int r = 0; r < i; r++) {
-      double s = 0.0;
-      for (int q = r; q < i; q++) s += tm[r + nb * q] * g[q + nb * i];
-      tm[r + nb * i] = -ti * s;
+    if (t < i) {
+      double sum = 0.0;
+      for (int q = t; q < i; q++) sum += ts[t][q] * gs[q][i];
+      ts[t][i] = -ti * sum;
+    } else if (t == i) {
+      ts[i][i] = ti;
     }
+    __syncthreads();
+  }
+  for (int idx = t; idx < nb * nb; idx += blockDim.x) {
+    const int r = idx % nb, c = idx / nb;
+    tm[r + nb * c] = ts[r][c];
   }
 }
 
-// selection sort of the eigenpairs (dsyevx2.f:381-400): permutation only
-__global__ void k_sortperm(int m, double *w, int *perm)
+// Cholesky G = R^T R of a small Gram matrix (cw <= 32) and R^-1 (upper);
+// *fail set when G is not numerically positive definite
+__global__ __launch_bounds__(64) void k_chol_rinv(int cw, const double *__restrict__ g, double *__restrict__ rinv,
+                                                  int *__restrict__ fail)
 {
-  if (threadIdx.x != 0) return;
-  for (int j = 0; j < m; j++) perm[j] = j;
-  for (int j = 0; j < m - 1; j++) {
-    int ii = -1;
-    double t = w[j];
-    for (int jj = j + 1; jj < m; jj++)
-      if (w[jj] < t) { ii = jj; t = w[jj]; }
-    if (ii >= 0) {
-      w[ii] = w[j];
-      w[j] = t;
-      const int p = perm[ii];
-      perm[ii] = perm[j];
-      perm[j] = p;
+  __shared__ double a[TNB][TNB + 1], ri[TNB][TNB + 1];
+  __shared__ int bad;
+  const int t = threadIdx.x;
+  if (t == 0) bad = 0;
+  for (int idx = t; idx < cw * cw; idx += blockDim.x) {
+    const int r = idx % cw, c = idx / cw;
+    a[r][c] = (r <= c) ? g[r + cw * c] : 0.0;
+    ri[r][c] = 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < cw; j++) {
+    if (t == 0) {
+      const double dj = a[j][j];
+      if (!(dj > 0.0) || !isfinite(dj)) {
+        bad = 1;
+        a[j][j] = 1.0;
+      } else {
+        a[j][j] = sqrt(dj);
+      }
+    }
+    __syncthreads();
+    if (t > j && t < cw) a[j][t] /= a[j][j];
+    __syncthreads();
+    const int m = cw - j - 1;
+    for (int idx = t; idx < m * m; idx += blockDim.x) {
+      const int r = j + 1 + idx % m, c = j + 1 + idx / m;
+      if (r <= c) a[r][c] -= a[j][r] * a[j][c];
+    }
+    __syncthreads();
+  }
+  if (t < cw) {  // column t of R^-1 by back substitution
+    const int c = t;
+    for (int r = c; r >= 0; r--) {
+      double v = (r == c) ? 1.0 : 0.0;
+      for (int q = r + 1; q <= c; q++) v -= a[r][q] * ri[q][c];
+      ri[r][c] = v / a[r][r];
     }
   }
+  __syncthreads();
+  for (int idx = t; idx < cw * cw; idx += blockDim.x) {
+    const int r = idx % cw, c = idx / cw;
+    rinv[r + cw * c] = (r <= c) ? ri[r][c] : 0.0;
+  }
+  if (t == 0 && bad) *fail = 1;
 }
 
 __global__ void k_gather_cols(int n, int m, const int *__restrict__ perm, const double *__restrict__ src,
@@ -672,14 +807,19 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
   if ((rc = X.alloc(nn))) return rc;
   // small buffers: info, d, e, e2, tau, panel V/W, partials, task arrays
   const int nchunk = (n + SYMV_COLS - 1) / SYMV_COLS;
-  const size_t small = (size_t)8 * n + (size_t)2 * n * TNB + (size_t)nchunk * n + 6 * (size_t)n + 64;
+  const size_t ntile = (size_t)nchunk;  // row tiles of the SYMV = column chunks
+  const size_t small = (size_t)14 * n + (size_t)3 * n * TNB + (size_t)nchunk * n + ntile * 2 * TNB +
+                       ntile * ntile + ntile + 64;
   if ((rc = c.w_small.alloc(small))) return rc;
-  double *d = c.w_small.p, *e = d + n, *e2 = e + n, *tau = e2 + n, *wv = tau + n;
-  double *vp = wv + n, *wp = vp + (size_t)n * TNB, *part = wp + (size_t)n * TNB;
+  double *d = c.w_small.p, *e = d + n, *e2 = e + n, *tau = e2 + n, *wv = tau + n, *scal = wv + n;
+  double *P = scal + n, *part = P + (size_t)3 * n * TNB;
   double *bgl = part + (size_t)nchunk * n, *bgu = bgl + n, *batol = bgu + n, *bnrm = batol + n;
-  double *xs = bnrm + n, *pivmin = xs + n;
-  if ((rc = c.w_int.alloc((size_t)6 * n + 16))) return rc;
-  int *info = c.w_int.p, *cnt = info + 4, *tblk = cnt + 8, *tidx = tblk + n, *bstart = tidx + n, *bend = bstart + n;
+  double *xs = bnrm + n, *tdots = xs + n, *tvy = tdots + ntile * 2 * TNB, *tnorm = tvy + ntile * ntile;
+  double *pivmin = tnorm + ntile;
+  if ((rc = c.w_int.alloc((size_t)6 * n + 32))) return rc;
+  // ints: [0,4) INFO, [4,12) counts, [12,16) grid-reduction tickets, then task arrays
+  int *info = c.w_int.p, *cnt = info + 4, *ticket = info + 12, *tblk = info + 16, *tidx = tblk + n;
+  int *bstart = tidx + n, *bend = bstart + n;
   int *cstart = bend + n, *perm = cstart + n;
   W2K_HIP(hipMemsetAsync(c.w_int.p, 0, sizeof(int) * 16, s));
 
@@ -735,35 +875,40 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
   // ---- 3. tridiagonalisation of C (in Hf)
   double *A = Hf;
   {
-    TriArgs g;
-    g.n = n; g.a = A; g.vp = vp; g.wp = wp; g.d = d; g.e = e; g.tau = tau; g.part = part;
+    TriWork g;
+    g.n = n; g.a = A; g.P = P; g.d = d; g.e = e; g.tau = tau; g.scal = scal;
+    g.part = part; g.dots = tdots; g.vy = tvy; g.norm = tnorm; g.counter = ticket;
     for (int p0 = 0; p0 < n; p0 += TNB) {
       const int nb = std::min(TNB, n - p0);
       g.p0 = p0;
-      g.nchunk = 0;
-      hipLaunchKernelGGL(k_tri_step, dim3(1), dim3(1024), 0, s, g, -1, p0);
-      W2K_STAGE(s, "k_tri_step");
+      g.nb = nb;
+      // reflector of the panel's first column (already updated by earlier panels)
+      hipLaunchKernelGGL(k_tri_col, dim3((n - p0 + 255) / 256), dim3(256), 0, s, g, -1, p0, 0, 0);
+      W2K_STAGE(s, "k_tri_col");
+      int nref = 0;  // panel columns that carry a reflector
       for (int k = 0; k < nb; k++) {
         const int i = p0 + k;
         if (i >= n - 1) break;
-        const int r0 = i + 1;
-        const int rows = n - r0;
-        const int nch = (rows + SYMV_COLS - 1) / SYMV_COLS;
-        hipLaunchKernelGGL(k_symv_part, dim3((rows + SYMV_ROWS - 1) / SYMV_ROWS, nch), dim3(SYMV_ROWS), 0, s, n, r0,
-                           A, vp + (size_t)n * k, part);
-        W2K_STAGE(s, "k_symv_part");
-        // partial sums are indexed from chunk 0 at column r0
-        g.nchunk = nch;
+        nref = k + 1;
+        const int rows = n - (i + 1);
+        const int nt = (rows + SYMV_ROWS - 1) / SYMV_ROWS, nch = (rows + SYMV_COLS - 1) / SYMV_COLS;
+        hipLaunchKernelGGL(k_tri_symv, dim3(nt, nch), dim3(SYMV_ROWS), 0, s, g, i, nch);
+        W2K_STAGE(s, "k_tri_symv");
         const int next = (k + 1 < nb) ? i + 1 : -1;
-        hipLaunchKernelGGL(k_tri_step, dim3(1), dim3(1024), 0, s, g, i, next);
-        W2K_STAGE(s, "k_tri_step");
+        hipLaunchKernelGGL(k_tri_col, dim3((rows + 255) / 256), dim3(256), 0, s, g, i, next, nt, nch);
+        W2K_STAGE(s, "k_tri_col");
+      }
+      if (nref > 0) {
+        hipLaunchKernelGGL(k_tri_store, dim3((n + 255) / 256, nref), dim3(256), 0, s, g, nref);
+        W2K_STAGE(s, "k_tri_store");
       }
       const int p1 = p0 + nb;
       if (p1 < n) {
+        // A(p1:n, p1:n) -= [V W] [W V]^T
         const int r = n - p1;
         double *A1 = A + (size_t)p1 + (size_t)n * p1;
-        if ((rc = dgemm(s, 'N', 'T', r, r, nb, -1.0, vp + p1, n, wp + p1, n, 1.0, A1, n))) return rc;
-        if ((rc = dgemm(s, 'N', 'T', r, r, nb, -1.0, wp + p1, n, vp + p1, n, 1.0, A1, n))) return rc;
+        if ((rc = dgemm(s, 'N', 'T', r, r, 2 * nb, -1.0, P + p1, n, P + (size_t)n * nb + p1, n, 1.0, A1, n)))
+          return rc;
       }
     }
   }
@@ -805,24 +950,56 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
     std::vector<int> cs(m);
     W2K_HIP(hipMemcpyAsync(cs.data(), cstart, sizeof(int) * m, hipMemcpyDeviceToHost, s));
     W2K_HIP(hipStreamSynchronize(s));
-    // clusters [t0, t1)
+    // clusters [t0, t1): chunks of 32 vectors, each orthogonalised against
+    // the cluster's earlier chunks (classical Gram-Schmidt, twice, on MFMA)
+    // and then within itself by CholQR2 (Z = Q R, R from the Cholesky of
+    // Z^T Z), which equals in-order Gram-Schmidt; a chunk whose Gram matrix
+    // is not numerically definite falls back to modified Gram-Schmidt
     if ((rc = c.w_bt.alloc((size_t)(n + 2 * m + 2 * TNB) * TNB))) return rc;
-    double *ybuf = c.w_bt.p;
+    if ((rc = c.w_cl.alloc((size_t)2 * n * TNB + 2 * TNB * TNB + (size_t)m * TNB))) return rc;
+    if ((rc = c.w_gk.alloc((size_t)1 << 22))) return rc;
+    double *csave = c.w_cl.p, *ctmp = csave + (size_t)n * TNB, *cg = ctmp + (size_t)n * TNB;
+    double *crinv = cg + TNB * TNB, *cy = crinv + TNB * TNB;
+    int *cfail = ticket + 1;
+    auto cgs2 = [&](int t0, int c0, int cw) -> int {
+      const int np = c0 - t0;
+      double *Q = Zt + (size_t)n * t0, *Zc = Zt + (size_t)n * c0;
+      for (int pass = 0; pass < 2 && np > 0; pass++) {
+        int r2;
+        if ((r2 = dgemm(s, 'T', 'N', np, cw, n, 1.0, Q, n, Zc, n, 0.0, cy, np, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
+          return r2;
+        if ((r2 = dgemm(s, 'N', 'N', n, cw, np, -1.0, Q, n, cy, np, 1.0, Zc, n))) return r2;
+      }
+      return 0;
+    };
     int t0 = 0;
     while (t0 < m) {
       int t1 = t0 + 1;
       while (t1 < m && cs[t1] == t0) t1++;
-      for (int c0 = t0; c0 < t1; c0 += 32) {
-        const int cw = std::min(32, t1 - c0);
-        for (int pass = 0; pass < 2 && c0 > t0; pass++) {
-          const int np = c0 - t0;
-          double *Q = Zt + (size_t)n * t0, *Zc = Zt + (size_t)n * c0;
-          if ((rc = dgemm(s, 'T', 'N', np, cw, n, 1.0, Q, n, Zc, n, 0.0, ybuf, np))) return rc;
-          if ((rc = dgemm(s, 'N', 'N', n, cw, np, -1.0, Q, n, ybuf, np, 1.0, Zc, n))) return rc;
-        }
-        if (t1 - t0 > 1) {
-          hipLaunchKernelGGL(k_mgs_chunk, dim3(1), dim3(256), 0, s, n, Zt, n, c0, cw);
-          W2K_STAGE(s, "k_mgs_chunk");
+      if (t1 - t0 > 1) {
+        for (int c0 = t0; c0 < t1; c0 += TNB) {
+          const int cw = std::min(TNB, t1 - c0);
+          double *Zc = Zt + (size_t)n * c0;
+          W2K_HIP(hipMemcpyAsync(csave, Zc, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
+          W2K_HIP(hipMemsetAsync(cfail, 0, sizeof(int), s));
+          if ((rc = cgs2(t0, c0, cw))) return rc;
+          for (int pass = 0; pass < 2; pass++) {
+            if ((rc = dgemm(s, 'T', 'N', cw, cw, n, 1.0, Zc, n, Zc, n, 0.0, cg, cw, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
+              return rc;
+            hipLaunchKernelGGL(k_chol_rinv, dim3(1), dim3(64), 0, s, cw, cg, crinv, cfail);
+            W2K_STAGE(s, "k_chol_rinv");
+            if ((rc = dgemm(s, 'N', 'N', n, cw, cw, 1.0, Zc, n, crinv, cw, 0.0, ctmp, n, 0, 0, 1))) return rc;
+            W2K_HIP(hipMemcpyAsync(Zc, ctmp, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
+          }
+          int hfail = 0;
+          W2K_HIP(hipMemcpyAsync(&hfail, cfail, sizeof(int), hipMemcpyDeviceToHost, s));
+          W2K_HIP(hipStreamSynchronize(s));
+          if (hfail) {
+            W2K_HIP(hipMemcpyAsync(Zc, csave, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
+            if ((rc = cgs2(t0, c0, cw))) return rc;
+            hipLaunchKernelGGL(k_mgs_chunk, dim3(1), dim3(256), 0, s, n, Zt, n, c0, cw);
+            W2K_STAGE(s, "k_mgs_chunk");
+          }
         }
       }
       t0 = t1;
@@ -844,15 +1021,34 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
       if (nb <= 0) continue;
       hipLaunchKernelGGL(k_vpanel, dim3((n + 255) / 256, nb), dim3(256), 0, s, n, p0, nb, A, vb);
       W2K_STAGE(s, "k_vpanel");
-      if ((rc = dgemm(s, 'T', 'N', nb, nb, n, 1.0, vb, n, vb, n, 0.0, gm, nb))) return rc;
+      if ((rc = dgemm(s, 'T', 'N', nb, nb, n, 1.0, vb, n, vb, n, 0.0, gm, nb, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
+        return rc;
       hipLaunchKernelGGL(k_larft, dim3(1), dim3(64), 0, s, nb, gm, tau + p0, tm);
       W2K_STAGE(s, "k_larft");
-      if ((rc = dgemm(s, 'T', 'N', nb, m, n, 1.0, vb, n, Zt, n, 0.0, y1, nb))) return rc;
+      if ((rc = dgemm(s, 'T', 'N', nb, m, n, 1.0, vb, n, Zt, n, 0.0, y1, nb, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
+        return rc;
       if ((rc = dgemm(s, 'N', 'N', nb, m, nb, 1.0, tm, nb, y1, nb, 0.0, y2, nb, 0, 1, 0))) return rc;
       if ((rc = dgemm(s, 'N', 'N', n, m, nb, -1.0, vb, n, y2, nb, 1.0, Zt, n))) return rc;
     }
-    hipLaunchKernelGGL(k_sortperm, dim3(1), dim3(64), 0, s, m, w, perm);
-    W2K_STAGE(s, "k_sortperm");
+    // selection sort of dsyevx2.f:381-400 on the host (M values), as a permutation
+    std::vector<double> hw(m);
+    std::vector<int> hp(m);
+    W2K_HIP(hipMemcpyAsync(hw.data(), w, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    W2K_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < m; j++) hp[j] = j;
+    for (int j = 0; j < m - 1; j++) {
+      int ii = -1;
+      double tmin = hw[j];
+      for (int jj = j + 1; jj < m; jj++)
+        if (hw[jj] < tmin) { ii = jj; tmin = hw[jj]; }
+      if (ii >= 0) {
+        hw[ii] = hw[j];
+        hw[j] = tmin;
+        std::swap(hp[ii], hp[j]);
+      }
+    }
+    W2K_HIP(hipMemcpyAsync(perm, hp.data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
+    W2K_HIP(hipMemcpyAsync(w, hw.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
     double *Zs = U;  // U is not needed after the inversion
     hipLaunchKernelGGL(k_gather_cols, dim3((n + 255) / 256, m), dim3(256), 0, s, n, m, perm, Zt, Zs);
     W2K_STAGE(s, "k_gather_cols");

@@ -634,22 +634,37 @@ int w2k_last_timings(w2k_ctx *x, double *ms, int n)
 
 }  // extern "C"
 
-extern "C" int w2k_test_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
-                              const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri)
+static int test_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda, const double *B,
+                      int ldb, double beta, double *C, int ldc, int tri, int atri, int btri, size_t wsn)
 {
   if (hipSetDevice(0) != hipSuccess) return W2K_ENODEV;
   const size_t na = (size_t)lda * ((ta == 'N' || ta == 'n') ? k : m);
   const size_t nb = (size_t)ldb * ((tb == 'N' || tb == 'n') ? n : k);
   const size_t nc = (size_t)ldc * n;
-  DBuf<double> dA, dB, dC;
+  DBuf<double> dA, dB, dC, dW;
   int rc;
   if ((rc = dA.alloc(na)) || (rc = dB.alloc(nb)) || (rc = dC.alloc(nc))) return rc;
+  if (wsn && (rc = dW.alloc(wsn))) return rc;
   W2K_HIP(hipMemcpy(dA.p, A, sizeof(double) * na, hipMemcpyHostToDevice));
   W2K_HIP(hipMemcpy(dB.p, B, sizeof(double) * nb, hipMemcpyHostToDevice));
   W2K_HIP(hipMemcpy(dC.p, C, sizeof(double) * nc, hipMemcpyHostToDevice));
-  rc = dgemm(nullptr, ta, tb, m, n, k, alpha, dA.p, lda, dB.p, ldb, beta, dC.p, ldc, tri, atri, btri);
+  rc = dgemm(nullptr, ta, tb, m, n, k, alpha, dA.p, lda, dB.p, ldb, beta, dC.p, ldc, tri, atri, btri,
+             wsn ? dW.p : nullptr, wsn);
   if (rc) return rc;
   W2K_HIP(hipDeviceSynchronize());
   W2K_HIP(hipMemcpy(C, dC.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
   return 0;
+}
+
+extern "C" int w2k_test_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+                              const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri)
+{
+  return test_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, atri, btri, 0);
+}
+
+extern "C" int w2k_test_dgemm_ws(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+                                 const double *B, int ldb, double beta, double *C, int ldc, long wsn)
+{
+  if (wsn < 0) return W2K_EINVAL;
+  return test_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, 0, 0, (size_t)wsn);
 }

@@ -119,6 +119,8 @@ struct Ctx {
   DBuf<double> w_fac;   // inverse-iteration factors, [plane][row][eigenvalue]
   DBuf<int> w_piv;      // inverse-iteration pivots
   DBuf<double> w_bt;    // back-transform panel work
+  DBuf<double> w_cl;    // cluster orthogonalisation work
+  DBuf<double> w_gk;    // split-K GEMM partial products
   // timing
   hipEvent_t ev[W2K_NEVENTS];
   double last_ms[W2K_NTIMINGS] = {0};
@@ -136,8 +138,11 @@ void take_ms(Ctx &c, int slot, int e0, int e1);
 // tri: 0 full, 1 lower (i >= j) only, 2 upper (i <= j) only (tiles outside skipped)
 // atri/btri: 0 dense, 1 A (resp. B) upper triangular, 2 lower triangular (zero
 // blocks skipped; the caller guarantees the structure)
+// ws/wsn: optional workspace (doubles) that lets a product with few output
+// tiles and a long K split K over the chip (partials reduced in fixed order)
 int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
-          const double *B, int ldb, double beta, double *C, int ldc, int tri = 0, int atri = 0, int btri = 0);
+          const double *B, int ldb, double beta, double *C, int ldc, int tri = 0, int atri = 0, int btri = 0,
+          double *ws = nullptr, size_t wsn = 0);
 
 }  // namespace w2k
 

@@ -15,6 +15,11 @@ extern "C" {
 int w2k_test_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
                    const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri);
 
+/* The same dense product with a device workspace of wsn doubles, which lets
+ * a product with few output tiles and a long K split K across the chip. */
+int w2k_test_dgemm_ws(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+                      const double *B, int ldb, double beta, double *C, int ldc, long wsn);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,27 @@ def test_dgemm_matches_numpy(ta, tb, m, n, k):
     assert np.max(np.abs(out - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("ta,tb", [("T", "N"), ("N", "N"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(32, 32, 5951), (32, 1184, 5951), (7, 33, 2049), (130, 20, 4097)])
+def test_dgemm_split_k_matches_numpy(ta, tb, m, n, k):
+    """Few output tiles and a long K: the split-K path (partials reduced in a fixed order)."""
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+    C = rng.standard_normal((m, n))
+    lib = L.load_library()
+    fn = lib.w2k_test_dgemm_ws
+    dp = ctypes.POINTER(ctypes.c_double)
+    fn.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                   dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int, ctypes.c_long]
+    A = np.asfortranarray(A); B = np.asfortranarray(B); out = np.asfortranarray(C.copy())
+    rc = fn(ta.encode(), tb.encode(), m, n, k, 0.5, A.ctypes.data_as(dp), A.shape[0], B.ctypes.data_as(dp),
+            B.shape[0], 2.0, out.ctypes.data_as(dp), out.shape[0], 1 << 22)
+    assert rc == 0
+    ref = 0.5 * (A if ta == "N" else A.T) @ (B if tb == "N" else B.T) + 2.0 * C
+    assert np.max(np.abs(out - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
 @pytest.mark.parametrize("tri", [1, 2])
 def test_dgemm_triangular_output(tri):
     rng = np.random.default_rng(5)
