@@ -195,7 +195,7 @@ int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
   // few output tiles and a long K: split K over z-slices so that the whole
   // chip works, then reduce the slices in a fixed order
   const long tiles = (long)grid.x * grid.y;
-  if (ws && tri == 0 && atri == 0 && btri == 0 && tiles < 256 && k >= 4 * 256) {
+  if (ws && tri == 0 && tiles < 256 && k >= 4 * 256) {
     int nz = (int)std::min<long>((k + 255) / 256, std::max<long>(1, 1024 / tiles));
     nz = (int)std::min<size_t>((size_t)nz, wsn / ((size_t)m * n));
     if (nz >= 2) {

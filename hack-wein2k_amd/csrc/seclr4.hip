@@ -38,7 +38,7 @@ namespace w2k {
 
 constexpr int NB = 64;    // Cholesky / inversion block
 constexpr int TNB = 32;   // tridiagonalisation panel
-constexpr int SYMV_ROWS = 256, SYMV_COLS = 256;
+constexpr int SYMV_T = 64;  // tile of the symmetric SYMV (one wave)
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ double block_sum(double v, double *red)
@@ -218,68 +218,87 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
-// y partials over column chunk blockIdx.y for the 256 rows of tile blockIdx.x
-__global__ __launch_bounds__(SYMV_ROWS) void k_tri_symv(TriWork g, int i, int nch)
+// y = A(r0:n, r0:n) v from the tiles on and below the diagonal only (A is
+// symmetric): one wave per 64 x 64 tile (I, J), J <= I, adds A_IJ v_J to the
+// rows of tile I and, for J < I, A_IJ^T v_I to the rows of tile J (column
+// sums through an LDS transpose).  Row r of tile R receives exactly one
+// partial per tile index: from (R, J) at slot J <= R and from (I, R) at slot
+// I > R, so y(r) = sum over slots of part[slot][r] in a fixed order.
+// Blocks with J = 0 also write V(:,k) and the partial dot products
+// V^T v, W^T v of their rows; every block writes its partial v^T y.
+__global__ __launch_bounds__(64) void k_tri_symv(TriWork g, int i, int nt)
 {
   const int n = g.n, r0 = i + 1, ii = i - g.p0, nb = g.nb;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int r = r0 + blockIdx.x * SYMV_ROWS + t;
-  const int c0 = r0 + blockIdx.y * SYMV_COLS;
+  const int lane = threadIdx.x;
+  // triangular block index -> (I, J), J <= I
+  const int bid = blockIdx.x;
+  int I = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= bid) I++;
+  while (I * (I + 1) / 2 > bid) I--;
+  const int J = bid - I * (I + 1) / 2;
+  const int R = r0 + SYMV_T * I + lane;
+  const int C0 = r0 + SYMV_T * J;
   const double sc = g.scal[i];
   const double *ai = g.a + (size_t)n * i;
-  __shared__ double vs[SYMV_COLS];
-  __shared__ double red[2 * TNB + 1][4];
+  __shared__ double vs[SYMV_T];
+  __shared__ double buf[SYMV_T][SYMV_T + 1];
   {
-    const int c = c0 + t;
-    vs[t] = (c < n) ? (c == r0 ? 1.0 : ai[c] * sc) : 0.0;
+    const int c = C0 + lane;
+    vs[lane] = (c < n) ? (c == r0 ? 1.0 : ai[c] * sc) : 0.0;
   }
+  const bool live = R < n;
+  const double vr = live ? (R == r0 ? 1.0 : ai[R] * sc) : 0.0;
   __syncthreads();
-  const bool live = r < n;
-  double y = 0.0;
+  const bool off = (I != J);
+  const int m = min(SYMV_T, n - C0);
+  double y0 = 0.0, y1 = 0.0;
   if (live) {
-    const int c1 = min(n, c0 + SYMV_COLS);
-    const double *ap = g.a + (size_t)r + (size_t)n * c0;
-    const int m = c1 - c0;
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
-    int c = 0;
-    for (; c + 4 <= m; c += 4) {
-      y0 += ap[(size_t)n * c] * vs[c];
-      y1 += ap[(size_t)n * (c + 1)] * vs[c + 1];
-      y2 += ap[(size_t)n * (c + 2)] * vs[c + 2];
-      y3 += ap[(size_t)n * (c + 3)] * vs[c + 3];
+    const double *ap = g.a + (size_t)R + (size_t)n * C0;
+    int j = 0;
+    for (; j + 2 <= m; j += 2) {
+      const double a0 = ap[(size_t)n * j], a1 = ap[(size_t)n * (j + 1)];
+      y0 += a0 * vs[j];
+      y1 += a1 * vs[j + 1];
+      if (off) { buf[j][lane] = a0 * vr; buf[j + 1][lane] = a1 * vr; }
     }
-    for (; c < m; c++) y0 += ap[(size_t)n * c] * vs[c];
-    y = (y0 + y1) + (y2 + y3);
-    g.part[(size_t)blockIdx.y * n + r] = y;
+    for (; j < m; j++) {
+      const double a0 = ap[(size_t)n * j];
+      y0 += a0 * vs[j];
+      if (off) buf[j][lane] = a0 * vr;
+    }
   }
-  const double vr = live ? (r == r0 ? 1.0 : ai[r] * sc) : 0.0;
-  // v^T y partial of this block
-  {
-    const double s = wave_sum(vr * y);
-    if (lane == 0) red[2 * TNB][wv] = s;
+  if (off && !live)
+    for (int j = 0; j < m; j++) buf[j][lane] = 0.0;
+  const double yrow = y0 + y1;
+  if (live) g.part[(size_t)J * n + R] = yrow;
+  double vy = yrow * vr;
+  if (off) {
+    __syncthreads();
+    if (lane < m) {
+      double cs = 0.0;
+      for (int rr = 0; rr < SYMV_T; rr++) cs += buf[lane][rr];
+      g.part[(size_t)I * n + C0 + lane] = cs;
+      vy += cs * vs[lane];
+    }
   }
-  if (blockIdx.y == 0) {
+  vy = wave_sum(vy);
+  if (lane == 0) g.vy[bid] = vy;
+  if (J == 0) {
     double *V = g.P + (size_t)n * ii, *V2 = g.P + (size_t)n * (2 * nb + ii);
-    if (live) { V[r] = vr; V2[r] = vr; }
-    for (int q = blockIdx.x * SYMV_ROWS + t; q < r0; q += gridDim.x * SYMV_ROWS) { V[q] = 0.0; V2[q] = 0.0; }
+    if (live) { V[R] = vr; V2[R] = vr; }
+    for (int q = I * SYMV_T + lane; q < r0; q += nt * SYMV_T) { V[q] = 0.0; V2[q] = 0.0; }
+    double *dd = g.dots + (size_t)I * 2 * TNB;
     for (int k = 0; k < ii; k++) {
-      const double pv = live ? g.P[(size_t)n * k + r] : 0.0;
-      const double pw = live ? g.P[(size_t)n * (nb + k) + r] : 0.0;
+      const double pv = live ? g.P[(size_t)n * k + R] : 0.0;
+      const double pw = live ? g.P[(size_t)n * (nb + k) + R] : 0.0;
       const double sa = wave_sum(pv * vr), sb = wave_sum(pw * vr);
-      if (lane == 0) { red[k][wv] = sa; red[TNB + k][wv] = sb; }
+      if (lane == 0) { dd[k] = sa; dd[TNB + k] = sb; }
     }
-  }
-  __syncthreads();
-  if (t == 0) g.vy[(size_t)blockIdx.x * gridDim.y + blockIdx.y] = (red[2 * TNB][0] + red[2 * TNB][1]) + (red[2 * TNB][2] + red[2 * TNB][3]);
-  if (blockIdx.y == 0 && t < ii) {
-    double *dd = g.dots + (size_t)blockIdx.x * 2 * TNB;
-    dd[t] = (red[t][0] + red[t][1]) + (red[t][2] + red[t][3]);
-    dd[TNB + t] = (red[TNB + t][0] + red[TNB + t][1]) + (red[TNB + t][2] + red[TNB + t][3]);
   }
 }
 
 // finish W(:, fin - p0) (fin >= 0) and update + reflect column col (col >= 0)
-__global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, int ntile, int nch)
+__global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, int ntile, int nch, int nvy)
 {
   const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x;
   const int rbase = fin >= 0 ? fin + 1 : col;
@@ -295,18 +314,21 @@ __global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, in
   double tauf = 0.0, alpha2 = 0.0;
   if (fin >= 0) {
     tauf = g.tau[fin];
-    // redundant fixed-order reduction of the SYMV tile partials
+    // fixed-order reductions of the SYMV partials, spread over the block
     for (int q = t; q < 2 * TNB; q += blockDim.x) {
       double s = 0.0;
       if ((q < TNB && q < ii) || (q >= TNB && q - TNB < ii))
         for (int b = 0; b < ntile; b++) s += g.dots[(size_t)b * 2 * TNB + q];
       ab[q] = s;
     }
-    if (t == 0) {
+    {
       double s = 0.0;
-      for (int b = 0; b < ntile * nch; b++) s += g.vy[b];
-      sh[0] = s;
+      for (int b = t; b < nvy; b += blockDim.x) s += g.vy[b];
+      s = wave_sum(s);
+      if ((t & 63) == 0) red[t >> 6] = s;
     }
+    __syncthreads();
+    if (t == 0) sh[0] = (red[0] + red[1]) + (red[2] + red[3]);
     __syncthreads();
     double ssum = 0.0;
     for (int k = 0; k < ii; k++) ssum += ab[k] * ab[TNB + k];
@@ -329,12 +351,18 @@ __global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, in
     vc[t] = V[(size_t)n * t + col];
     if (t < ii || fin < 0) wc[t] = W[(size_t)n * t + col];
   }
-  if (fin >= 0 && t == 0) {
+  if (fin >= 0) {
     // W(col, ii): every block recomputes the value of row col = fin + 1
-    double y = 0.0;
-    for (int c = 0; c < nch; c++) y += g.part[(size_t)c * n + col];
-    for (int k = 0; k < ii; k++) y -= W[(size_t)n * k + col] * ab[k] + V[(size_t)n * k + col] * ab[TNB + k];
-    wc[ii] = tauf * y + alpha2 * 1.0;
+    // (sum of the SYMV partials of that row minus the panel corrections)
+    double s = 0.0;
+    for (int c = t; c < nch; c += blockDim.x) s += g.part[(size_t)c * n + col];
+    for (int k = t; k < ii; k += blockDim.x)
+      s -= W[(size_t)n * k + col] * ab[k] + V[(size_t)n * k + col] * ab[TNB + k];
+    s = wave_sum(s);
+    __syncthreads();  // red[] reuse
+    if ((t & 63) == 0) red[t >> 6] = s;
+    __syncthreads();
+    if (t == 0) wc[ii] = tauf * ((red[0] + red[1]) + (red[2] + red[3])) + alpha2 * 1.0;
   }
   __syncthreads();
   double *ac = g.a + (size_t)n * col;
@@ -503,26 +531,42 @@ __global__ void k_bisect_prep(BisArgs g)
   g.cnt[2] = nb;
 }
 
-__global__ void k_bisect(BisArgs g, int m, double *__restrict__ w)
+// multisection: one wave per eigenvalue; each round the 64 lanes take the
+// Sturm count at 64 interior points of (lo, hi], and the bracket shrinks to the
+// sub-interval where the count first exceeds the eigenvalue's index (65x per
+// round instead of 2x); stopping test of dlaebz (ATOL, PIVMIN, RELTOL)
+__global__ __launch_bounds__(256) void k_bisect(BisArgs g, int m, double *__restrict__ w)
 {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
+  const int lane = threadIdx.x & 63;
+  const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (t >= m) return;  // wave-uniform
   const int b = g.tblk[t], k = g.tidx[t];
   const int ib = g.bstart[b], in = g.bend[b] - ib;
   if (k < 0) {
-    w[t] = g.d[ib];
+    if (lane == 0) w[t] = g.d[ib];
     return;
   }
   const double pivmin = *g.pivmin, atoli = g.batol[b], rtoli = 2.0 * DBL_EPSILON;
   double lo = g.bgl[b], hi = g.bgu[b];
-  for (int it = 0; it < 200; it++) {
+  for (int it = 0; it < 64; it++) {
     const double w1 = fabs(hi - lo), w2 = fmax(fabs(hi), fabs(lo));
     if (w1 < fmax(atoli, fmax(pivmin, rtoli * w2))) break;
-    const double mid = 0.5 * (lo + hi);
-    if (sturm_count(in, g.d + ib, g.e2 + ib, mid, pivmin) > k) hi = mid;
-    else lo = mid;
+    const double h = (hi - lo) / 65.0;
+    double x = lo + h * (double)(lane + 1);
+    if (lane == 63 || !(x < hi)) x = hi;  // the last point is hi itself
+    const int cnt = sturm_count(in, g.d + ib, g.e2 + ib, x, pivmin);
+    const unsigned long long above = __ballot(cnt > k);
+    if (above == 0ull) {
+      lo = hi;  // cannot happen for a valid bracket; stop narrowing
+      break;
+    }
+    const int f = __ffsll((long long)above) - 1;  // first point with count > k
+    const double xf = __shfl(x, f, 64);
+    const double xl = f > 0 ? __shfl(x, f > 0 ? f - 1 : 0, 64) : lo;
+    hi = xf;
+    lo = xl;
   }
-  w[t] = 0.5 * (lo + hi);
+  if (lane == 0) w[t] = 0.5 * (lo + hi);
 }
 
 // ------------------------------------------------------------------ inverse iteration
@@ -551,9 +595,14 @@ __global__ void k_stein_prep(BisArgs g, int m, const double *__restrict__ w, dou
 }
 
 // tridiagonal LU with partial pivoting (DLAGTF) and solves (DLAGTS), one
-// thread per eigenvalue; factors stored interleaved [row][task]
-__global__ void k_stein(BisArgs g, int m, const double *__restrict__ xs, double *__restrict__ z, int ldz,
-                        double *__restrict__ f, int *__restrict__ piv)
+// thread per eigenvalue; factors and the work vector interleaved
+// [row][eigenvalue] so that a wave's accesses coalesce, and every running
+// value carried in registers so that no pass waits on its own stores
+__global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__restrict__ xs,
+                                              double *__restrict__ z, int ldz, double *__restrict__ u0,
+                                              double *__restrict__ u1, double *__restrict__ u2,
+                                              double *__restrict__ mul, double *__restrict__ x,
+                                              int *__restrict__ piv)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m) return;
@@ -567,66 +616,118 @@ __global__ void k_stein(BisArgs g, int m, const double *__restrict__ xs, double 
   const double lam = xs[t];
   const double *d = g.d + ib, *e = g.e + ib;
   const size_t M = (size_t)m;
-  double *u0 = f, *u1 = f + (size_t)g.n * M, *u2 = f + 2 * (size_t)g.n * M, *mul = f + 3 * (size_t)g.n * M;
   const double onenrm = g.bnrm[b];
   const double tol = fmax(fabs(lam), onenrm) * DBL_EPSILON;
+  // factorisation; small pivots of U perturbed to +-tol for the solves
   double a0 = d[0] - lam, a1 = e[0];
   for (int k = 0; k < bs - 1; k++) {
     const double sub = e[k];
     const double dn = d[k + 1] - lam, en = (k + 1 < bs - 1) ? e[k + 1] : 0.0;
     const size_t o = (size_t)k * M + t;
+    double p0, p1, p2, mm;
+    int pv;
     if (fabs(a0) >= fabs(sub)) {
-      piv[o] = 0; u0[o] = a0; u1[o] = a1; u2[o] = 0.0;
-      const double mm = (a0 != 0.0) ? sub / a0 : 0.0;
-      mul[o] = mm;
+      pv = 0; p0 = a0; p1 = a1; p2 = 0.0;
+      mm = (a0 != 0.0) ? sub / a0 : 0.0;
       a0 = dn - mm * a1;
       a1 = en;
     } else {
-      piv[o] = 1; u0[o] = sub; u1[o] = dn; u2[o] = en;
-      const double mm = a0 / sub;
-      mul[o] = mm;
+      pv = 1; p0 = sub; p1 = dn; p2 = en;
+      mm = a0 / sub;
       const double na0 = a1 - mm * dn, na1 = -mm * en;
       a0 = na0;
       a1 = na1;
     }
+    if (fabs(p0) < tol) p0 = (p0 < 0 ? -tol : tol);
+    piv[o] = pv; u0[o] = p0; u1[o] = p1; u2[o] = p2; mul[o] = mm;
   }
   {
     const size_t o = (size_t)(bs - 1) * M + t;
-    u0[o] = a0; u1[o] = 0.0; u2[o] = 0.0;
+    double p0 = a0;
+    if (fabs(p0) < tol) p0 = (p0 < 0 ? -tol : tol);
+    u0[o] = p0; u1[o] = 0.0; u2[o] = 0.0;
   }
-  for (int k = 0; k < bs; k++) {
-    const size_t o = (size_t)k * M + t;
-    if (fabs(u0[o]) < tol) u0[o] = (u0[o] < 0 ? -tol : tol);
-  }
-  // deterministic start vector, then three solves on an interleaved work
-  // vector (plane 4 of the factor storage), copied to the column at the end
-  double *x = f + 4 * (size_t)g.n * M;
-  unsigned long long s = 0x9E3779B97F4A7C15ull ^ ((unsigned long long)(t + 1) * 0xD1B54A32D192ED03ull);
-  for (int k = 0; k < bs; k++) {
-    s = s * 6364136223846793005ull + 1442695040888963407ull;
-    x[(size_t)k * M + t] = ((double)(s >> 11) / 9007199254740992.0) * 2.0 - 1.0;
-  }
+  // deterministic start vector in [-1, 1), then three solves
+  unsigned long long seed = 0x9E3779B97F4A7C15ull ^ ((unsigned long long)(t + 1) * 0xD1B54A32D192ED03ull);
+  auto rnd = [&]() {
+    seed = seed * 6364136223846793005ull + 1442695040888963407ull;
+    return ((double)(seed >> 11) / 9007199254740992.0) * 2.0 - 1.0;
+  };
+  double sc = 1.0;
+  constexpr int G = 4;  // rows per group: loads of a group are issued before its arithmetic
   for (int it = 0; it < 3; it++) {
-    double mx = 0.0;
-    for (int k = 0; k < bs; k++) mx = fmax(mx, fabs(x[(size_t)k * M + t]));
-    const double sc = 1.0 / mx;
-    for (int k = 0; k < bs; k++) x[(size_t)k * M + t] *= sc;
-    for (int k = 0; k < bs - 1; k++) {
-      const size_t o = (size_t)k * M + t;
-      double xk = x[o], xk1 = x[o + M];
-      if (piv[o]) { const double tmp = xk; xk = xk1; xk1 = tmp; }
-      x[o] = xk;
-      x[o + M] = xk1 - mul[o] * xk;
+    // forward: L^-1 P x, x scaled by 1/max|x| of the previous solve
+    double xk = (it == 0) ? rnd() : sc * x[t];
+    int k = 0;
+    for (; k + G <= bs - 1; k += G) {
+      int pg[G];
+      double mg[G], xg[G];
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const size_t o = (size_t)(k + q) * M + t;
+        pg[q] = piv[o];
+        mg[q] = mul[o];
+        xg[q] = (it == 0) ? 0.0 : x[o + M];
+      }
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const size_t o = (size_t)(k + q) * M + t;
+        double xn = (it == 0) ? rnd() : sc * xg[q];
+        if (pg[q]) { const double tmp = xk; xk = xn; xn = tmp; }
+        x[o] = xk;
+        xk = xn - mg[q] * xk;
+      }
     }
-    for (int k = bs - 1; k >= 0; k--) {
+    for (; k < bs - 1; k++) {
+      const size_t o = (size_t)k * M + t;
+      double xn = (it == 0) ? rnd() : sc * x[o + M];
+      if (piv[o]) { const double tmp = xk; xk = xn; xn = tmp; }
+      x[o] = xk;
+      xk = xn - mul[o] * xk;
+    }
+    x[(size_t)(bs - 1) * M + t] = xk;
+    // backward: U^-1
+    double mx = 0.0, x1 = 0.0, x2 = 0.0;
+    const bool lastit = (it == 2);
+    k = bs - 1;
+    for (; k - G + 1 >= 0; k -= G) {
+      double xg[G], a0g[G], a1g[G], a2g[G];
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const size_t o = (size_t)(k - q) * M + t;
+        xg[q] = x[o];
+        a0g[q] = u0[o];
+        a1g[q] = u1[o];
+        a2g[q] = u2[o];
+      }
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const int kk = k - q;
+        double v = xg[q];
+        if (kk + 1 < bs) v -= a1g[q] * x1;
+        if (kk + 2 < bs) v -= a2g[q] * x2;
+        v /= a0g[q];
+        if (lastit) zc[ib + kk] = v;
+        else x[(size_t)kk * M + t] = v;
+        mx = fmax(mx, fabs(v));
+        x2 = x1;
+        x1 = v;
+      }
+    }
+    for (; k >= 0; k--) {
       const size_t o = (size_t)k * M + t;
       double v = x[o];
-      if (k + 1 < bs) v -= u1[o] * x[o + M];
-      if (k + 2 < bs) v -= u2[o] * x[o + 2 * M];
-      x[o] = v / u0[o];
+      if (k + 1 < bs) v -= u1[o] * x1;
+      if (k + 2 < bs) v -= u2[o] * x2;
+      v /= u0[o];
+      if (lastit) zc[ib + k] = v;
+      else x[o] = v;
+      mx = fmax(mx, fabs(v));
+      x2 = x1;
+      x1 = v;
     }
+    sc = 1.0 / mx;
   }
-  for (int k = 0; k < bs; k++) zc[ib + k] = x[(size_t)k * M + t];
 }
 
 // orthonormalise columns c0..c0+cw-1 of z in order (MGS), one work-group
@@ -805,16 +906,17 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
   double *U = c.w_a.p, *Hf = c.w_b.p, *W = c.w_c.p;
   DBuf<double> &X = c.w_x;
   if ((rc = X.alloc(nn))) return rc;
+  if ((rc = c.w_gk.alloc((size_t)1 << 22))) return rc;  // split-K partial products
   // small buffers: info, d, e, e2, tau, panel V/W, partials, task arrays
-  const int nchunk = (n + SYMV_COLS - 1) / SYMV_COLS;
-  const size_t ntile = (size_t)nchunk;  // row tiles of the SYMV = column chunks
+  const int nchunk = (n + SYMV_T - 1) / SYMV_T;  // SYMV tiles per dimension (at most)
+  const size_t ntile = (size_t)nchunk;
   const size_t small = (size_t)14 * n + (size_t)3 * n * TNB + (size_t)nchunk * n + ntile * 2 * TNB +
-                       ntile * ntile + ntile + 64;
+                       ntile * (ntile + 1) / 2 + ntile + 64;
   if ((rc = c.w_small.alloc(small))) return rc;
   double *d = c.w_small.p, *e = d + n, *e2 = e + n, *tau = e2 + n, *wv = tau + n, *scal = wv + n;
   double *P = scal + n, *part = P + (size_t)3 * n * TNB;
   double *bgl = part + (size_t)nchunk * n, *bgu = bgl + n, *batol = bgu + n, *bnrm = batol + n;
-  double *xs = bnrm + n, *tdots = xs + n, *tvy = tdots + ntile * 2 * TNB, *tnorm = tvy + ntile * ntile;
+  double *xs = bnrm + n, *tdots = xs + n, *tvy = tdots + ntile * 2 * TNB, *tnorm = tvy + ntile * (ntile + 1) / 2;
   double *pivmin = tnorm + ntile;
   if ((rc = c.w_int.alloc((size_t)6 * n + 32))) return rc;
   // ints: [0,4) INFO, [4,12) counts, [12,16) grid-reduction tickets, then task arrays
@@ -835,8 +937,16 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
       W2K_STAGE(s, "k_potf2");
       const int rest = n - k0 - kb;
       if (rest > 0) {
-        hipLaunchKernelGGL(k_trsm_row, dim3((rest + 255) / 256), dim3(256), 0, s, n, k0, kb, U);
-        W2K_STAGE(s, "k_trsm_row");
+        // U(k0:k0+kb, k0+kb:n) = U_kk^-T S(k0:k0+kb, k0+kb:n) with U_kk^-1 from
+        // the diagonal-block inversion (into W, free until the reduction)
+        hipLaunchKernelGGL(k_trtri_diag, dim3(1), dim3(256), 0, s, n, k0, kb, U, W);
+        W2K_STAGE(s, "k_trtri_diag");
+        double *S12 = U + (size_t)k0 + (size_t)n * (k0 + kb);
+        if ((rc = dgemm(s, 'T', 'N', kb, rest, kb, 1.0, W + (size_t)k0 + (size_t)n * k0, n, S12, n, 0.0, X.p, kb, 0,
+                        2, 0)))
+          return rc;
+        W2K_HIP(hipMemcpy2DAsync(S12, sizeof(double) * n, X.p, sizeof(double) * kb, sizeof(double) * kb, rest,
+                                 hipMemcpyDeviceToDevice, s));
         double *X0 = U + (size_t)k0 + (size_t)n * (k0 + kb);
         double *C0 = U + (size_t)(k0 + kb) + (size_t)n * (k0 + kb);
         if ((rc = dgemm(s, 'T', 'N', rest, rest, kb, -1.0, X0, n, X0, n, 1.0, C0, n, 2))) return rc;
@@ -859,7 +969,9 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
       if (j0 > 0) {
         // T = W(0:j0,0:j0) U(0:j0, j0:j0+kb); W(0:j0, j0:) = -T W_jj
         double *T = X.p;
-        if ((rc = dgemm(s, 'N', 'N', j0, kb, j0, 1.0, W, n, U + (size_t)n * j0, n, 0.0, T, n, 0, 1, 0))) return rc;
+        if ((rc = dgemm(s, 'N', 'N', j0, kb, j0, 1.0, W, n, U + (size_t)n * j0, n, 0.0, T, n, 0, 1, 0, c.w_gk.p,
+                        c.w_gk.n)))
+          return rc;
         if ((rc = dgemm(s, 'N', 'N', j0, kb, kb, -1.0, T, n, W + (size_t)j0 + (size_t)n * j0, n, 0.0,
                         W + (size_t)n * j0, n, 0, 0, 1)))
           return rc;
@@ -883,7 +995,7 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
       g.p0 = p0;
       g.nb = nb;
       // reflector of the panel's first column (already updated by earlier panels)
-      hipLaunchKernelGGL(k_tri_col, dim3((n - p0 + 255) / 256), dim3(256), 0, s, g, -1, p0, 0, 0);
+      hipLaunchKernelGGL(k_tri_col, dim3((n - p0 + 255) / 256), dim3(256), 0, s, g, -1, p0, 0, 0, 0);
       W2K_STAGE(s, "k_tri_col");
       int nref = 0;  // panel columns that carry a reflector
       for (int k = 0; k < nb; k++) {
@@ -891,11 +1003,11 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
         if (i >= n - 1) break;
         nref = k + 1;
         const int rows = n - (i + 1);
-        const int nt = (rows + SYMV_ROWS - 1) / SYMV_ROWS, nch = (rows + SYMV_COLS - 1) / SYMV_COLS;
-        hipLaunchKernelGGL(k_tri_symv, dim3(nt, nch), dim3(SYMV_ROWS), 0, s, g, i, nch);
+        const int nt = (rows + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
+        hipLaunchKernelGGL(k_tri_symv, dim3(nblk), dim3(SYMV_T), 0, s, g, i, nt);
         W2K_STAGE(s, "k_tri_symv");
         const int next = (k + 1 < nb) ? i + 1 : -1;
-        hipLaunchKernelGGL(k_tri_col, dim3((rows + 255) / 256), dim3(256), 0, s, g, i, next, nt, nch);
+        hipLaunchKernelGGL(k_tri_col, dim3((rows + 255) / 256), dim3(256), 0, s, g, i, next, nt, nt, nblk);
         W2K_STAGE(s, "k_tri_col");
       }
       if (nref > 0) {
@@ -934,7 +1046,7 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
     return 0;
   }
   double *w = wv;
-  hipLaunchKernelGGL(k_bisect, dim3((m + 63) / 64), dim3(64), 0, s, b, m, w);
+  hipLaunchKernelGGL(k_bisect, dim3((m + 3) / 4), dim3(256), 0, s, b, m, w);
   W2K_STAGE(s, "k_bisect");
   // ---- 5. inverse iteration + cluster orthogonalisation
   hipLaunchKernelGGL(k_stein_prep, dim3(1), dim3(64), 0, s, b, m, w, xs, cstart);
@@ -945,7 +1057,12 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((zc + 255) / 256)), dim3(256), 0, s, zc, Zt, 0.0);
     W2K_STAGE(s, "k_fill");
     if ((rc = c.w_fac.alloc((size_t)5 * n * m)) || (rc = c.w_piv.alloc((size_t)n * m))) return rc;
-    hipLaunchKernelGGL(k_stein, dim3((m + 63) / 64), dim3(64), 0, s, b, m, xs, Zt, n, c.w_fac.p, c.w_piv.p);
+    {
+      const size_t nm = (size_t)n * m;
+      double *f = c.w_fac.p;
+      hipLaunchKernelGGL(k_stein, dim3((m + 63) / 64), dim3(64), 0, s, b, m, xs, Zt, n, f, f + nm, f + 2 * nm,
+                         f + 3 * nm, f + 4 * nm, c.w_piv.p);
+    }
     W2K_STAGE(s, "k_stein");
     std::vector<int> cs(m);
     W2K_HIP(hipMemcpyAsync(cs.data(), cstart, sizeof(int) * m, hipMemcpyDeviceToHost, s));
@@ -957,7 +1074,6 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
     // is not numerically definite falls back to modified Gram-Schmidt
     if ((rc = c.w_bt.alloc((size_t)(n + 2 * m + 2 * TNB) * TNB))) return rc;
     if ((rc = c.w_cl.alloc((size_t)2 * n * TNB + 2 * TNB * TNB + (size_t)m * TNB))) return rc;
-    if ((rc = c.w_gk.alloc((size_t)1 << 22))) return rc;
     double *csave = c.w_cl.p, *ctmp = csave + (size_t)n * TNB, *cg = ctmp + (size_t)n * TNB;
     double *crinv = cg + TNB * TNB, *cy = crinv + TNB * TNB;
     int *cfail = ticket + 1;
