@@ -32,13 +32,9 @@
 #include <utility>
 #include <vector>
 
-#include "w2k_internal.h"
+#include "seclr4_shared.h"
 
 namespace w2k {
-
-constexpr int NB = 64;    // Cholesky / inversion block
-constexpr int TNB = 32;   // tridiagonalisation panel
-constexpr int SYMV_T = 64;  // tile of the symmetric SYMV (one wave)
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ double block_sum(double v, double *red)
@@ -212,11 +208,6 @@ struct TriWork {
   int *counter;       // last-block ticket (zero between uses)
 };
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
 
 // y = A(r0:n, r0:n) v from the tiles on and below the diagonal only (A is
 // symmetric): one wave per 64 x 64 tile (I, J), J <= I, adds A_IJ v_J to the
@@ -889,41 +880,184 @@ __global__ void k_gather_cols(int n, int m, const int *__restrict__ perm, const 
 }
 
 // ------------------------------------------------------------------ driver
-
-int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
+int seclr4_buffers(Ctx &c, int n, int cplx, Seclr4Bufs &B)
 {
-  *ne_out = 0;
-  *nbelw_out = 0;
-  if (c.run.is_complex) return W2K_EINVAL;
+  int rc;
+  const int nt = (n + SYMV_T - 1) / SYMV_T;
+  const size_t z = cplx ? 2 : 1;
+  const size_t nts = (size_t)nt;
+  const size_t small = (size_t)12 * n + z * 2 * n + z * 3 * (size_t)n * TNB + z * nts * n + z * nts * 2 * TNB +
+                       nts * (nts + 1) / 2 + nts + 64;
+  if ((rc = c.w_small.alloc(small))) return rc;
+  if ((rc = c.w_int.alloc((size_t)6 * n + 32))) return rc;
+  B.n = n;
+  B.nt = nt;
+  B.cplx = cplx;
+  double *p = c.w_small.p;
+  auto take = [&](size_t cnt) { double *q = p; p += cnt; return q; };
+  B.d = take(n); B.e = take(n); B.e2 = take(n);
+  B.tau = take(z * n); B.scal = take(z * n);
+  B.wv = take(n);
+  B.P = take(z * 3 * (size_t)n * TNB);
+  B.part = take(z * nts * n);
+  B.tdots = take(z * nts * 2 * TNB);
+  B.tvy = take(nts * (nts + 1) / 2);
+  B.tnorm = take(nts);
+  B.bgl = take(n); B.bgu = take(n); B.batol = take(n); B.bnrm = take(n); B.xs = take(n);
+  B.pivmin = take(1);
+  // ints: [0,4) INFO, [4,12) counts, [12,16) grid-reduction tickets, then task arrays
+  B.info = c.w_int.p;
+  B.cnt = B.info + 4;
+  B.ticket = B.info + 12;
+  B.tblk = B.info + 16;
+  B.tidx = B.tblk + n;
+  B.bstart = B.tidx + n;
+  B.bend = B.bstart + n;
+  B.cstart = B.bend + n;
+  B.perm = B.cstart + n;
+  W2K_HIP(hipMemsetAsync(c.w_int.p, 0, sizeof(int) * 16, c.stream));
+  return 0;
+}
+
+int tridiag_eigen(Ctx &c, Seclr4Bufs &B, double el, double eu, int nume, double *Zt, int *m_out, int *nbelw_out)
+{
+  const int n = B.n;
+  hipStream_t s = c.stream;
+  int rc;
+  *m_out = 0;
+  // ---- bisection
+  BisArgs b;
+  b.n = n; b.d = B.d; b.e = B.e; b.e2 = B.e2; b.el = el; b.eu = eu; b.nume = nume;
+  b.cnt = B.cnt; b.tblk = B.tblk; b.tidx = B.tidx; b.bstart = B.bstart; b.bend = B.bend;
+  b.bgl = B.bgl; b.bgu = B.bgu; b.batol = B.batol; b.bnrm = B.bnrm; b.pivmin = B.pivmin;
+  hipLaunchKernelGGL(k_bisect_prep, dim3(1), dim3(64), 0, s, b);
+  W2K_STAGE(s, "k_bisect_prep");
+  int hc[4];
+  W2K_HIP(hipMemcpyAsync(hc, B.cnt, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+  W2K_HIP(hipStreamSynchronize(s));
+  const int m = hc[0];
+  *nbelw_out = hc[1];
+  if ((rc = c.d_eig.alloc(m > 0 ? m : 1))) return rc;
+  if (m == 0) return 0;
+  double *w = B.wv;
+  hipLaunchKernelGGL(k_bisect, dim3((m + 3) / 4), dim3(256), 0, s, b, m, w);
+  W2K_STAGE(s, "k_bisect");
+  // ---- inverse iteration + cluster orthogonalisation
+  hipLaunchKernelGGL(k_stein_prep, dim3(1), dim3(64), 0, s, b, m, w, B.xs, B.cstart);
+  W2K_STAGE(s, "k_stein_prep");
+  const size_t zc = (size_t)n * m;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((zc + 255) / 256)), dim3(256), 0, s, zc, Zt, 0.0);
+  W2K_STAGE(s, "k_fill");
+  if ((rc = c.w_fac.alloc((size_t)5 * n * m)) || (rc = c.w_piv.alloc((size_t)n * m))) return rc;
+  {
+    double *f = c.w_fac.p;
+    hipLaunchKernelGGL(k_stein, dim3((m + 63) / 64), dim3(64), 0, s, b, m, B.xs, Zt, n, f, f + zc, f + 2 * zc,
+                       f + 3 * zc, f + 4 * zc, c.w_piv.p);
+    W2K_STAGE(s, "k_stein");
+  }
+  std::vector<int> cs(m);
+  W2K_HIP(hipMemcpyAsync(cs.data(), B.cstart, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+  W2K_HIP(hipStreamSynchronize(s));
+  // clusters [t0, t1): chunks of 32 vectors, each orthogonalised against
+  // the cluster's earlier chunks (classical Gram-Schmidt, twice, on MFMA)
+  // and then within itself by CholQR2 (Z = Q R, R from the Cholesky of
+  // Z^T Z), which equals in-order Gram-Schmidt; a chunk whose Gram matrix
+  // is not numerically definite falls back to modified Gram-Schmidt
+  if ((rc = c.w_cl.alloc((size_t)2 * n * TNB + 2 * TNB * TNB + (size_t)m * TNB))) return rc;
+  double *csave = c.w_cl.p, *ctmp = csave + (size_t)n * TNB, *cg = ctmp + (size_t)n * TNB;
+  double *crinv = cg + TNB * TNB, *cy = crinv + TNB * TNB;
+  int *cfail = B.ticket + 1;
+  auto cgs2 = [&](int t0, int c0, int cw) -> int {
+    const int np = c0 - t0;
+    double *Q = Zt + (size_t)n * t0, *Zc = Zt + (size_t)n * c0;
+    for (int pass = 0; pass < 2 && np > 0; pass++) {
+      int r2;
+      if ((r2 = dgemm(s, 'T', 'N', np, cw, n, 1.0, Q, n, Zc, n, 0.0, cy, np, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
+        return r2;
+      if ((r2 = dgemm(s, 'N', 'N', n, cw, np, -1.0, Q, n, cy, np, 1.0, Zc, n))) return r2;
+    }
+    return 0;
+  };
+  int t0 = 0;
+  while (t0 < m) {
+    int t1 = t0 + 1;
+    while (t1 < m && cs[t1] == t0) t1++;
+    if (t1 - t0 > 1) {
+      for (int c0 = t0; c0 < t1; c0 += TNB) {
+        const int cw = std::min(TNB, t1 - c0);
+        double *Zc = Zt + (size_t)n * c0;
+        W2K_HIP(hipMemcpyAsync(csave, Zc, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
+        W2K_HIP(hipMemsetAsync(cfail, 0, sizeof(int), s));
+        if ((rc = cgs2(t0, c0, cw))) return rc;
+        for (int pass = 0; pass < 2; pass++) {
+          if ((rc = dgemm(s, 'T', 'N', cw, cw, n, 1.0, Zc, n, Zc, n, 0.0, cg, cw, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
+            return rc;
+          hipLaunchKernelGGL(k_chol_rinv, dim3(1), dim3(64), 0, s, cw, cg, crinv, cfail);
+          W2K_STAGE(s, "k_chol_rinv");
+          if ((rc = dgemm(s, 'N', 'N', n, cw, cw, 1.0, Zc, n, crinv, cw, 0.0, ctmp, n, 0, 0, 1))) return rc;
+          W2K_HIP(hipMemcpyAsync(Zc, ctmp, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
+        }
+        int hfail = 0;
+        W2K_HIP(hipMemcpyAsync(&hfail, cfail, sizeof(int), hipMemcpyDeviceToHost, s));
+        W2K_HIP(hipStreamSynchronize(s));
+        if (hfail) {
+          W2K_HIP(hipMemcpyAsync(Zc, csave, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
+          if ((rc = cgs2(t0, c0, cw))) return rc;
+          hipLaunchKernelGGL(k_mgs_chunk, dim3(1), dim3(256), 0, s, n, Zt, n, c0, cw);
+          W2K_STAGE(s, "k_mgs_chunk");
+        }
+      }
+    }
+    t0 = t1;
+  }
+  hipLaunchKernelGGL(k_normsign, dim3(m), dim3(256), 0, s, n, Zt, n);
+  W2K_STAGE(s, "k_normsign");
+  W2K_HIP(hipStreamSynchronize(s));
+  *m_out = m;
+  return 0;
+}
+
+int sort_eigen(Ctx &c, Seclr4Bufs &B, int m)
+{
+  hipStream_t s = c.stream;
+  std::vector<double> hw(m);
+  std::vector<int> hp(m);
+  W2K_HIP(hipMemcpyAsync(hw.data(), B.wv, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+  W2K_HIP(hipStreamSynchronize(s));
+  for (int j = 0; j < m; j++) hp[j] = j;
+  for (int j = 0; j < m - 1; j++) {
+    int ii = -1;
+    double tmin = hw[j];
+    for (int jj = j + 1; jj < m; jj++)
+      if (hw[jj] < tmin) { ii = jj; tmin = hw[jj]; }
+    if (ii >= 0) {
+      hw[ii] = hw[j];
+      hw[j] = tmin;
+      std::swap(hp[ii], hp[j]);
+    }
+  }
+  W2K_HIP(hipMemcpyAsync(B.perm, hp.data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
+  W2K_HIP(hipMemcpyAsync(B.wv, hw.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+  W2K_HIP(hipStreamSynchronize(s));  // the host vectors above go out of scope
+  return 0;
+}
+
+static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
+{
   const int n = c.n;
   hipStream_t s = c.stream;
   int rc;
-  if (el >= eu) { el = -1.0e70; eu = 1.0e70; }  // seclr4.F:149-152
-  if (nume <= 0) nume = n;
   const size_t nn = (size_t)n * n;
-  // workspaces: w_a = S/U, w_b = H full then C, w_c = W = U^-1 then X
+  // workspaces: w_a = S/U, w_b = H full then C, w_c = W = U^-1, w_x = X
   if ((rc = c.w_a.alloc(nn)) || (rc = c.w_b.alloc(nn)) || (rc = c.w_c.alloc(nn))) return rc;
   double *U = c.w_a.p, *Hf = c.w_b.p, *W = c.w_c.p;
   DBuf<double> &X = c.w_x;
   if ((rc = X.alloc(nn))) return rc;
   if ((rc = c.w_gk.alloc((size_t)1 << 22))) return rc;  // split-K partial products
-  // small buffers: info, d, e, e2, tau, panel V/W, partials, task arrays
-  const int nchunk = (n + SYMV_T - 1) / SYMV_T;  // SYMV tiles per dimension (at most)
-  const size_t ntile = (size_t)nchunk;
-  const size_t small = (size_t)14 * n + (size_t)3 * n * TNB + (size_t)nchunk * n + ntile * 2 * TNB +
-                       ntile * (ntile + 1) / 2 + ntile + 64;
-  if ((rc = c.w_small.alloc(small))) return rc;
-  double *d = c.w_small.p, *e = d + n, *e2 = e + n, *tau = e2 + n, *wv = tau + n, *scal = wv + n;
-  double *P = scal + n, *part = P + (size_t)3 * n * TNB;
-  double *bgl = part + (size_t)nchunk * n, *bgu = bgl + n, *batol = bgu + n, *bnrm = batol + n;
-  double *xs = bnrm + n, *tdots = xs + n, *tvy = tdots + ntile * 2 * TNB, *tnorm = tvy + ntile * (ntile + 1) / 2;
-  double *pivmin = tnorm + ntile;
-  if ((rc = c.w_int.alloc((size_t)6 * n + 32))) return rc;
-  // ints: [0,4) INFO, [4,12) counts, [12,16) grid-reduction tickets, then task arrays
-  int *info = c.w_int.p, *cnt = info + 4, *ticket = info + 12, *tblk = info + 16, *tidx = tblk + n;
-  int *bstart = tidx + n, *bend = bstart + n;
-  int *cstart = bend + n, *perm = cstart + n;
-  W2K_HIP(hipMemsetAsync(c.w_int.p, 0, sizeof(int) * 16, s));
+  Seclr4Bufs B;
+  if ((rc = seclr4_buffers(c, n, 0, B))) return rc;
+  double *d = B.d, *e = B.e, *tau = B.tau, *scal = B.scal, *P = B.P;
+  int *info = B.info;
 
   W2K_HIP(hipEventRecord(c.ev[6], s));
   // ---- 1. Cholesky
@@ -947,9 +1081,8 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
           return rc;
         W2K_HIP(hipMemcpy2DAsync(S12, sizeof(double) * n, X.p, sizeof(double) * kb, sizeof(double) * kb, rest,
                                  hipMemcpyDeviceToDevice, s));
-        double *X0 = U + (size_t)k0 + (size_t)n * (k0 + kb);
         double *C0 = U + (size_t)(k0 + kb) + (size_t)n * (k0 + kb);
-        if ((rc = dgemm(s, 'T', 'N', rest, rest, kb, -1.0, X0, n, X0, n, 1.0, C0, n, 2))) return rc;
+        if ((rc = dgemm(s, 'T', 'N', rest, rest, kb, -1.0, S12, n, S12, n, 1.0, C0, n, 2))) return rc;
       }
     }
   }
@@ -986,10 +1119,18 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
   W2K_HIP(hipEventRecord(c.ev[8], s));
   // ---- 3. tridiagonalisation of C (in Hf)
   double *A = Hf;
+  int nsymv = 0;
   {
+    // optional per-launch events around the SYMV (kernel-level roofline)
+    const bool kt = c.ktime != 0;
+    if (kt && c.kev.size() < (size_t)2 * n) {
+      const size_t old = c.kev.size();
+      c.kev.resize((size_t)2 * n);
+      for (size_t q = old; q < c.kev.size(); q++) W2K_HIP(hipEventCreate(&c.kev[q]));
+    }
     TriWork g;
     g.n = n; g.a = A; g.P = P; g.d = d; g.e = e; g.tau = tau; g.scal = scal;
-    g.part = part; g.dots = tdots; g.vy = tvy; g.norm = tnorm; g.counter = ticket;
+    g.part = B.part; g.dots = B.tdots; g.vy = B.tvy; g.norm = B.tnorm; g.counter = B.ticket;
     for (int p0 = 0; p0 < n; p0 += TNB) {
       const int nb = std::min(TNB, n - p0);
       g.p0 = p0;
@@ -1004,8 +1145,11 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
         nref = k + 1;
         const int rows = n - (i + 1);
         const int nt = (rows + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
+        if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv], s));
         hipLaunchKernelGGL(k_tri_symv, dim3(nblk), dim3(SYMV_T), 0, s, g, i, nt);
         W2K_STAGE(s, "k_tri_symv");
+        if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv + 1], s));
+        nsymv++;
         const int next = (k + 1 < nb) ? i + 1 : -1;
         hipLaunchKernelGGL(k_tri_col, dim3((rows + 255) / 256), dim3(256), 0, s, g, i, next, nt, nt, nblk);
         W2K_STAGE(s, "k_tri_col");
@@ -1025,113 +1169,39 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
     }
   }
   W2K_HIP(hipEventRecord(c.ev[9], s));
-  // ---- 4. bisection
-  BisArgs b;
-  b.n = n; b.d = d; b.e = e; b.e2 = e2; b.el = el; b.eu = eu; b.nume = nume;
-  b.cnt = cnt; b.tblk = tblk; b.tidx = tidx; b.bstart = bstart; b.bend = bend;
-  b.bgl = bgl; b.bgu = bgu; b.batol = batol; b.bnrm = bnrm; b.pivmin = pivmin;
-  hipLaunchKernelGGL(k_bisect_prep, dim3(1), dim3(64), 0, s, b);
-  W2K_STAGE(s, "k_bisect_prep");
-  int hc[4];
-  W2K_HIP(hipMemcpyAsync(hc, cnt, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
-  W2K_HIP(hipStreamSynchronize(s));
-  const int m = hc[0];
-  *nbelw_out = hc[1];
-  c.nbelw = hc[1];
-  if ((rc = c.d_eig.alloc(m > 0 ? m : 1)) || (rc = c.d_z.alloc((size_t)n * (m > 0 ? m : 1)))) return rc;
-  if (m == 0) {
-    c.ne = 0;
-    W2K_HIP(hipEventRecord(c.ev[10], s));
-    W2K_HIP(hipEventRecord(c.ev[11], s));
-    return 0;
+  if (c.ktime) {
+    // [10] summed SYMV kernel time, [11] number of SYMV launches
+    W2K_HIP(hipStreamSynchronize(s));
+    double tot = 0.0;
+    int cntl = 0;
+    for (int q = 0; q < nsymv; q++) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, c.kev[2 * q], c.kev[2 * q + 1]) != hipSuccess) {
+        (void)hipGetLastError();
+        break;
+      }
+      tot += ms;
+      cntl++;
+    }
+    c.last_ms[10] = tot;
+    c.last_ms[11] = cntl;
   }
-  double *w = wv;
-  hipLaunchKernelGGL(k_bisect, dim3((m + 3) / 4), dim3(256), 0, s, b, m, w);
-  W2K_STAGE(s, "k_bisect");
-  // ---- 5. inverse iteration + cluster orthogonalisation
-  hipLaunchKernelGGL(k_stein_prep, dim3(1), dim3(64), 0, s, b, m, w, xs, cstart);
-  W2K_STAGE(s, "k_stein_prep");
+  // ---- 4./5. bisection, inverse iteration, cluster orthogonalisation
   double *Zt = X.p;  // n x m
-  {
-    const size_t zc = (size_t)n * m;
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)((zc + 255) / 256)), dim3(256), 0, s, zc, Zt, 0.0);
-    W2K_STAGE(s, "k_fill");
-    if ((rc = c.w_fac.alloc((size_t)5 * n * m)) || (rc = c.w_piv.alloc((size_t)n * m))) return rc;
-    {
-      const size_t nm = (size_t)n * m;
-      double *f = c.w_fac.p;
-      hipLaunchKernelGGL(k_stein, dim3((m + 63) / 64), dim3(64), 0, s, b, m, xs, Zt, n, f, f + nm, f + 2 * nm,
-                         f + 3 * nm, f + 4 * nm, c.w_piv.p);
-    }
-    W2K_STAGE(s, "k_stein");
-    std::vector<int> cs(m);
-    W2K_HIP(hipMemcpyAsync(cs.data(), cstart, sizeof(int) * m, hipMemcpyDeviceToHost, s));
-    W2K_HIP(hipStreamSynchronize(s));
-    // clusters [t0, t1): chunks of 32 vectors, each orthogonalised against
-    // the cluster's earlier chunks (classical Gram-Schmidt, twice, on MFMA)
-    // and then within itself by CholQR2 (Z = Q R, R from the Cholesky of
-    // Z^T Z), which equals in-order Gram-Schmidt; a chunk whose Gram matrix
-    // is not numerically definite falls back to modified Gram-Schmidt
-    if ((rc = c.w_bt.alloc((size_t)(n + 2 * m + 2 * TNB) * TNB))) return rc;
-    if ((rc = c.w_cl.alloc((size_t)2 * n * TNB + 2 * TNB * TNB + (size_t)m * TNB))) return rc;
-    double *csave = c.w_cl.p, *ctmp = csave + (size_t)n * TNB, *cg = ctmp + (size_t)n * TNB;
-    double *crinv = cg + TNB * TNB, *cy = crinv + TNB * TNB;
-    int *cfail = ticket + 1;
-    auto cgs2 = [&](int t0, int c0, int cw) -> int {
-      const int np = c0 - t0;
-      double *Q = Zt + (size_t)n * t0, *Zc = Zt + (size_t)n * c0;
-      for (int pass = 0; pass < 2 && np > 0; pass++) {
-        int r2;
-        if ((r2 = dgemm(s, 'T', 'N', np, cw, n, 1.0, Q, n, Zc, n, 0.0, cy, np, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
-          return r2;
-        if ((r2 = dgemm(s, 'N', 'N', n, cw, np, -1.0, Q, n, cy, np, 1.0, Zc, n))) return r2;
-      }
-      return 0;
-    };
-    int t0 = 0;
-    while (t0 < m) {
-      int t1 = t0 + 1;
-      while (t1 < m && cs[t1] == t0) t1++;
-      if (t1 - t0 > 1) {
-        for (int c0 = t0; c0 < t1; c0 += TNB) {
-          const int cw = std::min(TNB, t1 - c0);
-          double *Zc = Zt + (size_t)n * c0;
-          W2K_HIP(hipMemcpyAsync(csave, Zc, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
-          W2K_HIP(hipMemsetAsync(cfail, 0, sizeof(int), s));
-          if ((rc = cgs2(t0, c0, cw))) return rc;
-          for (int pass = 0; pass < 2; pass++) {
-            if ((rc = dgemm(s, 'T', 'N', cw, cw, n, 1.0, Zc, n, Zc, n, 0.0, cg, cw, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
-              return rc;
-            hipLaunchKernelGGL(k_chol_rinv, dim3(1), dim3(64), 0, s, cw, cg, crinv, cfail);
-            W2K_STAGE(s, "k_chol_rinv");
-            if ((rc = dgemm(s, 'N', 'N', n, cw, cw, 1.0, Zc, n, crinv, cw, 0.0, ctmp, n, 0, 0, 1))) return rc;
-            W2K_HIP(hipMemcpyAsync(Zc, ctmp, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
-          }
-          int hfail = 0;
-          W2K_HIP(hipMemcpyAsync(&hfail, cfail, sizeof(int), hipMemcpyDeviceToHost, s));
-          W2K_HIP(hipStreamSynchronize(s));
-          if (hfail) {
-            W2K_HIP(hipMemcpyAsync(Zc, csave, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
-            if ((rc = cgs2(t0, c0, cw))) return rc;
-            hipLaunchKernelGGL(k_mgs_chunk, dim3(1), dim3(256), 0, s, n, Zt, n, c0, cw);
-            W2K_STAGE(s, "k_mgs_chunk");
-          }
-        }
-      }
-      t0 = t1;
-    }
-    hipLaunchKernelGGL(k_normsign, dim3(m), dim3(256), 0, s, n, Zt, n);
-    W2K_STAGE(s, "k_normsign");
-    W2K_HIP(hipStreamSynchronize(s));
-  }
+  int m = 0, nbelw = 0;
+  if ((rc = tridiag_eigen(c, B, el, eu, nume, Zt, &m, &nbelw))) return rc;
+  *nbelw_out = nbelw;
+  c.nbelw = nbelw;
+  if ((rc = c.d_z.alloc((size_t)n * (m > 0 ? m : 1)))) return rc;
   W2K_HIP(hipEventRecord(c.ev[10], s));
-  // ---- 6. Z = Q Zt (reflectors of step 3, last panel first), sort, Z = W Z
-  {
+  if (m > 0) {
+    // ---- 6. Z = Q Zt (reflectors of step 3, last panel first), sort, Z = W Z
+    if ((rc = c.w_bt.alloc((size_t)(n + 2 * m + 2 * TNB) * TNB))) return rc;
     // w_bt: V panel (n x TNB), V^T V and T (TNB x TNB each), Y1 and Y2 (TNB x m each)
     double *vb = c.w_bt.p, *gm = vb + (size_t)n * TNB, *tm = gm + TNB * TNB, *y1 = tm + TNB * TNB;
     double *y2 = y1 + (size_t)TNB * m;
     const int nref = n - 1;  // reflectors 0..n-2
-    const int last = ((nref - 1) / TNB) * TNB;
+    const int last = nref > 0 ? ((nref - 1) / TNB) * TNB : -1;
     for (int p0 = last; p0 >= 0; p0 -= TNB) {
       const int nb = std::min(TNB, nref - p0);
       if (nb <= 0) continue;
@@ -1146,31 +1216,12 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
       if ((rc = dgemm(s, 'N', 'N', nb, m, nb, 1.0, tm, nb, y1, nb, 0.0, y2, nb, 0, 1, 0))) return rc;
       if ((rc = dgemm(s, 'N', 'N', n, m, nb, -1.0, vb, n, y2, nb, 1.0, Zt, n))) return rc;
     }
-    // selection sort of dsyevx2.f:381-400 on the host (M values), as a permutation
-    std::vector<double> hw(m);
-    std::vector<int> hp(m);
-    W2K_HIP(hipMemcpyAsync(hw.data(), w, sizeof(double) * m, hipMemcpyDeviceToHost, s));
-    W2K_HIP(hipStreamSynchronize(s));
-    for (int j = 0; j < m; j++) hp[j] = j;
-    for (int j = 0; j < m - 1; j++) {
-      int ii = -1;
-      double tmin = hw[j];
-      for (int jj = j + 1; jj < m; jj++)
-        if (hw[jj] < tmin) { ii = jj; tmin = hw[jj]; }
-      if (ii >= 0) {
-        hw[ii] = hw[j];
-        hw[j] = tmin;
-        std::swap(hp[ii], hp[j]);
-      }
-    }
-    W2K_HIP(hipMemcpyAsync(perm, hp.data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
-    W2K_HIP(hipMemcpyAsync(w, hw.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    if ((rc = sort_eigen(c, B, m))) return rc;
     double *Zs = U;  // U is not needed after the inversion
-    hipLaunchKernelGGL(k_gather_cols, dim3((n + 255) / 256, m), dim3(256), 0, s, n, m, perm, Zt, Zs);
+    hipLaunchKernelGGL(k_gather_cols, dim3((n + 255) / 256, m), dim3(256), 0, s, n, m, B.perm, Zt, Zs);
     W2K_STAGE(s, "k_gather_cols");
     if ((rc = dgemm(s, 'N', 'N', n, m, n, 1.0, W, n, Zs, n, 0.0, c.d_z.p, n, 0, 1, 0))) return rc;
-    W2K_HIP(hipMemcpyAsync(c.d_eig.p, w, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
-    W2K_HIP(hipStreamSynchronize(s));
+    W2K_HIP(hipMemcpyAsync(c.d_eig.p, B.wv, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
   }
   W2K_HIP(hipEventRecord(c.ev[11], s));
   W2K_HIP(hipStreamSynchronize(s));
@@ -1182,6 +1233,17 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
   c.ne = m;
   *ne_out = m;
   return 0;
+}
+
+int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
+{
+  *ne_out = 0;
+  *nbelw_out = 0;
+  if (el >= eu) { el = -1.0e70; eu = 1.0e70; }  // seclr4.F:149-152
+  if (nume <= 0) nume = c.n;
+  c.ne = 0;
+  if (c.run.is_complex) return seclr4_complex(c, el, eu, nume, ne_out, nbelw_out);
+  return seclr4_real(c, el, eu, nume, ne_out, nbelw_out);
 }
 
 }  // namespace w2k

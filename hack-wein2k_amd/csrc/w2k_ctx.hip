@@ -356,6 +356,13 @@ const char *w2k_version(void) { return "w2k-mi355x 0.1 (gfx950)"; }
 
 const char *w2k_last_error(void) { return g_last_error; }
 
+int w2k_set_kernel_timing(w2k_ctx *x, int on)
+{
+  if (!x) return W2K_EINVAL;
+  x->c.ktime = on ? 1 : 0;
+  return 0;
+}
+
 int w2k_create(w2k_ctx **out, int device, const w2k_run_desc *run)
 {
   if (!out || !run) return W2K_EINVAL;
@@ -395,6 +402,7 @@ void w2k_destroy(w2k_ctx *x)
   (void)hipSetDevice(x->c.device);
   if (x->c.stream) (void)hipStreamSynchronize(x->c.stream);
   for (int k = 0; k < W2K_NEVENTS; k++) (void)hipEventDestroy(x->c.ev[k]);
+  for (hipEvent_t e : x->c.kev) (void)hipEventDestroy(e);
   if (x->c.stream) (void)hipStreamDestroy(x->c.stream);
   x->c.stream = nullptr;
   delete x;

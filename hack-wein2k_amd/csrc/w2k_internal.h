@@ -12,7 +12,7 @@
 #define W2K_LOMAXP (W2K_LOMAX + 1)
 #define W2K_NSL2 (W2K_NSLMAX * W2K_NSLMAX)
 #define W2K_NEVENTS 16
-#define W2K_NTIMINGS 10
+#define W2K_NTIMINGS 12
 
 namespace w2k {
 
@@ -124,6 +124,9 @@ struct Ctx {
   // timing
   hipEvent_t ev[W2K_NEVENTS];
   double last_ms[W2K_NTIMINGS] = {0};
+  // per-launch timing of the tridiagonalisation SYMV (off unless requested)
+  int ktime = 0;
+  std::vector<hipEvent_t> kev;
 };
 
 // kernels / stages (implemented in the .hip files)

@@ -199,6 +199,7 @@ def load_library(path: Optional[str] = None):
     lib.w2k_last_timings.argtypes = [V, _c_dbl_p, ctypes.c_int]
     lib.w2k_version.restype = ctypes.c_char_p
     lib.w2k_last_error.restype = ctypes.c_char_p
+    lib.w2k_set_kernel_timing.argtypes = [V, ctypes.c_int]
     if path is None:
         _LIB = lib
     return lib
@@ -298,7 +299,11 @@ class Lapw1:
         return ev[:ne], z[:, :ne]
 
     TIMING_NAMES = ("hamilt", "hns", "seclr4", "cholesky", "reduction", "tridiag", "bisect_invit",
-                    "backtransform", "hamilt_pw_kernel", "hns_gemm")
+                    "backtransform", "hamilt_pw_kernel", "hns_gemm", "tri_symv_total", "tri_symv_launches")
+
+    def set_kernel_timing(self, on: bool):
+        """Per-launch events around the tridiagonalisation SYMV kernels."""
+        self._chk("w2k_set_kernel_timing", self.lib.w2k_set_kernel_timing(self.ctx, int(bool(on))))
 
     def timings(self) -> np.ndarray:
         """Device milliseconds of the last calls (order of TIMING_NAMES)."""

@@ -151,9 +151,16 @@ int w2k_sync(w2k_ctx *ctx);
  * events on the context stream: [0] HAMILT, [1] HNS, [2] SECLR4 total,
  * [3] Cholesky, [4] reduction to standard form, [5] tridiagonalisation,
  * [6] bisection + inverse iteration, [7] back-transforms, [8] the HAMILT
- * plane-wave pair kernel alone, [9] the HNS rank-K GEMM alone.  At most
- * n values are written. */
+ * plane-wave pair kernel alone, [9] the HNS rank-K GEMM alone, [10] the
+ * summed time of the tridiagonalisation SYMV kernels and [11] their number of
+ * launches (both only while per-launch timing is on).  At most n values are
+ * written. */
 int w2k_last_timings(w2k_ctx *ctx, double *ms, int n);
+
+/* Per-launch HIP events around the tridiagonalisation SYMV kernels (slots
+ * [10], [11] of w2k_last_timings); off by default since the events add a
+ * little time between launches.  No counterpart in the reference. */
+int w2k_set_kernel_timing(w2k_ctx *ctx, int on);
 
 /* Library version string. */
 const char *w2k_version(void);
