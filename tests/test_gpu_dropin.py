@@ -3,7 +3,8 @@ SRC_lapw1, oracle/Makefile.ref) linked with hack-wein2k_amd/fortran/
 w2k_dropin.F90 in place of hamilt.F, hns.F and seclr4.F, so that CALKPT
 (calkpt.F:143-197) reaches the MI355X library through ISO_C_BINDING.
 
-Each case writes the seeded input deck, runs oracle/_ref/lapw1_gpu_real and
+Each case writes the seeded input deck, runs oracle/_ref/lapw1_gpu_real (or
+lapw1_gpu_complex for the complex build) and
 compares NE, NBELW and the eigenvalues of every k-point with the reference
 run stored in tests/golden (bar: 1e-10 Ry)."""
 from __future__ import annotations
@@ -19,7 +20,8 @@ import pytest
 from conftest import GOLDEN, REPO
 
 pytestmark = pytest.mark.gpu
-EXE = os.path.join(REPO, "oracle", "_ref", "lapw1_gpu_real")
+EXE = {False: os.path.join(REPO, "oracle", "_ref", "lapw1_gpu_real"),
+       True: os.path.join(REPO, "oracle", "_ref", "lapw1_gpu_complex")}
 
 sys.path.insert(0, GOLDEN)
 import w2k_inputs as W  # noqa: E402
@@ -30,15 +32,17 @@ def _unlimited_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
-@pytest.mark.skipif(not os.path.exists(EXE), reason="lapw1_gpu_real not built (make -f oracle/Makefile.ref)")
-@pytest.mark.parametrize("name,nk", [("rs8", 1), ("c1_fcc", 2)])
+@pytest.mark.parametrize("name,nk", [("rs8", 1), ("c1_fcc", 2), ("lo_cplx", 1)])
 def test_reference_program_with_dropin(name, nk, tmp_path):
     fx = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    exe = EXE[bool(int(fx["meta/complex"]))]
+    if not os.path.exists(exe):
+        pytest.skip(f"{os.path.basename(exe)} not built (make -f oracle/Makefile.ref)")
     case = W.CASES[name]()
     case.window = (float(fx["k0/EL"]), float(fx["k0/EU"]))
     case.kpoints = case.kpoints[:nk]
     W.write_case(case, str(tmp_path))
-    r = subprocess.run([EXE, "eigonly"], cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+    r = subprocess.run([exe, "eigonly"], cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                        preexec_fn=_unlimited_stack, timeout=300)
     assert r.returncode == 0, r.stdout.decode(errors="replace")[-2000:]
     for i in range(nk):

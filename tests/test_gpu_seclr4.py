@@ -16,10 +16,19 @@ pytestmark = pytest.mark.gpu
 L = PKG.lapw1
 
 REAL_FULL = ["c1_fcc", "mgo", "lo_mult2", "rs8"]
+FULL = REAL_FULL + ["lo_cplx"]
 
 
 def _unpack(hs, hd):
-    """H and S of the packed HS/HSDIAG as CALKPT leaves it (real build)."""
+    """The pencil SECLR4 solves from HS/HSDIAG as CALKPT leaves it: H from the
+    lower triangle (stored conjugated in the complex build), S from the strict
+    upper triangle and HSDIAG (seclr4.F:159-166; ZHCGST reduces with U^T)."""
+    if np.iscomplexobj(hs):
+        Hl = np.tril(hs, -1).conj()
+        H = Hl + Hl.conj().T + np.diag(np.diag(hs).real)
+        Su = np.triu(hs, 1)
+        S = Su + Su.conj().T + np.diag(np.asarray(hd).real)
+        return H, S
     H = np.tril(hs) + np.tril(hs, -1).T
     S = np.triu(hs, 1) + np.triu(hs, 1).T + np.diag(hd)
     return H, S
@@ -35,15 +44,17 @@ def _groups(ev, tol):
 
 
 def _check_vectors(ev, z, zr, S):
-    """Columns of isolated eigenvalues up to sign, degenerate groups by projector."""
+    """Columns of isolated eigenvalues up to a unit phase, degenerate groups by
+    their S-projector."""
     for a, b in _groups(ev, 1e-6):
         if b - a == 1:
             v, r = z[:, a], zr[:, a]
-            sgn = np.sign(np.dot(v, r))
-            assert np.max(np.abs(v * sgn - r)) < 1e-8 * np.max(np.abs(r)), f"vector {a}"
+            ph = np.vdot(v, r)
+            ph = ph / abs(ph)
+            assert np.max(np.abs(v * ph - r)) < 1e-8 * np.max(np.abs(r)), f"vector {a}"
         else:
-            P = z[:, a:b] @ z[:, a:b].T @ S
-            Pr = zr[:, a:b] @ zr[:, a:b].T @ S
+            P = z[:, a:b] @ z[:, a:b].conj().T @ S
+            Pr = zr[:, a:b] @ zr[:, a:b].conj().T @ S
             assert np.max(np.abs(P - Pr)) < 1e-8 * max(1.0, np.max(np.abs(Pr))), f"group {a}:{b}"
 
 
@@ -52,7 +63,7 @@ def _basis_of_size(n):
     return L.Basis(n, 0, np.zeros((3, n), np.int32), z, z, z, z)
 
 
-@pytest.mark.parametrize("name", REAL_FULL)
+@pytest.mark.parametrize("name", FULL)
 def test_seclr4_on_reference_matrix(name):
     """SECLR4 alone on the reference's own post-HNS HS/HSDIAG."""
     fx = Fixture(name)
@@ -67,7 +78,7 @@ def test_seclr4_on_reference_matrix(name):
     assert np.max(np.abs(ev - kd["EIGVAL"])) < 1e-10
     H, S = _unpack(hs, kd["HSDIAG_HNS"])
     _check_vectors(ev, z, kd["Z"], S)
-    G = z.T @ S @ z
+    G = z.conj().T @ S @ z
     assert np.max(np.abs(G - np.eye(ne))) < 1e-10
     R = H @ z - (S @ z) * ev
     assert np.max(np.abs(R)) < 1e-9 * max(1.0, np.max(np.abs(H)))
@@ -84,6 +95,7 @@ _MULTI_K_LO = ("a device fault was seen once in this chain (LO basis, basis grow
     ("rs8", 1),
     ("lo_mult2", 1),
     ("mgo", 1),
+    ("lo_cplx", 1),
     pytest.param("mgo", 2, marks=pytest.mark.skip(reason=_MULTI_K_LO)),
     pytest.param("lo_mult2", 2, marks=pytest.mark.skip(reason=_MULTI_K_LO)),
 ])
@@ -105,21 +117,25 @@ def test_calkpt_chain_matches_reference(name, nk):
     ctx.close()
 
 
-def _pencil_with_spectrum(lam, seed):
+def _pencil_with_spectrum(lam, seed, cpx=False):
     """(H, S) with generalized eigenvalues lam exactly (degeneracies kept)."""
     n = len(lam)
     rng = np.random.default_rng(seed)
-    X = rng.standard_normal((n, n))
-    S = X @ X.T / n + np.eye(n)
+    X = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cpx else 0)
+    S = X @ X.conj().T / n + np.eye(n)
     Lc = np.linalg.cholesky(S)
-    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
-    C = (Q * lam) @ Q.T
-    H = Lc @ C @ Lc.T
-    H = (H + H.T) / 2
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cpx else 0))
+    C = (Q * lam) @ Q.conj().T
+    H = Lc @ C @ Lc.conj().T
+    H = (H + H.conj().T) / 2
     return H, S
 
 
 def _pack(H, S):
+    """HS/HSDIAG as CALKPT leaves them (complex: H conjugated in the lower triangle)."""
+    if np.iscomplexobj(H):
+        hs = np.triu(S, 1) + np.tril(H, -1).conj() + np.diag(np.diag(H))
+        return np.asfortranarray(hs), np.diag(S).copy()
     hs = np.triu(S, 1) + np.tril(H)
     return np.asfortranarray(hs), np.diag(S).copy()
 
@@ -127,7 +143,7 @@ def _pack(H, S):
 def _run_pencil(H, S, el, eu, nume):
     n = H.shape[0]
     hs, hd = _pack(H, S)
-    ctx = L.Lapw1(Fixture("c1_fcc").run())
+    ctx = L.Lapw1(Fixture("lo_cplx" if np.iscomplexobj(H) else "c1_fcc").run())
     ctx.set_basis(_basis_of_size(n))
     ctx.put_hs(hs, hd)
     ne, nb = ctx.seclr4(el, eu, nume)
@@ -160,6 +176,30 @@ def test_seclr4_random_pencils(n, window, nume):
         assert np.max(np.abs(R)) < 1e-10 * max(1.0, np.max(np.abs(H)))
 
 
+@pytest.mark.parametrize("n", [1, 2, 33, 65, 200])
+@pytest.mark.parametrize("window,nume", [((-0.5, 0.8), 100000), ((-1e9, 1e9), 7), ((1.0, 0.0), 100000)])
+def test_seclr4_random_complex_pencils(n, window, nume):
+    rng = np.random.default_rng(100 + n)
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    S = X @ X.conj().T / n + np.eye(n)
+    Y = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = (Y + Y.conj().T) / 2
+    w_all = sla.eigh(H, S, eigvals_only=True)
+    el, eu = window
+    ne, nb, ev, z = _run_pencil(H, S, el, eu, nume)
+    if el >= eu:
+        el, eu = -1e70, 1e70
+    sel = w_all[(w_all > el) & (w_all <= eu)][:nume]
+    assert ne == len(sel)
+    assert nb == int(np.sum(w_all <= el))
+    if ne:
+        assert np.max(np.abs(ev - sel)) < 1e-11 * max(1.0, np.max(np.abs(w_all)))
+        G = z.conj().T @ S @ z
+        assert np.max(np.abs(G - np.eye(ne))) < 1e-10
+        R = H @ z - (S @ z) * ev
+        assert np.max(np.abs(R)) < 1e-10 * max(1.0, np.max(np.abs(H)))
+
+
 @pytest.mark.parametrize("n", [50, 300])
 def test_seclr4_degenerate_clusters(n):
     """Exactly degenerate and tightly clustered eigenvalues: the vectors of a
@@ -169,14 +209,15 @@ def test_seclr4_degenerate_clusters(n):
     rest = rng.uniform(-2, 2, n - 3 * len(base) - 2)
     lam = np.sort(np.concatenate([base, base, base + 1e-9, rest, [0.5, 0.5 + 1e-5]]))
     assert len(lam) == n
-    H, S = _pencil_with_spectrum(lam, n)
-    ne, nb, ev, z = _run_pencil(H, S, -1e9, 1e9, n)
-    assert ne == n and nb == 0
-    assert np.max(np.abs(ev - lam)) < 1e-10
-    G = z.T @ S @ z
-    assert np.max(np.abs(G - np.eye(n))) < 1e-10
-    R = H @ z - (S @ z) * ev
-    assert np.max(np.abs(R)) < 1e-9
+    for cpx in (False, True):
+        H, S = _pencil_with_spectrum(lam, n, cpx)
+        ne, nb, ev, z = _run_pencil(H, S, -1e9, 1e9, n)
+        assert ne == n and nb == 0
+        assert np.max(np.abs(ev - lam)) < 1e-10
+        G = z.conj().T @ S @ z
+        assert np.max(np.abs(G - np.eye(n))) < 1e-10
+        R = H @ z - (S @ z) * ev
+        assert np.max(np.abs(R)) < 1e-9
 
 
 def test_seclr4_not_positive_definite_reports_info():
