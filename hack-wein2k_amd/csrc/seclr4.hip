@@ -242,32 +242,43 @@ __global__ __launch_bounds__(64) void k_tri_symv(TriWork g, int i, int nt)
   __syncthreads();
   const bool off = (I != J);
   const int m = min(SYMV_T, n - C0);
-  double y0 = 0.0, y1 = 0.0;
-  if (live) {
-    const double *ap = g.a + (size_t)R + (size_t)n * C0;
-    int j = 0;
-    for (; j + 2 <= m; j += 2) {
-      const double a0 = ap[(size_t)n * j], a1 = ap[(size_t)n * (j + 1)];
-      y0 += a0 * vs[j];
-      y1 += a1 * vs[j + 1];
-      if (off) { buf[j][lane] = a0 * vr; buf[j + 1][lane] = a1 * vr; }
-    }
-    for (; j < m; j++) {
-      const double a0 = ap[(size_t)n * j];
-      y0 += a0 * vs[j];
-      if (off) buf[j][lane] = a0 * vr;
-    }
+  // the lane's row of the tile, loaded in one burst (all loads in flight)
+  double av[SYMV_T];
+  const double *ap = g.a + (size_t)(live ? R : r0) + (size_t)n * C0;
+  if (m == SYMV_T) {
+#pragma unroll
+    for (int j = 0; j < SYMV_T; j++) av[j] = live ? ap[(size_t)n * j] : 0.0;
+  } else {
+#pragma unroll
+    for (int j = 0; j < SYMV_T; j++) av[j] = (live && j < m) ? ap[(size_t)n * j] : 0.0;
   }
-  if (off && !live)
-    for (int j = 0; j < m; j++) buf[j][lane] = 0.0;
-  const double yrow = y0 + y1;
+  double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+#pragma unroll
+  for (int j = 0; j < SYMV_T; j += 4) {
+    y0 += av[j] * vs[j];
+    y1 += av[j + 1] * vs[j + 1];
+    y2 += av[j + 2] * vs[j + 2];
+    y3 += av[j + 3] * vs[j + 3];
+  }
+  if (off) {
+#pragma unroll
+    for (int j = 0; j < SYMV_T; j++) buf[j][lane] = av[j] * vr;
+  }
+  const double yrow = (y0 + y1) + (y2 + y3);
   if (live) g.part[(size_t)J * n + R] = yrow;
   double vy = yrow * vr;
   if (off) {
     __syncthreads();
     if (lane < m) {
-      double cs = 0.0;
-      for (int rr = 0; rr < SYMV_T; rr++) cs += buf[lane][rr];
+      double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+#pragma unroll
+      for (int rr = 0; rr < SYMV_T; rr += 4) {
+        c0 += buf[lane][rr];
+        c1 += buf[lane][rr + 1];
+        c2 += buf[lane][rr + 2];
+        c3 += buf[lane][rr + 3];
+      }
+      const double cs = (c0 + c1) + (c2 + c3);
       g.part[(size_t)I * n + C0 + lane] = cs;
       vy += cs * vs[lane];
     }
@@ -278,12 +289,28 @@ __global__ __launch_bounds__(64) void k_tri_symv(TriWork g, int i, int nt)
     double *V = g.P + (size_t)n * ii, *V2 = g.P + (size_t)n * (2 * nb + ii);
     if (live) { V[R] = vr; V2[R] = vr; }
     for (int q = I * SYMV_T + lane; q < r0; q += nt * SYMV_T) { V[q] = 0.0; V2[q] = 0.0; }
-    double *dd = g.dots + (size_t)I * 2 * TNB;
-    for (int k = 0; k < ii; k++) {
-      const double pv = live ? g.P[(size_t)n * k + R] : 0.0;
-      const double pw = live ? g.P[(size_t)n * (nb + k) + R] : 0.0;
-      const double sa = wave_sum(pv * vr), sb = wave_sum(pw * vr);
-      if (lane == 0) { dd[k] = sa; dd[TNB + k] = sb; }
+    if (ii > 0) {
+      // partial V^T v and W^T v of the tile's rows: products through LDS, one
+      // column sum per lane
+      __syncthreads();
+      for (int k = 0; k < ii; k++) {
+        buf[k][lane] = live ? g.P[(size_t)n * k + R] * vr : 0.0;
+        buf[TNB + k][lane] = live ? g.P[(size_t)n * (nb + k) + R] * vr : 0.0;
+      }
+      __syncthreads();
+      double *dd = g.dots + (size_t)I * 2 * TNB;
+      const int q = lane;  // 0..63: a_k for q < TNB, b_k for q >= TNB
+      if ((q < TNB && q < ii) || (q >= TNB && q - TNB < ii)) {
+        double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+#pragma unroll
+        for (int rr = 0; rr < SYMV_T; rr += 4) {
+          c0 += buf[q][rr];
+          c1 += buf[q][rr + 1];
+          c2 += buf[q][rr + 2];
+          c3 += buf[q][rr + 3];
+        }
+        dd[q] = (c0 + c1) + (c2 + c3);
+      }
     }
   }
 }
@@ -306,16 +333,28 @@ __global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, in
   if (fin >= 0) {
     tauf = g.tau[fin];
     // fixed-order reductions of the SYMV partials, spread over the block
-    for (int q = t; q < 2 * TNB; q += blockDim.x) {
+    // 4 threads per dot product (tiles strided by 4), combined in fixed order
+    {
+      const int q = t >> 2, part4 = t & 3;
       double s = 0.0;
-      if ((q < TNB && q < ii) || (q >= TNB && q - TNB < ii))
-        for (int b = 0; b < ntile; b++) s += g.dots[(size_t)b * 2 * TNB + q];
-      ab[q] = s;
+      if (q < 2 * TNB && ((q < TNB && q < ii) || (q >= TNB && q - TNB < ii)))
+        for (int b = part4; b < ntile; b += 4) s += g.dots[(size_t)b * 2 * TNB + q];
+      s += __shfl_down(s, 2, 4);
+      s += __shfl_down(s, 1, 4);
+      if (part4 == 0 && q < 2 * TNB) ab[q] = s;
     }
     {
-      double s = 0.0;
-      for (int b = t; b < nvy; b += blockDim.x) s += g.vy[b];
-      s = wave_sum(s);
+      const int bd = blockDim.x;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int b = t;
+      for (; b + 3 * bd < nvy; b += 4 * bd) {
+        s0 += g.vy[b];
+        s1 += g.vy[b + bd];
+        s2 += g.vy[b + 2 * bd];
+        s3 += g.vy[b + 3 * bd];
+      }
+      for (; b < nvy; b += bd) s0 += g.vy[b];
+      double s = wave_sum((s0 + s1) + (s2 + s3));
       if ((t & 63) == 0) red[t >> 6] = s;
     }
     __syncthreads();
@@ -327,9 +366,22 @@ __global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, in
     const double *V = g.P, *W = g.P + (size_t)n * nb;
     double *Wi = g.P + (size_t)n * (nb + ii);
     if (live) {
-      double y = 0.0;
-      for (int c = 0; c < nch; c++) y += g.part[(size_t)c * n + r];
-      for (int k = 0; k < ii; k++) y -= W[(size_t)n * k + r] * ab[k] + V[(size_t)n * k + r] * ab[TNB + k];
+      double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+      int c = 0;
+      for (; c + 4 <= nch; c += 4) {
+        y0 += g.part[(size_t)c * n + r];
+        y1 += g.part[(size_t)(c + 1) * n + r];
+        y2 += g.part[(size_t)(c + 2) * n + r];
+        y3 += g.part[(size_t)(c + 3) * n + r];
+      }
+      for (; c < nch; c++) y0 += g.part[(size_t)c * n + r];
+      double y = (y0 + y1) + (y2 + y3);
+      double z0 = 0.0, z1 = 0.0;
+      for (int k = 0; k < ii; k++) {
+        z0 += W[(size_t)n * k + r] * ab[k];
+        z1 += V[(size_t)n * k + r] * ab[TNB + k];
+      }
+      y -= z0 + z1;
       wr = tauf * y + alpha2 * V[(size_t)n * ii + r];
       Wi[r] = wr;
     }
@@ -359,11 +411,13 @@ __global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, in
   double *ac = g.a + (size_t)n * col;
   double nrm = 0.0;
   if (live && r >= col) {
-    double a = ac[r];
+    double a0 = 0.0, a1 = 0.0;
     for (int k = 0; k < jj; k++) {
       const double wk = (k == ii) ? wr : W[(size_t)n * k + r];
-      a -= V[(size_t)n * k + r] * wc[k] + wk * vc[k];
+      a0 += V[(size_t)n * k + r] * wc[k];
+      a1 += wk * vc[k];
     }
+    const double a = ac[r] - (a0 + a1);
     ac[r] = a;
     if (r >= col + 2) nrm = a * a;
   }
