@@ -92,6 +92,40 @@ def cpu_baseline(kd, threads: int):
     }
 
 
+def _progress(msg: str):
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def copy_calibration(L):
+    """A plain device copy of 512 MiB (3 launches): the reference point that
+    scales the profiler's FETCH_SIZE / WRITE_SIZE counters to HBM bytes."""
+    import ctypes
+    lib = L.load_library()
+    lib.w2k_test_copy.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    n = 64 << 20
+    ms = ctypes.c_double(0.0)
+    rc = lib.w2k_test_copy(n, 3, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"w2k_test_copy failed ({rc})")
+    return {"kernel": "k_copy", "bytes_read": 8 * n, "bytes_written": 8 * n, "ms": round(ms.value, 4),
+            "GB/s": round(16.0 * n / (ms.value * 1e-3) / 1e9, 1)}
+
+
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "c2_rs64_hbm_counters.json")
+
+
+def measured_traffic(kname: str):
+    """HBM bytes per launch of the roofline kernel from the committed counter
+    passes (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each scaled by the
+    same counter's reading on the calibration copy), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        rec = json.load(f).get("kernels", {}).get(kname)
+    return None if rec is None else rec.get("hbm_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +133,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="skip the extra k-point with per-launch SYMV events (for profiler passes)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 512 MiB device copy (calibrates the profiler's byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -112,11 +150,16 @@ def main():
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(kd, args.cpu_threads)
 
+    copy = None
+    if args.copy_calibration:
+        copy = copy_calibration(L)
+        _progress(f"copy calibration: {copy['GB/s']} GB/s")
     ctx = L.Lapw1(run)
     ctx.set_basis(basis)
     for _ in range(args.warmup):
         ctx.calkpt(el, eu, nume)
     ctx.sync()
+    _progress(f"{args.warmup} warmup k-point(s) done")
     names = L.Lapw1.TIMING_NAMES
     acc = np.zeros(len(names))
     t0 = time.perf_counter()
@@ -125,15 +168,18 @@ def main():
         acc += ctx.timings()
     ctx.sync()
     t1 = time.perf_counter()
+    _progress(f"{args.steps} timed k-point(s) done")
     ms_step = (t1 - t0) * 1e3 / args.steps
     dev = acc / args.steps
     ev, _ = ctx.get_eig()
-    # one more k-point, outside the timed region, with per-launch events
-    # around the tridiagonalisation SYMV (its kernel-level roofline)
-    ctx.set_kernel_timing(True)
-    ctx.calkpt(el, eu, nume)
-    tk = dict(zip(names, ctx.timings()))
-    ctx.set_kernel_timing(False)
+    tk = None
+    if not args.no_kernel_timing:
+        # one more k-point, outside the timed region, with per-launch events
+        # around the tridiagonalisation SYMV (its kernel-level roofline)
+        ctx.set_kernel_timing(True)
+        ctx.calkpt(el, eu, nume)
+        tk = dict(zip(names, ctx.timings()))
+        ctx.set_kernel_timing(False)
     ctx.close()
 
     ok = ne == int(kd["NE"]) and nb == int(kd["NBELW"])
@@ -145,15 +191,16 @@ def main():
     f_hamilt = 8.0 * nv * nv * nat * lt
     f_hns = 4.0 * n * n * sum_lindex
     t = dict(zip(names, dev))
-    # algorithmic bytes of the SYMV at step i: the lower triangle (diagonal
-    # included) of the trailing order m = n - i - 1, FP64
-    nsym = int(tk["tri_symv_launches"])
-    sym_bytes = sum(8.0 * m * (m + 1) / 2 for m in range(n - 1, n - 1 - nsym, -1))
     kernels = {
-        "k_tri_symv": ("hbm", sym_bytes, tk["tri_symv_total"], nsym),
         "k_hamilt_pw": ("mfma", f_hamilt, t["hamilt_pw_kernel"], 1),
         "k_dgemm (HNS rank-K)": ("mfma", f_hns, t["hns_gemm"], 1),
     }
+    if tk is not None and tk["tri_symv_launches"] > 0:
+        # algorithmic bytes of the SYMV at step i: the lower triangle
+        # (diagonal included) of the trailing order m = n - i - 1, FP64
+        nsym = int(tk["tri_symv_launches"])
+        sym_bytes = sum(8.0 * m * (m + 1) / 2 for m in range(n - 1, n - 1 - nsym, -1))
+        kernels["k_tri_symv"] = ("hbm", sym_bytes, tk["tri_symv_total"], nsym)
     kname, (bound, work, kms, nl) = max(kernels.items(), key=lambda kv: kv[1][2])
     if bound == "hbm":
         achieved, peak, unit = work / (kms * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s"
@@ -183,13 +230,15 @@ def main():
         "config": {"workload": f"64-atom rocksalt 2x2x2 (Mg/O), real, Gamma, RKmax 7, N={n} (NV {nv} + NLO {nlo}), "
                                f"NE={ne}", "nat": nat, "lt": lt, "sum_lindex": sum_lindex},
         "roofline": {"kernel": kname, "bound": bound, "achieved": round(achieved, 2), "peak": peak,
-                     "unit": unit, "frac": round(achieved / peak, 4), "traffic": None,
+                     "unit": unit, "frac": round(achieved / peak, 4), "traffic": measured_traffic(kname),
                      "work_per_launch": work / nl, "ms_per_launch": round(kms / nl, 4), "launches": nl,
                      "other_kernels": others},
         "cpu_baseline": cpu,
         "phases_ms": {k: round(v, 2) for k, v in t.items()},
         "check": {"ne": ne, "nbelw": nb, "ne_ref": int(kd["NE"]), "max_abs_eig_dev_ry": dmax},
     }
+    if copy is not None:
+        out["copy_calibration"] = copy
     print(json.dumps(out))
 
 

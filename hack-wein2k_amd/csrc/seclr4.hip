@@ -24,8 +24,8 @@
 //   6. Z = Q Zt with the reflectors in compact WY form (DORMTR), then the
 //      selection sort of dsyevx2.f:381-400, then Z = U^-1 Z (DTRSM,
 //      seclr4.F:354) as a triangular MFMA GEMM with W = U^-1.
-// The arithmetic is real FP64 throughout (the complex build is rejected with
-// W2K_EINVAL here; HAMILT/HNS support it).
+// The arithmetic here is real FP64 throughout; the complex build (lapw1c)
+// runs the same stages in seclr4_cplx.hip.
 #include <cfloat>
 #include <cmath>
 #include <algorithm>
@@ -189,13 +189,14 @@ __global__ void k_mirror_lower(int n, double *c)
 // second copy of V(:,k) at column 2nb+k, so that the trailing update
 // A -= V W^T + W V^T is one GEMM with operands P(:,0:2nb) and P(:,nb:3nb).
 // Per column i of the panel two multi-workgroup kernels run:
-//   k_tri_symv : y = A(i+1:n, i+1:n) v as partial sums per column chunk, the
-//                dot products V^T v, W^T v and v^T y as per-tile partials,
-//                and V(:,k) written out (v = A(:,i) * scal, v(i+1) = 1);
+//   k_tri_symv : DLARFG of column i (beta, tau, scal) from the norm partials,
+//                y = A(i+1:n, i+1:n) v as partial sums per column chunk, the
+//                dot products V^T v, W^T v and v^T y as partials, and V(:,k)
+//                written out (v = A(:,i) * scal, v(i+1) = 1);
 //   k_tri_col  : w = tau (y - W V^T v - V W^T v) - (tau/2)(w^T v) v written
-//                to W(:,k); the next column updated with the panel so far;
-//                the norm of its sub-subdiagonal part reduced over the grid
-//                (last work-group finishes DLARFG: beta, tau, scal).
+//                to W(:,k); the next column updated with the panel so far and
+//                the squared norm of its sub-subdiagonal part written as one
+//                partial per block.
 struct TriWork {
   int n, p0, nb;
   double *a;          // n x n, both triangles
@@ -205,191 +206,257 @@ struct TriWork {
   double *dots;       // tiles x 2 TNB partial V^T v, W^T v
   double *vy;         // tiles x chunks partial v^T y
   double *norm;       // blocks: partial squared norms
-  int *counter;       // last-block ticket (zero between uses)
+  int *counter;       // unused by the real kernels
 };
 
+
+// DLARFG of column i on (A(i+1,i), A(i+2:n,i)), from the squared norm
+// partials of the column-update kernel; every wave of the SYMV evaluates it
+// with identical arithmetic (fixed-order sums), so no grid-wide step is
+// needed between the two kernels.  Returns scal = 1/(alpha - beta).
+__device__ __forceinline__ double tri_reflector(const TriWork &g, int i, int lane, bool store)
+{
+  const int n = g.n;
+  const int nnorm = (n - i + 63) / 64;  // blocks of the update of column i
+  double xs = 0.0;
+  for (int b = lane; b < nnorm; b += 64) xs += g.norm[b];
+  xs = wave_sum(xs);
+  xs = __shfl(xs, 0, 64);
+  const double alpha = g.a[(size_t)n * i + i + 1];
+  const double xnorm = sqrt(xs);
+  double tau = 0.0, beta = alpha, scal = 0.0;
+  if (xnorm != 0.0) {
+    beta = -copysign(sqrt(alpha * alpha + xnorm * xnorm), alpha);
+    tau = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+  }
+  if (store && lane == 0) {
+    g.e[i] = beta;
+    g.tau[i] = tau;
+    g.scal[i] = scal;
+  }
+  return scal;
+}
 
 // y = A(r0:n, r0:n) v from the tiles on and below the diagonal only (A is
 // symmetric): one wave per 64 x 64 tile (I, J), J <= I, adds A_IJ v_J to the
 // rows of tile I and, for J < I, A_IJ^T v_I to the rows of tile J (column
-// sums through an LDS transpose).  Row r of tile R receives exactly one
-// partial per tile index: from (R, J) at slot J <= R and from (I, R) at slot
-// I > R, so y(r) = sum over slots of part[slot][r] in a fixed order.
-// Blocks with J = 0 also write V(:,k) and the partial dot products
-// V^T v, W^T v of their rows; every block writes its partial v^T y.
-__global__ __launch_bounds__(64) void k_tri_symv(TriWork g, int i, int nt)
+// sums by a wave-wide reduce-scatter, no LDS).  Row r of tile R receives
+// exactly one partial per tile index: from (R, J) at slot J <= R and from
+// (I, R) at slot I > R, so y(r) = sum over slots of part[slot][r] in a fixed
+// order.  Every tile wave writes its partial v^T y.  The nt waves after the
+// tiles each take one row block I: they write V(:,k) (v = A(:,i) * scal,
+// v(i+1) = 1) and the partial dot products V^T v, W^T v of the block's rows.
+__global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
 {
   const int n = g.n, r0 = i + 1, ii = i - g.p0, nb = g.nb;
   const int lane = threadIdx.x;
-  // triangular block index -> (I, J), J <= I
+  const int nblk = nt * (nt + 1) / 2;
   const int bid = blockIdx.x;
+  const double *ai = g.a + (size_t)n * i;
+  double t[SYMV_T];
+  if (bid >= nblk) {
+    // row block I: V column and panel dot products
+    const int I = bid - nblk;
+    const int R = r0 + SYMV_T * I + lane;
+    const bool live = R < n;
+    const int Rs = live ? R : r0;
+    const double a_r = ai[Rs];
+    if (ii > 0) {
+#pragma unroll
+      for (int k = 0; k < TNB; k++) {
+        t[k] = g.P[(size_t)n * k + Rs];
+        t[TNB + k] = g.P[(size_t)n * (nb + k) + Rs];
+      }
+    }
+    const double sc = tri_reflector(g, i, lane, false);
+    const double vr = live ? (R == r0 ? 1.0 : a_r * sc) : 0.0;
+    double *V = g.P + (size_t)n * ii, *V2 = g.P + (size_t)n * (2 * nb + ii);
+    if (live) { V[R] = vr; V2[R] = vr; }
+    for (int q = I * SYMV_T + lane; q < r0; q += nt * SYMV_T) { V[q] = 0.0; V2[q] = 0.0; }
+    if (ii > 0) {
+      // t[k] = V(R,k) v(R), t[TNB+k] = W(R,k) v(R) for k < ii, summed over
+      // the lanes: lane q ends with a_q (q < TNB) or b_(q-TNB)
+#pragma unroll
+      for (int k = 0; k < TNB; k++) {
+        const bool use = live && k < ii;
+        t[k] = use ? t[k] * vr : 0.0;
+        t[TNB + k] = use ? t[TNB + k] * vr : 0.0;
+      }
+      const double dq = wave_transpose_sum(t, lane);
+      g.dots[(size_t)I * 2 * TNB + lane] = dq;
+    }
+    return;
+  }
+  // triangular block index -> (I, J), J <= I
   int I = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
   while ((I + 1) * (I + 2) / 2 <= bid) I++;
   while (I * (I + 1) / 2 > bid) I--;
   const int J = bid - I * (I + 1) / 2;
   const int R = r0 + SYMV_T * I + lane;
   const int C0 = r0 + SYMV_T * J;
-  const double sc = g.scal[i];
-  const double *ai = g.a + (size_t)n * i;
   __shared__ double vs[SYMV_T];
-  __shared__ double buf[SYMV_T][SYMV_T + 1];
-  {
-    const int c = C0 + lane;
-    vs[lane] = (c < n) ? (c == r0 ? 1.0 : ai[c] * sc) : 0.0;
-  }
   const bool live = R < n;
-  const double vr = live ? (R == r0 ? 1.0 : ai[R] * sc) : 0.0;
-  __syncthreads();
   const bool off = (I != J);
   const int m = min(SYMV_T, n - C0);
-  // the lane's row of the tile, loaded in one burst (all loads in flight)
-  double av[SYMV_T];
-  const double *ap = g.a + (size_t)(live ? R : r0) + (size_t)n * C0;
-  if (m == SYMV_T) {
+  const int Rs = live ? R : r0;
+  // every load of the tile and of v is issued before the first use: clamped
+  // (in-bounds) addresses, values masked afterwards
+  const double a_c = ai[min(C0 + lane, n - 1)];
+  const double a_r = ai[Rs];
+  const double *ap = g.a + (size_t)Rs + (size_t)n * C0;
 #pragma unroll
-    for (int j = 0; j < SYMV_T; j++) av[j] = live ? ap[(size_t)n * j] : 0.0;
-  } else {
+  for (int j = 0; j < SYMV_T; j++) t[j] = ap[(size_t)n * min(j, m - 1)];
+  const double sc = tri_reflector(g, i, lane, bid == 0);
+  vs[lane] = (C0 + lane < n) ? (C0 + lane == r0 ? 1.0 : a_c * sc) : 0.0;
+  const double vr = live ? (R == r0 ? 1.0 : a_r * sc) : 0.0;
+  if (m < SYMV_T || !live) {
 #pragma unroll
-    for (int j = 0; j < SYMV_T; j++) av[j] = (live && j < m) ? ap[(size_t)n * j] : 0.0;
+    for (int j = 0; j < SYMV_T; j++) t[j] = (live && j < m) ? t[j] : 0.0;
   }
+  __syncthreads();
   double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
 #pragma unroll
   for (int j = 0; j < SYMV_T; j += 4) {
-    y0 += av[j] * vs[j];
-    y1 += av[j + 1] * vs[j + 1];
-    y2 += av[j + 2] * vs[j + 2];
-    y3 += av[j + 3] * vs[j + 3];
-  }
-  if (off) {
-#pragma unroll
-    for (int j = 0; j < SYMV_T; j++) buf[j][lane] = av[j] * vr;
+    y0 += t[j] * vs[j];
+    y1 += t[j + 1] * vs[j + 1];
+    y2 += t[j + 2] * vs[j + 2];
+    y3 += t[j + 3] * vs[j + 3];
   }
   const double yrow = (y0 + y1) + (y2 + y3);
   if (live) g.part[(size_t)J * n + R] = yrow;
   double vy = yrow * vr;
   if (off) {
-    __syncthreads();
-    if (lane < m) {
-      double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
 #pragma unroll
-      for (int rr = 0; rr < SYMV_T; rr += 4) {
-        c0 += buf[lane][rr];
-        c1 += buf[lane][rr + 1];
-        c2 += buf[lane][rr + 2];
-        c3 += buf[lane][rr + 3];
-      }
-      const double cs = (c0 + c1) + (c2 + c3);
+    for (int j = 0; j < SYMV_T; j++) t[j] *= vr;
+    const double cs = wave_transpose_sum(t, lane);  // column C0 + lane of A_IJ^T v_I
+    if (lane < m) {
       g.part[(size_t)I * n + C0 + lane] = cs;
       vy += cs * vs[lane];
     }
   }
   vy = wave_sum(vy);
   if (lane == 0) g.vy[bid] = vy;
-  if (J == 0) {
-    double *V = g.P + (size_t)n * ii, *V2 = g.P + (size_t)n * (2 * nb + ii);
-    if (live) { V[R] = vr; V2[R] = vr; }
-    for (int q = I * SYMV_T + lane; q < r0; q += nt * SYMV_T) { V[q] = 0.0; V2[q] = 0.0; }
-    if (ii > 0) {
-      // partial V^T v and W^T v of the tile's rows: products through LDS, one
-      // column sum per lane
-      __syncthreads();
-      for (int k = 0; k < ii; k++) {
-        buf[k][lane] = live ? g.P[(size_t)n * k + R] * vr : 0.0;
-        buf[TNB + k][lane] = live ? g.P[(size_t)n * (nb + k) + R] * vr : 0.0;
-      }
-      __syncthreads();
-      double *dd = g.dots + (size_t)I * 2 * TNB;
-      const int q = lane;  // 0..63: a_k for q < TNB, b_k for q >= TNB
-      if ((q < TNB && q < ii) || (q >= TNB && q - TNB < ii)) {
-        double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
-#pragma unroll
-        for (int rr = 0; rr < SYMV_T; rr += 4) {
-          c0 += buf[q][rr];
-          c1 += buf[q][rr + 1];
-          c2 += buf[q][rr + 2];
-          c3 += buf[q][rr + 3];
-        }
-        dd[q] = (c0 + c1) + (c2 + c3);
-      }
-    }
-  }
 }
 
-// finish W(:, fin - p0) (fin >= 0) and update + reflect column col (col >= 0)
-__global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, int ntile, int nch, int nvy)
+// sum over the 4 lanes of a quad (lanes 4q..4q+3), the same value in each
+__device__ __forceinline__ double quad_sum(double v)
 {
-  const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x;
+  v += bfly_partner<1>(v);
+  v += bfly_partner<2>(v);
+  return v;
+}
+
+// Finish W(:, fin - p0) (fin >= 0) and update column col (col >= 0) with the
+// panel so far; write the squared-norm partial of its sub-subdiagonal part
+// per block (the reflector itself is formed by the next SYMV).  Four threads
+// per row (a quad), each with a quarter of the row's SYMV partials and of its
+// panel columns.  Every sum over a runtime count is read in unconditional
+// batches (clamped index, masked value) so that its loads are in flight
+// together.
+__global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, int ntile, int nvy)
+{
+  constexpr int BT = 32;          // loads per batch
+  constexpr int KQ = TNB / 4;     // panel columns per quad lane
+  const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int q4 = t & 3;
+  const int nch = ntile;
   const int rbase = fin >= 0 ? fin + 1 : col;
-  const int r = rbase + blockIdx.x * 256 + t;
+  const int r = rbase + blockIdx.x * 64 + (t >> 2);
   const bool live = r < n;
+  const int rs = live ? r : rbase;  // rbase < n: a safe row for dead lanes
   __shared__ double ab[2 * TNB];  // a_k = V_k^T v, b_k = W_k^T v
   __shared__ double wc[TNB], vc[TNB];
+  __shared__ double red[4][2 * TNB];
   __shared__ double sh[4];
-  __shared__ double red[4];
-  __shared__ int last;
   const int ii = fin >= 0 ? fin - p0 : -1;
-  double wr = 0.0;
+  const double *V = g.P, *W = g.P + (size_t)n * nb;
+  double *ac = g.a + (size_t)n * (col >= 0 ? col : 0);
+  const double acr = (col >= 0) ? ac[rs] : 0.0;
+  double wr = 0.0, vii = 0.0;
+  double vk[KQ], wk[KQ];  // V(r,k), W(r,k) for k = KQ q4 + u
   double tauf = 0.0, alpha2 = 0.0;
   if (fin >= 0) {
     tauf = g.tau[fin];
-    // fixed-order reductions of the SYMV partials, spread over the block
-    // 4 threads per dot product (tiles strided by 4), combined in fixed order
-    {
-      const int q = t >> 2, part4 = t & 3;
-      double s = 0.0;
-      if (q < 2 * TNB && ((q < TNB && q < ii) || (q >= TNB && q - TNB < ii)))
-        for (int b = part4; b < ntile; b += 4) s += g.dots[(size_t)b * 2 * TNB + q];
-      s += __shfl_down(s, 2, 4);
-      s += __shfl_down(s, 1, 4);
-      if (part4 == 0 && q < 2 * TNB) ab[q] = s;
+#pragma unroll
+    for (int u = 0; u < KQ; u++) {
+      vk[u] = V[(size_t)n * (KQ * q4 + u) + rs];
+      wk[u] = W[(size_t)n * (KQ * q4 + u) + rs];
     }
+    vii = V[(size_t)n * ii + rs];
+    // a_k, b_k: lane q of wave wid sums the tile partials b = wid (mod 4)
     {
-      const int bd = blockDim.x;
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      int b = t;
-      for (; b + 3 * bd < nvy; b += 4 * bd) {
-        s0 += g.vy[b];
-        s1 += g.vy[b + bd];
-        s2 += g.vy[b + 2 * bd];
-        s3 += g.vy[b + 3 * bd];
+      double s0 = 0.0, s1 = 0.0;
+      for (int b0 = wid; b0 < ntile; b0 += 4 * BT) {
+        double x[BT];
+#pragma unroll
+        for (int u = 0; u < BT; u++) x[u] = g.dots[(size_t)min(b0 + 4 * u, ntile - 1) * 2 * TNB + lane];
+#pragma unroll
+        for (int u = 0; u < BT; u += 2) {
+          s0 += (b0 + 4 * u < ntile) ? x[u] : 0.0;
+          s1 += (b0 + 4 * u + 4 < ntile) ? x[u + 1] : 0.0;
+        }
       }
-      for (; b < nvy; b += bd) s0 += g.vy[b];
-      double s = wave_sum((s0 + s1) + (s2 + s3));
-      if ((t & 63) == 0) red[t >> 6] = s;
+      red[wid][lane] = s0 + s1;
+    }
+    // v^T y: the tile partials over the block, fixed order
+    double vsum;
+    {
+      double s0 = 0.0, s1 = 0.0;
+      for (int b0 = t; b0 < nvy; b0 += 256 * BT) {
+        double x[BT];
+#pragma unroll
+        for (int u = 0; u < BT; u++) x[u] = g.vy[min(b0 + 256 * u, nvy - 1)];
+#pragma unroll
+        for (int u = 0; u < BT; u += 2) {
+          s0 += (b0 + 256 * u < nvy) ? x[u] : 0.0;
+          s1 += (b0 + 256 * u + 256 < nvy) ? x[u + 1] : 0.0;
+        }
+      }
+      vsum = wave_sum(s0 + s1);
+    }
+    // y(r): this lane's quarter of the row's SYMV partials (slots q4 + 4j)
+    double y;
+    {
+      double y0 = 0.0, y1 = 0.0;
+      for (int c0 = q4; c0 < nch; c0 += 4 * BT) {
+        double x[BT];
+#pragma unroll
+        for (int u = 0; u < BT; u++) x[u] = g.part[(size_t)min(c0 + 4 * u, nch - 1) * n + rs];
+#pragma unroll
+        for (int u = 0; u < BT; u += 2) {
+          y0 += (c0 + 4 * u < nch) ? x[u] : 0.0;
+          y1 += (c0 + 4 * u + 4 < nch) ? x[u + 1] : 0.0;
+        }
+      }
+      y = y0 + y1;
     }
     __syncthreads();
-    if (t == 0) sh[0] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (t < 2 * TNB) {
+      const double q = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+      ab[t] = ((t & (TNB - 1)) < ii) ? q : 0.0;
+    }
+    if (lane == 0) sh[wid] = vsum;
     __syncthreads();
+    const double vty = (sh[0] + sh[1]) + (sh[2] + sh[3]);
     double ssum = 0.0;
-    for (int k = 0; k < ii; k++) ssum += ab[k] * ab[TNB + k];
-    alpha2 = -0.5 * tauf * tauf * (sh[0] - 2.0 * ssum);
-    const double *V = g.P, *W = g.P + (size_t)n * nb;
-    double *Wi = g.P + (size_t)n * (nb + ii);
-    if (live) {
-      double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
-      int c = 0;
-      for (; c + 4 <= nch; c += 4) {
-        y0 += g.part[(size_t)c * n + r];
-        y1 += g.part[(size_t)(c + 1) * n + r];
-        y2 += g.part[(size_t)(c + 2) * n + r];
-        y3 += g.part[(size_t)(c + 3) * n + r];
-      }
-      for (; c < nch; c++) y0 += g.part[(size_t)c * n + r];
-      double y = (y0 + y1) + (y2 + y3);
-      double z0 = 0.0, z1 = 0.0;
-      for (int k = 0; k < ii; k++) {
-        z0 += W[(size_t)n * k + r] * ab[k];
-        z1 += V[(size_t)n * k + r] * ab[TNB + k];
-      }
-      y -= z0 + z1;
-      wr = tauf * y + alpha2 * V[(size_t)n * ii + r];
-      Wi[r] = wr;
+#pragma unroll
+    for (int k = 0; k < TNB; k++) ssum += ab[k] * ab[TNB + k];
+    alpha2 = -0.5 * tauf * tauf * (vty - 2.0 * ssum);
+    double z = 0.0;
+#pragma unroll
+    for (int u = 0; u < KQ; u++) {
+      const int k = KQ * q4 + u;
+      z += (k < ii ? wk[u] : 0.0) * ab[k] + (k < ii ? vk[u] : 0.0) * ab[TNB + k];
     }
+    y = quad_sum(y) - quad_sum(z);
+    wr = tauf * y + alpha2 * vii;
+    double *Wi = g.P + (size_t)n * (nb + ii);
+    if (live && q4 == 0) Wi[r] = wr;
     for (int q = blockIdx.x * 256 + t; q < rbase; q += gridDim.x * 256) Wi[q] = 0.0;
   }
   if (col < 0) return;
   const int jj = col - p0;  // columns 0..jj-1 of the panel update this column
-  const double *V = g.P, *W = g.P + (size_t)n * nb;
   if (t < jj) {
     vc[t] = V[(size_t)n * t + col];
     if (t < ii || fin < 0) wc[t] = W[(size_t)n * t + col];
@@ -399,60 +466,41 @@ __global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, in
     // (sum of the SYMV partials of that row minus the panel corrections)
     double s = 0.0;
     for (int c = t; c < nch; c += blockDim.x) s += g.part[(size_t)c * n + col];
-    for (int k = t; k < ii; k += blockDim.x)
-      s -= W[(size_t)n * k + col] * ab[k] + V[(size_t)n * k + col] * ab[TNB + k];
+    if (t < ii) s -= W[(size_t)n * t + col] * ab[t] + V[(size_t)n * t + col] * ab[TNB + t];
     s = wave_sum(s);
-    __syncthreads();  // red[] reuse
-    if ((t & 63) == 0) red[t >> 6] = s;
+    if (lane == 0) red[wid][0] = s;  // red[][] was last read before the previous barrier
     __syncthreads();
-    if (t == 0) wc[ii] = tauf * ((red[0] + red[1]) + (red[2] + red[3])) + alpha2 * 1.0;
+    if (t == 0) wc[ii] = tauf * ((red[0][0] + red[1][0]) + (red[2][0] + red[3][0])) + alpha2 * 1.0;
   }
   __syncthreads();
-  double *ac = g.a + (size_t)n * col;
-  double nrm = 0.0;
-  if (live && r >= col) {
-    double a0 = 0.0, a1 = 0.0;
-    for (int k = 0; k < jj; k++) {
-      const double wk = (k == ii) ? wr : W[(size_t)n * k + r];
-      a0 += V[(size_t)n * k + r] * wc[k];
-      a1 += wk * vc[k];
+  double a = acr;
+  if (fin >= 0) {
+    double a0 = 0.0;
+#pragma unroll
+    for (int u = 0; u < KQ; u++) {
+      const int k = KQ * q4 + u;
+      const bool use = k < ii;
+      a0 += (use ? vk[u] : 0.0) * (use ? wc[k] : 0.0) + (use ? wk[u] : 0.0) * (use ? vc[k] : 0.0);
     }
-    const double a = ac[r] - (a0 + a1);
+    a -= quad_sum(a0) + (vii * wc[ii] + wr * vc[ii]);
+  }
+  double nrm = 0.0;
+  if (live && q4 == 0) {
     ac[r] = a;
+    if (r == col) {
+      g.d[col] = a;
+      if (col == n - 1) {
+        g.tau[col] = 0.0;
+        g.scal[col] = 0.0;
+      }
+    }
     if (r >= col + 2) nrm = a * a;
   }
   nrm = wave_sum(nrm);
-  if ((t & 63) == 0) red[t >> 6] = nrm;
+  __syncthreads();  // red[][] reuse
+  if (lane == 0) red[wid][1] = nrm;
   __syncthreads();
-  if (t == 0) {
-    g.norm[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
-    __threadfence();
-    last = (atomicAdd(g.counter, 1) == (int)gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last || t != 0) return;
-  __threadfence();
-  *g.counter = 0;
-  double xs = 0.0;
-  for (int b = 0; b < (int)gridDim.x; b++) xs += g.norm[b];
-  g.d[col] = ac[col];
-  if (col >= n - 1) {
-    g.tau[col] = 0.0;
-    g.scal[col] = 0.0;
-    return;
-  }
-  // DLARFG on (A(col+1,col), A(col+2:n,col))
-  const double xnorm = sqrt(xs);
-  const double alpha = ac[col + 1];
-  double tau = 0.0, beta = alpha, scal = 0.0;
-  if (xnorm != 0.0) {
-    beta = -copysign(sqrt(alpha * alpha + xnorm * xnorm), alpha);
-    tau = (beta - alpha) / beta;
-    scal = 1.0 / (alpha - beta);
-  }
-  g.e[col] = beta;
-  g.tau[col] = tau;
-  g.scal[col] = scal;
+  if (t == 0) g.norm[blockIdx.x] = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
 }
 
 // reflectors of panel columns into A below the subdiagonal (read by the back-transform)
@@ -1190,7 +1238,7 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
       g.p0 = p0;
       g.nb = nb;
       // reflector of the panel's first column (already updated by earlier panels)
-      hipLaunchKernelGGL(k_tri_col, dim3((n - p0 + 255) / 256), dim3(256), 0, s, g, -1, p0, 0, 0, 0);
+      hipLaunchKernelGGL(k_tri_col, dim3((n - p0 + 63) / 64), dim3(256), 0, s, g, -1, p0, 0, 0);
       W2K_STAGE(s, "k_tri_col");
       int nref = 0;  // panel columns that carry a reflector
       for (int k = 0; k < nb; k++) {
@@ -1200,12 +1248,12 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
         const int rows = n - (i + 1);
         const int nt = (rows + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
         if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv], s));
-        hipLaunchKernelGGL(k_tri_symv, dim3(nblk), dim3(SYMV_T), 0, s, g, i, nt);
+        hipLaunchKernelGGL(k_tri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
         W2K_STAGE(s, "k_tri_symv");
         if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv + 1], s));
         nsymv++;
         const int next = (k + 1 < nb) ? i + 1 : -1;
-        hipLaunchKernelGGL(k_tri_col, dim3((rows + 255) / 256), dim3(256), 0, s, g, i, next, nt, nt, nblk);
+        hipLaunchKernelGGL(k_tri_col, dim3((rows + 63) / 64), dim3(256), 0, s, g, i, next, nt, nblk);
         W2K_STAGE(s, "k_tri_col");
       }
       if (nref > 0) {

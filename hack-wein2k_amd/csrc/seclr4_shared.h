@@ -18,6 +18,58 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
+// v from the partner lane of a halving step with stride S: any lane that has
+// bit S flipped and the bits above S unchanged will do, so the short strides
+// use row-local DPP moves (S = 1, 2: quad permutes; S = 4, 8: half-row and
+// row mirrors), S = 16 a swizzle within 32 lanes and S = 32 a bpermute.
+template <int S>
+__device__ __forceinline__ double bfly_partner(double v)
+{
+  if constexpr (S == 32) {
+    return __shfl_xor(v, 32, 64);
+  } else {
+    const long long b = __double_as_longlong(v);
+    int lo = (int)(unsigned)(b & 0xffffffffll), hi = (int)(unsigned)((unsigned long long)b >> 32);
+    if constexpr (S == 16) {
+      lo = __builtin_amdgcn_ds_swizzle(lo, 0x401F);
+      hi = __builtin_amdgcn_ds_swizzle(hi, 0x401F);
+    } else {
+      constexpr int ctrl = S == 1 ? 0xB1 : S == 2 ? 0x4E : S == 4 ? 0x141 : 0x140;
+      lo = __builtin_amdgcn_update_dpp(0, lo, ctrl, 0xF, 0xF, false);
+      hi = __builtin_amdgcn_update_dpp(0, hi, ctrl, 0xF, 0xF, false);
+    }
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+  }
+}
+
+// One halving step of a wave-wide reduce-scatter: lanes with bit S set keep
+// entries S..2S-1, the others keep 0..S-1, and each adds the half that its
+// partner lane gives away.
+template <int S>
+__device__ __forceinline__ void bfly_step(double *t, int lane)
+{
+  const bool hi = (lane & S) != 0;
+#pragma unroll
+  for (int j = 0; j < S; j++) {
+    const double keep = hi ? t[j + S] : t[j];
+    const double send = hi ? t[j] : t[j + S];
+    t[j] = keep + bfly_partner<S>(send);
+  }
+}
+
+// t[0..63] per lane -> returns, in lane l, the sum over the 64 lanes of t[l]
+// (a transpose-and-sum in six halving steps, no LDS).  Overwrites t.
+__device__ __forceinline__ double wave_transpose_sum(double *t, int lane)
+{
+  bfly_step<32>(t, lane);
+  bfly_step<16>(t, lane);
+  bfly_step<8>(t, lane);
+  bfly_step<4>(t, lane);
+  bfly_step<2>(t, lane);
+  bfly_step<1>(t, lane);
+  return t[0];
+}
+
 // device work buffers of one SECLR4 call (carved from Ctx::w_small / w_int);
 // complex quantities are planar: real parts, then imaginary parts at +stride
 struct Seclr4Bufs {

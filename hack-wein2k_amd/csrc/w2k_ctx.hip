@@ -676,3 +676,41 @@ extern "C" int w2k_test_dgemm_ws(char ta, char tb, int m, int n, int k, double a
   if (wsn < 0) return W2K_EINVAL;
   return test_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, 0, 0, (size_t)wsn);
 }
+
+// plain device-to-device copy, the calibration kernel for the HBM byte
+// counters (its algorithmic traffic is exactly 8 n bytes read + 8 n written)
+__global__ void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n2)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n2) dst[i] = src[i];
+}
+
+extern "C" int w2k_test_copy(long n, int reps, double *ms_per_copy)
+{
+  if (n <= 0 || (n & 1) || reps <= 0) return W2K_EINVAL;
+  if (hipSetDevice(0) != hipSuccess) return W2K_ENODEV;
+  DBuf<double> a, b;
+  int rc;
+  if ((rc = a.alloc((size_t)n)) || (rc = b.alloc((size_t)n))) return rc;
+  const size_t n2 = (size_t)n / 2;
+  const unsigned grid = (unsigned)((n2 + 255) / 256);
+  W2K_HIP(hipMemset(a.p, 0, sizeof(double) * (size_t)n));
+  // one untimed copy first (first-touch page mapping)
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, nullptr, (const double2 *)a.p, (double2 *)b.p, n2);
+  W2K_HIP(hipGetLastError());
+  hipEvent_t e0, e1;
+  W2K_HIP(hipEventCreate(&e0));
+  W2K_HIP(hipEventCreate(&e1));
+  W2K_HIP(hipEventRecord(e0, nullptr));
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, nullptr, (const double2 *)a.p, (double2 *)b.p, n2);
+  W2K_HIP(hipGetLastError());
+  W2K_HIP(hipEventRecord(e1, nullptr));
+  W2K_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  W2K_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_per_copy) *ms_per_copy = ms / reps;
+  return 0;
+}
