@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Seconds per k-point of lapw1's hot path (HAMILT + HNS + SECLR4) on one MI355X.
+"""Seconds per k-point of lapw1's hot path (HAMILT + HNS + SECLR4) on one MI355X,
+plain FP64 and with the split-precision HNS update (INT8 digit slices).
 
 Workload: the 64-atom rocksalt supercell (2x2x2, Mg/O, RMT 2.0/1.6, RKmax 7,
 real/inversion-symmetric, one k-point at Gamma, N = 5951 = 5695 plane waves +
@@ -12,7 +13,14 @@ basis already resident on the device (uploaded before the timed region).
 The CPU baseline is the reference lapw1 itself (compiled from its sources by
 oracle/Makefile.ref into oracle/_ref/), run on this host on the same deck for
 the same k-point, timed by its own phase clocks; it runs before the GPU is
-touched.  Prints one JSON line.
+touched.
+
+The k-point is timed twice: with every product in FP64 ("fp64") and with the
+HNS rank-K update in split precision (--split-slices INT8 digits per element,
+FP64 recombination).  `value` is the split-precision time when its
+eigenvalues stay within 1e-8 Ry of the reference with identical NE/NBELW
+(checked in the run), else the FP64 time; both are in the line.  Prints one
+JSON line.
 """
 from __future__ import annotations
 
@@ -34,6 +42,8 @@ REF_EXE = os.path.join(REPO, "oracle", "_ref", "lapw1_real")
 
 METRIC = "s per k-point (H/S setup+eigensolve), split-prec vs FP64 at ΔE≤1e-8 Ry"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_INT8_TOPS = 5000.0
+DE_BAR = 1e-8  # Ry: split precision must stay this close to the reference eigenvalues
 PEAK_HBM_GBS = 8000.0
 
 
@@ -135,6 +145,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="skip the extra k-point with per-launch SYMV events (for profiler passes)")
+    ap.add_argument("--split-slices", type=int, default=4,
+                    help="INT8 digits per element for the split-precision pass (0: FP64 only)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy (calibrates the profiler's byte counters)")
     args = ap.parse_args()
@@ -156,22 +168,30 @@ def main():
         _progress(f"copy calibration: {copy['GB/s']} GB/s")
     ctx = L.Lapw1(run)
     ctx.set_basis(basis)
-    for _ in range(args.warmup):
-        ctx.calkpt(el, eu, nume)
-    ctx.sync()
-    _progress(f"{args.warmup} warmup k-point(s) done")
     names = L.Lapw1.TIMING_NAMES
-    acc = np.zeros(len(names))
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ne, nb = ctx.calkpt(el, eu, nume)
-        acc += ctx.timings()
-    ctx.sync()
-    t1 = time.perf_counter()
-    _progress(f"{args.steps} timed k-point(s) done")
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    dev = acc / args.steps
-    ev, _ = ctx.get_eig()
+
+    def timed(slices: int):
+        """W warmup and K timed k-points with the HNS rank-K update in FP64
+        (slices = 0) or in split precision (INT8 digit slices)."""
+        ctx.set_split_precision(slices)
+        for _ in range(args.warmup):
+            ctx.calkpt(el, eu, nume)
+        ctx.sync()
+        acc = np.zeros(len(names))
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ne, nb = ctx.calkpt(el, eu, nume)
+            acc += ctx.timings()
+        ctx.sync()
+        t1 = time.perf_counter()
+        ev, _ = ctx.get_eig()
+        tag = "FP64" if slices == 0 else f"split precision ({slices} digits)"
+        _progress(f"{args.warmup} warmup + {args.steps} timed k-point(s) done, {tag}")
+        return {"ms": (t1 - t0) * 1e3 / args.steps, "dev": acc / args.steps, "ne": ne, "nb": nb, "ev": ev}
+
+    fp = timed(0)
+    sp = timed(args.split_slices) if args.split_slices > 0 else None
+    ctx.set_split_precision(0)
     tk = None
     if not args.no_kernel_timing:
         # one more k-point, outside the timed region, with per-launch events
@@ -182,8 +202,24 @@ def main():
         ctx.set_kernel_timing(False)
     ctx.close()
 
-    ok = ne == int(kd["NE"]) and nb == int(kd["NBELW"])
-    dmax = float(np.max(np.abs(ev - kd["EIGVAL"]))) if ok else float("nan")
+    def check(r):
+        ok = r["ne"] == int(kd["NE"]) and r["nb"] == int(kd["NBELW"])
+        return ok, (float(np.max(np.abs(r["ev"] - kd["EIGVAL"]))) if ok else float("nan"))
+
+    ok, dmax = check(fp)
+    ne, nb, dev, ms_fp64 = fp["ne"], fp["nb"], fp["dev"], fp["ms"]
+    split = None
+    ms_step = ms_fp64
+    if sp is not None:
+        sok, sd = check(sp)
+        de_fp = float(np.max(np.abs(sp["ev"] - fp["ev"]))) if sok else float("nan")
+        usable = sok and sd <= DE_BAR
+        split = {"slices": args.split_slices, "ms_per_step": round(sp["ms"], 2),
+                 "max_abs_eig_dev_vs_reference_ry": sd, "max_abs_eig_dev_vs_fp64_ry": de_fp,
+                 "ne": sp["ne"], "nbelw": sp["nb"], "meets_de_bar": usable,
+                 "hns_rank_k_ms": round(float(sp["dev"][names.index("hns_gemm")]), 3)}
+        if usable:
+            ms_step = sp["ms"]
 
     nv, nlo, n = basis.nv, basis.nlo, basis.n
     nat, lt = run.nat, run.ltop
@@ -214,6 +250,14 @@ def main():
             others[kn] = {"ms": round(ms_, 3), "GB/s": round(wk / (ms_ * 1e-3) / 1e9, 1)}
         else:
             others[kn] = {"ms": round(ms_, 3), "TFLOP/s": round(wk / (ms_ * 1e-3) / 1e12, 2)}
+    if split is not None and split["hns_rank_k_ms"] > 0:
+        # the same FP64-equivalent work; the INT8 rate counts the S(S+1)/2
+        # digit-pair products actually issued
+        sl, tms = split["slices"], split["hns_rank_k_ms"]
+        others["k_split_gemm (HNS rank-K, INT8 digits)"] = {
+            "ms": tms, "FP64-equivalent TFLOP/s": round(f_hns / (tms * 1e-3) / 1e12, 2),
+            "INT8 TOP/s": round(f_hns * sl * (sl + 1) / 2 / (tms * 1e-3) / 1e12, 1), "INT8 peak TOP/s": PEAK_INT8_TOPS}
+    use_split = split is not None and split["meets_de_bar"]
     out = {
         "metric": METRIC,
         "value": round(ms_step / 1e3, 4),
@@ -225,8 +269,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: seeded WIEN2k input deck (tests/golden/w2k_inputs.py), plain FP64 path",
+        "dtype": "f64" if not use_split else "f64 (HNS rank-K: int8 digit slices, FP64 recombination)",
+        "data": "synthetic: seeded WIEN2k input deck (tests/golden/w2k_inputs.py)",
         "config": {"workload": f"64-atom rocksalt 2x2x2 (Mg/O), real, Gamma, RKmax 7, N={n} (NV {nv} + NLO {nlo}), "
                                f"NE={ne}", "nat": nat, "lt": lt, "sum_lindex": sum_lindex},
         "roofline": {"kernel": kname, "bound": bound, "achieved": round(achieved, 2), "peak": peak,
@@ -236,6 +280,9 @@ def main():
         "cpu_baseline": cpu,
         "phases_ms": {k: round(v, 2) for k, v in t.items()},
         "check": {"ne": ne, "nbelw": nb, "ne_ref": int(kd["NE"]), "max_abs_eig_dev_ry": dmax},
+        "fp64": {"value": round(ms_fp64 / 1e3, 4), "ms_per_step": round(ms_fp64, 2), "max_abs_eig_dev_ry": dmax},
+        "split_precision": split,
+        "value_is": "split precision" if use_split else "fp64",
     }
     if copy is not None:
         out["copy_calibration"] = copy

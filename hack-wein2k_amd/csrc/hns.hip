@@ -205,9 +205,17 @@ int hns(Ctx &c)
   else hipLaunchKernelGGL(k_hns_prep<false>, grid, dim3(64), 0, s, g);
   W2K_STAGE(s, "k_hns_prep");
   const int K4 = (int)kcols, K2 = 2 * ktot;
+  const int sl = c.hns_slices;
+  if (sl > 0 && (rc = c.w_dig.alloc(split_gemm_ws_bytes(n, n, K4, sl)))) return rc;
+  // the rank-K product L R^T (lower triangle): FP64 MFMA, or INT8 digit
+  // slices with FP64 recombination in split precision
+  auto rank_k = [&](const double *Rop, double *C, int accumulate) -> int {
+    if (sl > 0) return split_gemm_nt(s, n, n, K4, g.L, n, Rop, n, C, n, 1, accumulate, sl, c.w_dig.p);
+    return dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, Rop, n, accumulate ? 1.0 : 0.0, C, n, 1);
+  };
   if (!cpx) {
     W2K_HIP(hipEventRecord(c.ev[14], s));
-    if ((rc = dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, g.R, n, 1.0, c.d_hs.p, n, 1))) return rc;
+    if ((rc = rank_k(g.R, c.d_hs.p, 1))) return rc;
     W2K_HIP(hipEventRecord(c.ev[15], s));
     if (nlo > 0) {
       double *hsl = c.d_hs.p + (size_t)c.nv + (size_t)n * c.nv;
@@ -218,8 +226,8 @@ int hns(Ctx &c)
     if ((rc = c.h_re.alloc((size_t)n * n))) return rc;
     if ((rc = c.h_im.alloc((size_t)n * n))) return rc;
     W2K_HIP(hipEventRecord(c.ev[14], s));
-    if ((rc = dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, g.R, n, 0.0, c.h_re.p, n, 1))) return rc;
-    if ((rc = dgemm(s, 'N', 'T', n, n, K4, 1.0, g.L, n, g.R2, n, 0.0, c.h_im.p, n, 1))) return rc;
+    if ((rc = rank_k(g.R, c.h_re.p, 0))) return rc;
+    if ((rc = rank_k(g.R2, c.h_im.p, 0))) return rc;
     W2K_HIP(hipEventRecord(c.ev[15], s));
     hipLaunchKernelGGL(k_add_cplx_lower, dim3((n + 255) / 256, n), dim3(256), 0, s, n, 0, n, c.h_re.p, c.h_im.p, n,
                        c.d_hs.p);

@@ -63,6 +63,7 @@ void DBuf<T>::release()
 }
 template struct DBuf<double>;
 template struct DBuf<int>;
+template struct DBuf<signed char>;
 
 template <class T>
 static int upload(DBuf<T> &b, const T *src, size_t count, hipStream_t s)
@@ -360,6 +361,14 @@ int w2k_set_kernel_timing(w2k_ctx *x, int on)
 {
   if (!x) return W2K_EINVAL;
   x->c.ktime = on ? 1 : 0;
+  return 0;
+}
+
+int w2k_set_split_precision(w2k_ctx *x, int slices)
+{
+  if (!x) return W2K_EINVAL;
+  if (slices != 0 && (slices < 3 || slices > 7)) return W2K_EINVAL;
+  x->c.hns_slices = slices;
   return 0;
 }
 
@@ -675,6 +684,28 @@ extern "C" int w2k_test_dgemm_ws(char ta, char tb, int m, int n, int k, double a
 {
   if (wsn < 0) return W2K_EINVAL;
   return test_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, 0, 0, (size_t)wsn);
+}
+
+extern "C" int w2k_test_split_gemm(int m, int n, int k, const double *A, int lda, const double *B, int ldb, double *C,
+                                   int ldc, int tri, int accumulate, int slices)
+{
+  if (m <= 0 || n <= 0 || k <= 0 || lda < m || ldb < n || ldc < m) return W2K_EINVAL;
+  if (slices < 3 || slices > 7 || (tri != 0 && tri != 1)) return W2K_EINVAL;
+  if (hipSetDevice(0) != hipSuccess) return W2K_ENODEV;
+  const size_t na = (size_t)lda * k, nb = (size_t)ldb * k, nc = (size_t)ldc * n;
+  DBuf<double> dA, dB, dC;
+  DBuf<signed char> ws;
+  int rc;
+  if ((rc = dA.alloc(na)) || (rc = dB.alloc(nb)) || (rc = dC.alloc(nc))) return rc;
+  if ((rc = ws.alloc(split_gemm_ws_bytes(m, n, k, slices)))) return rc;
+  W2K_HIP(hipMemcpy(dA.p, A, sizeof(double) * na, hipMemcpyHostToDevice));
+  W2K_HIP(hipMemcpy(dB.p, B, sizeof(double) * nb, hipMemcpyHostToDevice));
+  W2K_HIP(hipMemcpy(dC.p, C, sizeof(double) * nc, hipMemcpyHostToDevice));
+  rc = split_gemm_nt(nullptr, m, n, k, dA.p, lda, dB.p, ldb, dC.p, ldc, tri, accumulate, slices, ws.p);
+  if (rc) return rc;
+  W2K_HIP(hipDeviceSynchronize());
+  W2K_HIP(hipMemcpy(C, dC.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
+  return 0;
 }
 
 // plain device-to-device copy, the calibration kernel for the HBM byte

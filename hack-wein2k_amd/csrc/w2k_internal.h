@@ -109,6 +109,8 @@ struct Ctx {
   DBuf<double> h_small;         // LO-block operands
   DBuf<int> h_int;              // per-atom column offsets
   DBuf<double> h_re, h_im;      // complex build: planar products
+  int hns_slices = 0;           // 0: FP64 rank-K update; 3..7: INT8 digit slices (split precision)
+  DBuf<signed char> w_dig;      // split precision: digit planes and row exponents
   // eigen results
   int ne = 0, nbelw = 0;
   DBuf<double> d_eig, d_z;
@@ -136,6 +138,13 @@ int hns(Ctx &c);
 int seclr4(Ctx &c, double el, double eu, int nume, int *ne, int *nbelw);
 // elapsed milliseconds between events e0 and e1 into last_ms[slot]
 void take_ms(Ctx &c, int slot, int e0, int e1);
+
+// split precision C (+)= A B^T (A: m x k, B: n x k, column-major, FP64) on the
+// INT8 matrix cores with `slices` 7-bit digits per element (3..7); tri 0
+// full, 1 lower; ws must hold split_gemm_ws_bytes(m, n, k, slices) bytes
+size_t split_gemm_ws_bytes(int m, int n, int k, int slices);
+int split_gemm_nt(hipStream_t s, int m, int n, int k, const double *A, int lda, const double *B, int ldb, double *C,
+                  int ldc, int tri, int accumulate, int slices, void *ws);
 
 // generic FP64 GEMM on the device: C = alpha * op(A) * op(B) + beta * C
 // tri: 0 full, 1 lower (i >= j) only, 2 upper (i <= j) only (tiles outside skipped)

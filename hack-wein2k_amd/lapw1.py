@@ -200,6 +200,7 @@ def load_library(path: Optional[str] = None):
     lib.w2k_version.restype = ctypes.c_char_p
     lib.w2k_last_error.restype = ctypes.c_char_p
     lib.w2k_set_kernel_timing.argtypes = [V, ctypes.c_int]
+    lib.w2k_set_split_precision.argtypes = [V, ctypes.c_int]
     if path is None:
         _LIB = lib
     return lib
@@ -304,6 +305,11 @@ class Lapw1:
     def set_kernel_timing(self, on: bool):
         """Per-launch events around the tridiagonalisation SYMV kernels."""
         self._chk("w2k_set_kernel_timing", self.lib.w2k_set_kernel_timing(self.ctx, int(bool(on))))
+
+    def set_split_precision(self, slices: int):
+        """HNS rank-K update in split precision: 0 = FP64 (default), 3..7 = INT8
+        digit slices per element with FP64 recombination."""
+        self._chk("w2k_set_split_precision", self.lib.w2k_set_split_precision(self.ctx, int(slices)))
 
     def timings(self) -> np.ndarray:
         """Device milliseconds of the last calls (order of TIMING_NAMES)."""

@@ -162,6 +162,16 @@ int w2k_last_timings(w2k_ctx *ctx, double *ms, int n);
  * little time between launches.  No counterpart in the reference. */
 int w2k_set_kernel_timing(w2k_ctx *ctx, int on);
 
+/* Split precision for the HNS rank-K update (hns.F:443-488, which the
+ * reference does in FP64 with DSYR2M / ZHER2M): slices = 0 keeps the FP64
+ * product (default); slices = 3..7 computes it on the INT8 matrix cores with
+ * that many signed 7-bit digits per element and exact INT32 sums, recombined
+ * in FP64.  The error of each element is at most
+ * K (slices + 2) 2^(E_i + F_j - 7 slices), with 2^E_i, 2^F_j bounding the
+ * rows of the two operands.  An extended form with no counterpart in the
+ * reference; returns W2K_EINVAL for other values. */
+int w2k_set_split_precision(w2k_ctx *ctx, int slices);
+
 /* Library version string. */
 const char *w2k_version(void);
 

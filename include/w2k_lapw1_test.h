@@ -21,6 +21,12 @@ int w2k_test_dgemm(char ta, char tb, int m, int n, int k, double alpha, const do
 int w2k_test_dgemm_ws(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
                       const double *B, int ldb, double beta, double *C, int ldc, long wsn);
 
+/* Split-precision C (+)= A B^T (A: m x k, B: n x k, column-major FP64) with
+ * `slices` INT8 digits per element (3..7), the kernel behind
+ * w2k_set_split_precision; tri 0 full, 1 lower; accumulate 1 adds to C. */
+int w2k_test_split_gemm(int m, int n, int k, const double *A, int lda, const double *B, int ldb, double *C, int ldc,
+                        int tri, int accumulate, int slices);
+
 /* reps launches of a plain device-to-device copy of n doubles (n even) on
  * device 0, for calibrating the HBM byte counters of a profiler; the mean
  * time of one copy in *ms_per_copy. */
