@@ -438,7 +438,7 @@ static int dispatch_pw(const PwArgs &a, hipStream_t s)
 
 int hamilt(Ctx &c)
 {
-  RunDev &R = c.run;
+  RunDev &R = c.run();
   const int n = c.n, nv = c.nv, lt = R.ltop, nat = R.nat, ndf = R.ndf;
   const int cpx = R.is_complex;
   hipStream_t s = c.stream;

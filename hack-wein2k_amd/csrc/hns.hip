@@ -160,7 +160,7 @@ __global__ void k_add_cplx_lower(int n, int off, int ld, const double *__restric
 
 int hns(Ctx &c)
 {
-  RunDev &R = c.run;
+  RunDev &R = c.run();
   if (!c.have_hamilt) return W2K_ESTATE;
   const int n = c.n, nlo = c.nlo, ndf = R.ndf;
   const int cpx = R.is_complex;
@@ -182,6 +182,7 @@ int hns(Ctx &c)
   if ((rc = c.h_int.alloc(ndf))) return rc;
   d_koff = c.h_int.p;
   W2K_HIP(hipMemcpyAsync(d_koff, koff.data(), sizeof(int) * ndf, hipMemcpyHostToDevice, s));
+  W2K_HIP(hipStreamSynchronize(s));  // koff is a host temporary
   HnsArgs g;
   g.n = n; g.nv = c.nv; g.nnsl = R.nnsl; g.lt = R.ltop; g.ndf = ndf;
   g.kv = c.d_kvec.p; g.cs = c.d_cs.p; g.al = c.d_al.p; g.bl = c.d_bl.p;

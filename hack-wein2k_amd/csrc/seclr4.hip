@@ -1344,7 +1344,7 @@ int seclr4(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
   if (el >= eu) { el = -1.0e70; eu = 1.0e70; }  // seclr4.F:149-152
   if (nume <= 0) nume = c.n;
   c.ne = 0;
-  if (c.run.is_complex) return seclr4_complex(c, el, eu, nume, ne_out, nbelw_out);
+  if (c.run().is_complex) return seclr4_complex(c, el, eu, nume, ne_out, nbelw_out);
   return seclr4_real(c, el, eu, nume, ne_out, nbelw_out);
 }
 

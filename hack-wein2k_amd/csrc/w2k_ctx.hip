@@ -4,6 +4,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
 
 #include "w2k_internal.h"
 #include "../../include/w2k_lapw1_test.h"
@@ -65,12 +68,17 @@ template struct DBuf<double>;
 template struct DBuf<int>;
 template struct DBuf<signed char>;
 
+// host -> device copy of setup data; waits for the copy, so the host source
+// (often a temporary) may be released as soon as this returns
 template <class T>
 static int upload(DBuf<T> &b, const T *src, size_t count, hipStream_t s)
 {
   int rc = b.alloc(count);
   if (rc) return rc;
-  if (count) W2K_HIP(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+  if (count) {
+    W2K_HIP(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    W2K_HIP(hipStreamSynchronize(s));
+  }
   return 0;
 }
 
@@ -102,7 +110,7 @@ static std::vector<GEntry> lmsort(const w2k_run_desc *r, int a)
 
 int run_upload(Ctx &c, const w2k_run_desc *r)
 {
-  RunDev &R = c.run;
+  RunDev &R = c.run();
   hipStream_t s = c.stream;
   if (!r || r->nat < 1 || r->ndf < r->nat || r->ltop < 3 || r->ltop > W2K_LMAX || r->lmaxd != W2K_LMAX ||
       r->nnsl < 1 || r->nnsl > W2K_NSLMAX)
@@ -372,9 +380,9 @@ int w2k_set_split_precision(w2k_ctx *x, int slices)
   return 0;
 }
 
-int w2k_create(w2k_ctx **out, int device, const w2k_run_desc *run)
+// a context without per-run data: device, stream and events
+static int ctx_open(w2k_ctx **out, int device)
 {
-  if (!out || !run) return W2K_EINVAL;
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return W2K_ENODEV;
@@ -388,7 +396,18 @@ int w2k_create(w2k_ctx **out, int device, const w2k_run_desc *run)
     return W2K_EDEVICE;
   }
   for (int k = 0; k < W2K_NEVENTS; k++) (void)hipEventCreate(&c.ev[k]);
-  int rc = run_upload(c, run);
+  *out = x;
+  return 0;
+}
+
+int w2k_create(w2k_ctx **out, int device, const w2k_run_desc *run)
+{
+  if (!out || !run) return W2K_EINVAL;
+  *out = nullptr;
+  w2k_ctx *x;
+  int rc = ctx_open(&x, device);
+  if (rc) return rc;
+  rc = run_upload(x->c, run);
   if (rc) {
     w2k_destroy(x);
     return rc;
@@ -397,10 +416,17 @@ int w2k_create(w2k_ctx **out, int device, const w2k_run_desc *run)
   return 0;
 }
 
+static void drop_workers(Ctx &c)
+{
+  for (w2k_ctx *w : c.workers) w2k_destroy(w);
+  c.workers.clear();
+}
+
 int w2k_set_run(w2k_ctx *x, const w2k_run_desc *run)
 {
   if (!x) return W2K_EINVAL;
   if (hipSetDevice(x->c.device) != hipSuccess) return W2K_ENODEV;
+  drop_workers(x->c);  // they read the tables being replaced
   x->c.have_hamilt = 0;
   return run_upload(x->c, run);
 }
@@ -408,6 +434,7 @@ int w2k_set_run(w2k_ctx *x, const w2k_run_desc *run)
 void w2k_destroy(w2k_ctx *x)
 {
   if (!x) return;
+  drop_workers(x->c);
   (void)hipSetDevice(x->c.device);
   if (x->c.stream) (void)hipStreamSynchronize(x->c.stream);
   for (int k = 0; k < W2K_NEVENTS; k++) (void)hipEventDestroy(x->c.ev[k]);
@@ -422,7 +449,7 @@ int w2k_set_basis(w2k_ctx *x, int nv, int nlo, const int *kzz, const double *xk,
 {
   if (!x || nv < 0 || nlo < 0 || nv + nlo < 1 || !kzz || !xk || !yk || !zk || !rk) return W2K_EINVAL;
   Ctx &c = x->c;
-  if (nlo != 0 && nlo != c.run.nlo_cols) return W2K_EINVAL;
+  if (nlo != 0 && nlo != c.run().nlo_cols) return W2K_EINVAL;
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
   const int n = nv + nlo;
   c.nv = nv;
@@ -440,7 +467,7 @@ int w2k_set_basis(w2k_ctx *x, int nv, int nlo, const int *kzz, const double *xk,
   int rc;
   if ((rc = upload(c.d_kzz, kzz, (size_t)3 * n, c.stream))) return rc;
   if ((rc = upload(c.d_kvec, kv.data(), kv.size(), c.stream))) return rc;
-  const int w = c.run.is_complex ? 2 : 1;
+  const int w = c.run().is_complex ? 2 : 1;
   if ((rc = c.d_hs.alloc((size_t)w * n * n))) return rc;
   if ((rc = c.d_hsdiag.alloc((size_t)w * n))) return rc;
   W2K_HIP(hipMemsetAsync(c.d_hs.p, 0, sizeof(double) * w * (size_t)n * n, c.stream));
@@ -473,7 +500,7 @@ int w2k_swap_diag(w2k_ctx *x)
   Ctx &c = x->c;
   if (!c.have_basis) return W2K_ESTATE;
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
-  hipLaunchKernelGGL(k_swap_diag, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run.is_complex, c.d_hs.p,
+  hipLaunchKernelGGL(k_swap_diag, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run().is_complex, c.d_hs.p,
                      c.d_hsdiag.p);
   W2K_STAGE(c.stream, "k_swap_diag");
   return 0;
@@ -520,13 +547,88 @@ int w2k_calkpt(w2k_ctx *x, int nohns, double el, double eu, int nume, int *ne, i
   return w2k_seclr4(x, el, eu, nume, ne, nbelw);
 }
 
+int w2k_calkpt_batched(w2k_ctx *x, int nk, const w2k_basis *bases, int nohns, const double *el, const double *eu,
+                       int nume, int nworkers, int *ne, int *nbelw, int *status, double *eigval, int ldeig,
+                       double *const *z, const int *ldz)
+{
+  if (!x || nk < 0 || nume < 1 || ldeig < nume) return W2K_EINVAL;
+  if (nk == 0) return 0;
+  if (!bases || !el || !eu || !ne || !nbelw || !eigval) return W2K_EINVAL;
+  Ctx &c = x->c;
+  int nmax = 0;
+  for (int q = 0; q < nk; q++) {
+    const w2k_basis &b = bases[q];
+    const int n = b.nv + b.nlo;
+    if (b.nv < 0 || b.nlo < 0 || n < 1 || !b.kzz || !b.xk || !b.yk || !b.zk || !b.rk) return W2K_EINVAL;
+    if (b.nlo != 0 && b.nlo != c.run().nlo_cols) return W2K_EINVAL;
+    if (z && z[q] && (!ldz || ldz[q] < n)) return W2K_EINVAL;
+    nmax = std::max(nmax, n);
+  }
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  // concurrent streams: enough to keep the chip busy with small orders
+  int P = nworkers > 0 ? nworkers : (nmax <= 600 ? 8 : nmax <= 1500 ? 6 : nmax <= 3000 ? 4 : 2);
+  P = std::min(std::min(P, nk), 16);
+  while ((int)c.workers.size() < P) {
+    w2k_ctx *w;
+    const int rc = ctx_open(&w, c.device);
+    if (rc) return rc;
+    w->c.runp = &c.own_run;
+    c.workers.push_back(w);
+  }
+  for (int w = 0; w < P; w++) c.workers[w]->c.hns_slices = c.hns_slices;
+  // longest first: k-points in decreasing N, each to the least loaded stream
+  // (load ~ N^3); a stream's list is then in decreasing N, so its buffers
+  // are sized by its first k-point
+  std::vector<int> order(nk);
+  std::iota(order.begin(), order.end(), 0);
+  auto nof = [&](int q) { return bases[q].nv + bases[q].nlo; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nof(a) > nof(b); });
+  std::vector<std::vector<int>> plan(P);
+  std::vector<double> load(P, 0.0);
+  for (int q : order) {
+    const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    plan[w].push_back(q);
+    const double n = nof(q);
+    load[w] += n * n * n;
+  }
+  std::vector<int> st(nk, 0);
+  std::vector<std::string> err(nk);
+  auto work = [&](int w) {
+    (void)hipSetDevice(c.device);
+    w2k_ctx *wx = c.workers[w];
+    for (int q : plan[w]) {
+      const w2k_basis &b = bases[q];
+      int kne = 0, knb = 0;
+      int rc = w2k_set_basis(wx, b.nv, b.nlo, b.kzz, b.xk, b.yk, b.zk, b.rk);
+      if (!rc) rc = w2k_calkpt(wx, nohns, el[q], eu[q], nume, &kne, &knb);
+      const bool wz = z && z[q];
+      if (!rc) rc = w2k_get_eig(wx, eigval + (size_t)q * ldeig, wz ? z[q] : nullptr, wz ? ldz[q] : nof(q));
+      ne[q] = rc ? 0 : kne;
+      nbelw[q] = rc ? 0 : knb;
+      st[q] = rc;
+      if (rc) err[q] = w2k_last_error();
+    }
+  };
+  std::vector<std::thread> th;
+  for (int w = 1; w < P; w++) th.emplace_back(work, w);
+  work(0);
+  for (auto &t : th) t.join();
+  if (status) std::copy(st.begin(), st.end(), status);
+  for (int q = 0; q < nk; q++)
+    if (st[q]) {
+      snprintf(g_last_error, sizeof g_last_error, "k-point %d: %s", q, err[q].c_str());
+      return st[q];
+    }
+  return 0;
+}
+
 int w2k_get_hs(w2k_ctx *x, double *hs, int ld, double *hsdiag)
 {
   if (!x) return W2K_EINVAL;
   Ctx &c = x->c;
   if (!c.have_basis || ld < c.n) return W2K_EINVAL;
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
-  const int w = c.run.is_complex ? 2 : 1;
+  const int w = c.run().is_complex ? 2 : 1;
   if (hs)
     W2K_HIP(hipMemcpy2DAsync(hs, sizeof(double) * w * (size_t)ld, c.d_hs.p, sizeof(double) * w * (size_t)c.n,
                              sizeof(double) * w * (size_t)c.n, c.n, hipMemcpyDeviceToHost, c.stream));
@@ -542,7 +644,7 @@ int w2k_put_hs(w2k_ctx *x, const double *hs, int ld, const double *hsdiag)
   Ctx &c = x->c;
   if (!c.have_basis || ld < c.n) return W2K_EINVAL;
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
-  const int w = c.run.is_complex ? 2 : 1;
+  const int w = c.run().is_complex ? 2 : 1;
   if (hs)
     W2K_HIP(hipMemcpy2DAsync(c.d_hs.p, sizeof(double) * w * (size_t)c.n, hs, sizeof(double) * w * (size_t)ld,
                              sizeof(double) * w * (size_t)c.n, c.n, hipMemcpyHostToDevice, c.stream));
@@ -558,11 +660,11 @@ int w2k_get_diag(w2k_ctx *x, double *hs_diag, double *hsdiag)
   Ctx &c = x->c;
   if (!c.have_basis) return W2K_ESTATE;
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
-  const int w = c.run.is_complex ? 2 : 1;
+  const int w = c.run().is_complex ? 2 : 1;
   int rc;
   if ((rc = c.w_v.alloc((size_t)w * c.n))) return rc;
   if (hs_diag) {
-    hipLaunchKernelGGL(k_diag_copy, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run.is_complex, 0,
+    hipLaunchKernelGGL(k_diag_copy, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run().is_complex, 0,
                        c.d_hs.p, c.w_v.p);
     W2K_STAGE(c.stream, "k_diag_copy");
     W2K_HIP(hipMemcpyAsync(hs_diag, c.w_v.p, sizeof(double) * w * c.n, hipMemcpyDeviceToHost, c.stream));
@@ -578,12 +680,12 @@ int w2k_put_diag(w2k_ctx *x, const double *hs_diag, const double *hsdiag)
   Ctx &c = x->c;
   if (!c.have_basis) return W2K_ESTATE;
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
-  const int w = c.run.is_complex ? 2 : 1;
+  const int w = c.run().is_complex ? 2 : 1;
   int rc;
   if ((rc = c.w_v.alloc((size_t)w * c.n))) return rc;
   if (hs_diag) {
     W2K_HIP(hipMemcpyAsync(c.w_v.p, hs_diag, sizeof(double) * w * c.n, hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_diag_copy, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run.is_complex, 1,
+    hipLaunchKernelGGL(k_diag_copy, dim3((c.n + 255) / 256), dim3(256), 0, c.stream, c.n, c.run().is_complex, 1,
                        c.d_hs.p, c.w_v.p);
     W2K_STAGE(c.stream, "k_diag_copy");
   }
@@ -598,7 +700,7 @@ int w2k_get_albl(w2k_ctx *x, double *al, double *bl)
   Ctx &c = x->c;
   if (!c.have_hamilt) return W2K_ESTATE;
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
-  const size_t cnt = (size_t)c.n * c.run.ltop * c.run.nat;
+  const size_t cnt = (size_t)c.n * c.run().ltop * c.run().nat;
   if (al) W2K_HIP(hipMemcpyAsync(al, c.d_al.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c.stream));
   if (bl) W2K_HIP(hipMemcpyAsync(bl, c.d_bl.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c.stream));
   W2K_HIP(hipStreamSynchronize(c.stream));
@@ -611,7 +713,7 @@ int w2k_get_eig(w2k_ctx *x, double *eigval, double *z, int ldz)
   Ctx &c = x->c;
   if (ldz < c.n) return W2K_EINVAL;
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
-  const int w = c.run.is_complex ? 2 : 1;
+  const int w = c.run().is_complex ? 2 : 1;
   if (c.ne > 0) {
     if (eigval) W2K_HIP(hipMemcpyAsync(eigval, c.d_eig.p, sizeof(double) * c.ne, hipMemcpyDeviceToHost, c.stream));
     if (z)

@@ -91,7 +91,13 @@ struct RunDev {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  RunDev run;
+  // per-run tables: this context's own, or (worker contexts of a batched
+  // call) the parent's, read-only while k-points run
+  RunDev own_run;
+  RunDev *runp = &own_run;
+  RunDev &run() { return *runp; }
+  // worker contexts of batched k-points (owned; share own_run)
+  std::vector<w2k_ctx *> workers;
   // basis
   int nv = 0, nlo = 0, n = 0;
   int have_basis = 0, have_hamilt = 0;

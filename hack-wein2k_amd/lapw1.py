@@ -158,6 +158,13 @@ class Basis:
         return self.nv + self.nlo
 
 
+class _BasisDesc(ctypes.Structure):
+    """w2k_basis of include/w2k_lapw1.h."""
+    _fields_ = [("nv", ctypes.c_int), ("nlo", ctypes.c_int), ("kzz", ctypes.POINTER(ctypes.c_int)),
+                ("xk", ctypes.POINTER(ctypes.c_double)), ("yk", ctypes.POINTER(ctypes.c_double)),
+                ("zk", ctypes.POINTER(ctypes.c_double)), ("rk", ctypes.POINTER(ctypes.c_double))]
+
+
 class Lapw1Error(RuntimeError):
     def __init__(self, where: str, code: int, detail: str = ""):
         msg = f"{where} failed with status {code}"
@@ -201,6 +208,9 @@ def load_library(path: Optional[str] = None):
     lib.w2k_last_error.restype = ctypes.c_char_p
     lib.w2k_set_kernel_timing.argtypes = [V, ctypes.c_int]
     lib.w2k_set_split_precision.argtypes = [V, ctypes.c_int]
+    lib.w2k_calkpt_batched.argtypes = [V, ctypes.c_int, ctypes.POINTER(_BasisDesc), ctypes.c_int, _c_dbl_p, _c_dbl_p,
+                                       ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p,
+                                       ctypes.c_int, ctypes.POINTER(_c_dbl_p), _c_int_p]
     if path is None:
         _LIB = lib
     return lib
@@ -272,6 +282,34 @@ class Lapw1:
         self._chk("CALKPT", self.lib.w2k_calkpt(self.ctx, int(nohns), float(el), float(eu), int(nume), ctypes.byref(ne), ctypes.byref(nb)))
         self.ne = ne.value
         return ne.value, nb.value
+
+    def calkpt_batched(self, bases, el, eu, nume: int, nworkers: int = 0, nohns: bool = False,
+                       vectors: bool = True):
+        """Batched k-points (w2k_calkpt_batched): CALKPT for every basis in
+        `bases` (ragged N) on concurrent device streams.  el/eu: one window
+        per k-point (or scalars).  Returns one (ne, nbelw, eigval, z or None)
+        per k-point; raises Lapw1Error if any k-point failed."""
+        nk = len(bases)
+        el = np.broadcast_to(np.asarray(el, np.float64), (nk,)).copy()
+        eu = np.broadcast_to(np.asarray(eu, np.float64), (nk,)).copy()
+        descs = (_BasisDesc * nk)()
+        for q, b in enumerate(bases):
+            descs[q] = _BasisDesc(b.nv, b.nlo, b.kzz.ctypes.data_as(_c_int_p), _dp(b.xk), _dp(b.yk), _dp(b.zk),
+                                  _dp(b.rk))
+        ne = np.zeros(nk, np.int32)
+        nb = np.zeros(nk, np.int32)
+        st = np.zeros(nk, np.int32)
+        ev = np.zeros((nk, int(nume)), np.float64)
+        zs = [np.zeros((b.n, int(nume)), self.cdt, order="F") for b in bases] if vectors else None
+        zp = (_c_dbl_p * nk)(*[_dp(a) for a in zs]) if vectors else None
+        ldz = np.array([b.n for b in bases], np.int32)
+        rc = self.lib.w2k_calkpt_batched(self.ctx, nk, descs, int(nohns), _dp(el), _dp(eu), int(nume), int(nworkers),
+                                         ne.ctypes.data_as(_c_int_p), nb.ctypes.data_as(_c_int_p),
+                                         st.ctypes.data_as(_c_int_p), _dp(ev), int(nume), zp,
+                                         ldz.ctypes.data_as(_c_int_p))
+        self._chk("w2k_calkpt_batched", rc)
+        return [(int(ne[q]), int(nb[q]), ev[q, :ne[q]].copy(), zs[q][:, :ne[q]] if vectors else None)
+                for q in range(nk)]
 
     def get_hs(self) -> Tuple[np.ndarray, np.ndarray]:
         n = self.basis.n

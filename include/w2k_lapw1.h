@@ -127,6 +127,32 @@ int w2k_seclr4(w2k_ctx *ctx, double el, double eu, int nume, int *ne, int *nbelw
 int w2k_calkpt(w2k_ctx *ctx, int nohns, double el, double eu, int nume,
                int *ne, int *nbelw);
 
+/* One k-point's basis for the batched call (the arguments of w2k_set_basis). */
+typedef struct w2k_basis {
+  int nv, nlo;
+  const int *kzz;                              /* (3, nv+nlo) */
+  const double *xk, *yk, *zk, *rk;             /* (nv+nlo) */
+} w2k_basis;
+
+/* Batched k-points: the CALKPT hot path (as w2k_calkpt) for nk k-points of
+ * one run, each with its own basis (ragged N) and window (el[q], eu[q]],
+ * at most nume eigenpairs each, run concurrently on `nworkers` streams of
+ * the context's device (0: chosen from the largest N).  The k-points of one
+ * stream are taken in decreasing N, assigned longest-first to the least
+ * loaded stream.  Every k-point's result is identical to a w2k_calkpt call
+ * on its own.  Outputs, per k-point q: ne[q], nbelw[q], status[q] (0 or the
+ * code w2k_calkpt would return; may be NULL), EIGVAL in
+ * eigval[q*ldeig .. q*ldeig+ne[q]) (ldeig >= nume), and, when z and z[q] are
+ * non-NULL, Z(1:N_q, 1:ne[q]) at z[q] with leading dimension ldz[q] >= N_q
+ * (complex interleaved).  Returns 0 when every k-point succeeded, else the
+ * status of the first failed k-point in index order.  The worker streams and
+ * their workspaces persist in the context for later batches.  An extended
+ * form with no counterpart in the reference (lapw1 runs k-points in a
+ * serial loop, lapw1.F:50-66). */
+int w2k_calkpt_batched(w2k_ctx *ctx, int nk, const w2k_basis *bases, int nohns, const double *el,
+                       const double *eu, int nume, int nworkers, int *ne, int *nbelw, int *status,
+                       double *eigval, int ldeig, double *const *z, const int *ldz);
+
 /* Host <-> device transfers of the reference-layout arrays (ld >= N).  */
 int w2k_get_hs(w2k_ctx *ctx, double *hs, int ld, double *hsdiag);
 int w2k_put_hs(w2k_ctx *ctx, const double *hs, int ld, const double *hsdiag);

@@ -3,7 +3,9 @@
 !
 ! Runs the program sequence of lapw1.F:33-84 with CALKPT (calkpt.F:143-197)
 ! unrolled, so that the module state can be written out (argument 'nodump':
-! timings only; 'eigonly': per k-point NE, NBELW and EIGVAL only)
+! timings only; 'eigonly': per k-point NE, NBELW and EIGVAL only; 'basis':
+! the run state and per k-point the basis, window, NE, NBELW and EIGVAL, but
+! no matrices)
 !   - once per run after INILPW/INIKPT            -> run.bin
 !   - per k-point at CALKPT entry (basis), after HAMILT, after HNS and after
 !     SECLR4                                       -> k<NNNN>.bin
@@ -87,7 +89,7 @@ program ref_dump
   use out, only : WARP, WARP1, WARP2, WARP3
   implicit none
   include 'param.inc'
-  logical :: nmat_only, dodump, eigdump
+  logical :: nmat_only, dodump, eigdump, basisdump
   integer :: numwk, info, kpoint, u, nn, i, j
   integer, allocatable :: ltmp(:,:)
   character(len=80) :: deffn, fname
@@ -99,12 +101,17 @@ program ref_dump
   deffn = 'lapw1.def'
   dodump = .true.
   eigdump = .false.
+  basisdump = .false.
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg)
     if (arg(1:6) == 'nodump') dodump = .false.
     if (arg(1:7) == 'eigonly') then
       dodump = .false.
       eigdump = .true.
+    end if
+    if (arg(1:5) == 'basis') then
+      dodump = .false.
+      basisdump = .true.
     end if
   end if
   call init_parallel
@@ -116,7 +123,7 @@ program ref_dump
   if (info /= 0) stop 'ref_dump: INIKPT failed'
 
   u = 77
-  if (dodump) then
+  if (dodump .or. basisdump) then
   open(u, file='run.bin', access='stream', form='unformatted', status='replace')
   call ws(u, 'NAT', NAT)
   call ws(u, 'NDF', NDF)
@@ -190,7 +197,7 @@ program ref_dump
     NVAA = NV
     EIGVAL(1) = EL
     EIGVAL(2) = EU
-    if (dodump) then
+    if (dodump .or. basisdump) then
     write(fname, '(a,i4.4,a)') 'k', kpoint, '.bin'
     open(u, file=fname, access='stream', form='unformatted', status='replace')
     call ws(u, 'NV', NV)
@@ -238,12 +245,14 @@ program ref_dump
     if (iter) stop 'ref_dump: iterative diagonalisation is not part of the dump'
     call SECLR4
     call system_clock(c1); td = td + dble(c1 - c0)/dble(crate)
-    if (dodump) then
+    if (dodump .or. basisdump) then
     call ws(u, 'NE', NE)
     call ws(u, 'NBELW', NBELW)
     call wd(u, 'EIGVAL', EIGVAL, [max(NE,1)])
+    if (dodump) then
 !_REAL    call wd(u, 'Z', Z, [nn, max(NE,1)])
 !_COMPLEX    call wz(u, 'Z', Z, [nn, max(NE,1)])
+    end if
     close(u)
     end if
     if (eigdump) then
