@@ -147,6 +147,8 @@ def main():
                     help="skip the extra k-point with per-launch SYMV events (for profiler passes)")
     ap.add_argument("--split-slices", type=int, default=4,
                     help="INT8 digits per element for the split-precision pass (0: FP64 only)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-k measurement (64 k-points of the 8-atom cell)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 512 MiB device copy (calibrates the profiler's byte counters)")
     args = ap.parse_args()
@@ -286,7 +288,62 @@ def main():
     }
     if copy is not None:
         out["copy_calibration"] = copy
+    if not args.no_batched:
+        out["batched"] = batched_kpoints(L)
     print(json.dumps(out))
+
+
+BATCH_FIXTURE = os.path.join(REPO, "tests", "golden", "c4_ortho8.npz")
+
+
+def batched_kpoints(L):
+    """64 k-points of the 8-atom orthorhombic complex cell (N 1936-1978):
+    a serial loop of single k-point calls against one batched call, both over
+    every k-point with its own window; eigenvalues checked against the
+    reference (1e-10 Ry) and the batch against the loop (bit-identical)."""
+    raw = np.load(BATCH_FIXTURE)
+    run = L.RunData({k[4:]: raw[k] for k in raw.files if k.startswith("run/")}, bool(int(raw["meta/complex"])))
+    nk = int(raw["meta/nk"])
+    kds = [{k[len(f"k{q}/"):]: raw[k] for k in raw.files if k.startswith(f"k{q}/")} for q in range(nk)]
+    bases = [L.Basis(int(kd["NV"]), int(kd["NLO"]), kd["KZZ"], kd["XK"], kd["YK"], kd["ZK"], kd["RK"]) for kd in kds]
+    el = [float(kd["EL"]) for kd in kds]
+    eu = [float(kd["EU"]) for kd in kds]
+    nume = max(int(kd["NUME"]) for kd in kds)
+    ctx = L.Lapw1(run)
+    # serial loop, largest basis first (one context, one k-point per call)
+    order = sorted(range(nk), key=lambda q: -bases[q].n)
+    ctx.set_basis(bases[order[0]])
+    ctx.calkpt(el[order[0]], eu[order[0]], nume)  # warm-up (workspaces)
+    ctx.sync()
+    ser = [None] * nk
+    t0 = time.perf_counter()
+    for q in order:
+        ctx.set_basis(bases[q])
+        ne, nb = ctx.calkpt(el[q], eu[q], nume)
+        ev, z = ctx.get_eig()
+        ser[q] = (ne, nb, ev, z)
+    t_serial = time.perf_counter() - t0
+    _progress(f"batched-k cell: serial loop {t_serial:.3f} s")
+    big = order[:8]  # warm-up (worker streams sized by the largest bases)
+    ctx.calkpt_batched([bases[q] for q in big], [el[q] for q in big], [eu[q] for q in big], nume)
+    t0 = time.perf_counter()
+    bat = ctx.calkpt_batched(bases, el, eu, nume)
+    t_batch = time.perf_counter() - t0
+    ctx.close()
+    _progress(f"batched-k cell: one batched call {t_batch:.3f} s")
+    dmax = 0.0
+    same = True
+    for q in range(nk):
+        ne, nb, ev, z = bat[q]
+        ok = ne == int(kds[q]["NE"]) and nb == int(kds[q]["NBELW"])
+        dmax = max(dmax, float(np.max(np.abs(ev - kds[q]["EIGVAL"]))) if ok else float("inf"))
+        same = same and ser[q][0] == ne and np.array_equal(ser[q][2], ev) and np.array_equal(ser[q][3], z)
+    ns = [b.n for b in bases]
+    return {"workload": f"8-atom orthorhombic cell, 2 species, complex, {nk} k-points, N {min(ns)}-{max(ns)}, "
+                        f"NE {min(int(k['NE']) for k in kds)}-{max(int(k['NE']) for k in kds)}",
+            "serial_s_per_kpoint": round(t_serial / nk, 4), "batched_s_per_kpoint": round(t_batch / nk, 4),
+            "speedup": round(t_serial / t_batch, 2), "max_abs_eig_dev_ry": dmax, "batched_equals_serial": same,
+            "cpu_reference_s_per_kpoint_8_threads_build_host": round(float(raw["meta/ref_seconds"]) / nk, 3)}
 
 
 def _sum_lindex(run) -> int:

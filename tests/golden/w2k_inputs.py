@@ -446,6 +446,40 @@ def case_rocksalt(n: int, rkmax: float, name: str, seed: int) -> Case:
                 ops_inversion(), rkmax=rkmax, kpoints=[(0, 0, 0, 1)], window=(-9.0, 0.5), seed=seed)
 
 
+def case_c4_ortho8(nk: int = 64) -> Case:
+    """Batched-k cell: 8-atom orthorhombic P cell, two species (4 x Z=12,
+    RMT 2.0 and 4 x Z=8, RMT 1.6) at seeded random positions (no inversion,
+    so the complex build; nearest distances above the RMT sums), RKmax 7,
+    lmax 10, lnsmax 4, no local orbitals, 64 seeded general k-points; basis
+    sizes around 1900-2000."""
+    rng = np.random.Generator(np.random.PCG64(SEED_BASE + 3))
+    alat = (10.4, 11.2, 11.85)
+    species = [("Mg", 12.0, 2.0)] * 4 + [("O", 8.0, 1.6)] * 4
+    pos: List[Tuple[float, float, float]] = []
+    for _, _, r in species:
+        while True:
+            f = tuple(float(x) for x in rng.uniform(0.0, 1.0, size=3))
+            ok = True
+            for (g, (_, _, r2)) in zip(pos, species):
+                d = np.array(f) - np.array(g)
+                d -= np.round(d)
+                if np.linalg.norm(d * np.array(alat)) < r + r2 + 0.3:
+                    ok = False
+                    break
+            if ok:
+                pos.append(f)
+                break
+    vlms = [[(1, 0), (2, 0), (2, 2), (3, 1), (4, 0)], [(1, 1), (2, -1), (3, 0), (4, 2)]]
+    atoms = [AtomType(nm, z, r, [p], vlm=vlms[i % 2]) for i, ((nm, z, r), p) in enumerate(zip(species, pos))]
+    ks = []
+    while len(ks) < nk:
+        k = tuple(int(x) for x in rng.integers(0, 8, size=3)) + (8,)
+        if k not in ks:
+            ks.append(k)
+    return Case("c4_ortho8", "8-atom orthorhombic cell, 2 species, seeded positions", "P", alat, atoms,
+                ops_identity(), rkmax=7.0, kpoints=ks, window=(-9.0, 12.0), seed=SEED_BASE + 3, complex=True)
+
+
 def _with_window(c: Case, w) -> Case:
     c.window = w
     return c
@@ -457,4 +491,5 @@ CASES = {
     "lo_cplx": lambda: case_lo_mult2(True),
     "rs8": lambda: case_rocksalt(1, 5.0, "rs8", SEED_BASE + 21),
     "c2_rs64": lambda: _with_window(case_rocksalt(2, 7.0, "c2_rs64", SEED_BASE + 1), (-9.0, 12.0)),
+    "c4_ortho8": case_c4_ortho8,
 }

@@ -98,6 +98,25 @@ def test_batched_reuses_streams_and_subsets():
         assert np.array_equal(out2[i][2], out1[q][2]) and np.array_equal(out2[i][3], out1[q][3])
 
 
+def test_batched_c4_sixty_four_kpoints():
+    """The batched-k cell: 64 complex k-points, N from 1936 to 1978 (no local
+    orbitals), all in one batch; NE, NBELW and eigenvalues against the
+    reference for every k-point, and four of them bit-identical to single calls."""
+    fx = Fixture("c4_ortho8")
+    ks = list(range(fx.nk))
+    out = _batch(fx, ks, 0)
+    worst = 0.0
+    for q, (ne, nb, ev, z) in zip(ks, out):
+        kd = fx.k(q)
+        assert ne == int(kd["NE"]) and nb == int(kd["NBELW"]), f"k{q}"
+        worst = max(worst, float(np.max(np.abs(ev - kd["EIGVAL"]))))
+    print(f"c4_ortho8: 64 k-points, max |dE| = {worst:.3e} Ry")
+    assert worst < TOL_E
+    for q in (0, 17, 40, 63):
+        ne1, nb1, ev1, z1 = _single(fx, q)
+        assert np.array_equal(out[q][2], ev1) and np.array_equal(out[q][3], z1), f"k{q}"
+
+
 def test_batched_rejects_bad_arguments():
     fx = Fixture("c1_fcc")
     ctx = L.Lapw1(fx.run())
