@@ -465,6 +465,7 @@ int w2k_set_basis(w2k_ctx *x, int nv, int nlo, const int *kzz, const double *xk,
     kv[4 * i + 3] = rk[i];
   }
   int rc;
+  c.h_kzz.assign(kzz, kzz + (size_t)3 * n);
   if ((rc = upload(c.d_kzz, kzz, (size_t)3 * n, c.stream))) return rc;
   if ((rc = upload(c.d_kvec, kv.data(), kv.size(), c.stream))) return rc;
   const int w = c.run().is_complex ? 2 : 1;
@@ -620,6 +621,15 @@ int w2k_calkpt_batched(w2k_ctx *x, int nk, const w2k_basis *bases, int nohns, co
       return st[q];
     }
   return 0;
+}
+
+int w2k_adjoint_warp(w2k_ctx *x, const double *f, double *e_band, double *g_warp)
+{
+  if (!x || (!f && x->c.ne > 0)) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  return adjoint_warp(c, f, e_band, g_warp);
 }
 
 int w2k_get_hs(w2k_ctx *x, double *hs, int ld, double *hsdiag)

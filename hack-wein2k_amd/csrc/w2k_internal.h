@@ -102,6 +102,7 @@ struct Ctx {
   int nv = 0, nlo = 0, n = 0;
   int have_basis = 0, have_hamilt = 0;
   DBuf<int> d_kzz;          // [n][3]
+  std::vector<int> h_kzz;   // host copy of KZZ (index bounds for the adjoint)
   DBuf<double> d_kvec;      // [n][4]: xk, yk, zk, rk
   DBuf<double> d_hs;        // n*n (complex: 2*n*n)
   DBuf<double> d_hsdiag;    // n (complex: 2n)
@@ -142,6 +143,8 @@ int run_upload(Ctx &c, const w2k_run_desc *r);
 int hamilt(Ctx &c);
 int hns(Ctx &c);
 int seclr4(Ctx &c, double el, double eu, int nume, int *ne, int *nbelw);
+// band-energy adjoint with respect to the warped interstitial box (after SECLR4)
+int adjoint_warp(Ctx &c, const double *f_host, double *e_band, double *g_warp);
 // elapsed milliseconds between events e0 and e1 into last_ms[slot]
 void take_ms(Ctx &c, int slot, int e0, int e1);
 

@@ -208,6 +208,7 @@ def load_library(path: Optional[str] = None):
     lib.w2k_last_error.restype = ctypes.c_char_p
     lib.w2k_set_kernel_timing.argtypes = [V, ctypes.c_int]
     lib.w2k_set_split_precision.argtypes = [V, ctypes.c_int]
+    lib.w2k_adjoint_warp.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p]
     lib.w2k_calkpt_batched.argtypes = [V, ctypes.c_int, ctypes.POINTER(_BasisDesc), ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                        ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p,
                                        ctypes.c_int, ctypes.POINTER(_c_dbl_p), _c_int_p]
@@ -310,6 +311,19 @@ class Lapw1:
         self._chk("w2k_calkpt_batched", rc)
         return [(int(ne[q]), int(nb[q]), ev[q, :ne[q]].copy(), zs[q][:, :ne[q]] if vectors else None)
                 for q in range(nk)]
+
+    def adjoint_warp(self, f) -> Tuple[float, np.ndarray]:
+        """Band energy sum_n f_n eps_n (weights held fixed) of the last solved
+        k-point and its gradient with respect to the warped interstitial box:
+        complex array shaped like the run's WARP, dE/dRe + i dE/dIm."""
+        f = np.ascontiguousarray(np.asarray(f, np.float64))
+        if f.shape != (self.ne,):
+            raise ValueError(f"need {self.ne} weights")
+        e = ctypes.c_double(0.0)
+        w1, w2, w3 = self.run.warp
+        g = np.zeros((2 * w1 + 1, 2 * w2 + 1, 2 * w3 + 1), np.complex128, order="F")
+        self._chk("w2k_adjoint_warp", self.lib.w2k_adjoint_warp(self.ctx, _dp(f), ctypes.byref(e), _dp(g)))
+        return e.value, g
 
     def get_hs(self) -> Tuple[np.ndarray, np.ndarray]:
         n = self.basis.n

@@ -153,6 +153,19 @@ int w2k_calkpt_batched(w2k_ctx *ctx, int nk, const w2k_basis *bases, int nohns, 
                        const double *eu, int nume, int nworkers, int *ne, int *nbelw, int *status,
                        double *eigval, int ldeig, double *const *z, const int *ldz);
 
+/* Band-energy adjoint at the k-point of the last w2k_seclr4 / w2k_calkpt
+ * (an extended form with no counterpart in the reference).  With the NE
+ * weights f[0..NE) held fixed, E_band = sum_n f[n] EIGVAL(n) is written to
+ * *e_band, and, when g_warp is non-NULL, its gradient with respect to every
+ * entry of the run's warped interstitial box WARP(Q) (w2k_run_desc.warp,
+ * hamilt.F:335-345): g = dE/dRe WARP(Q) + i dE/dIm WARP(Q), interleaved, in
+ * the box's own layout ((2 warp1+1) x (2 warp2+1) x (2 warp3+1), Fortran
+ * order).  By Hellmann-Feynman g(Q) = sum over plane-wave pairs I > J with
+ * KZZ_J - KZZ_I = Q of 2 conj(Gamma(I,J)), plus sum_I Gamma(I,I) at Q = 0,
+ * Gamma = sum_n f_n z_n z_n^H (the real build: real parts only).  The
+ * radial V_LM part of the adjoint is not provided by this entry point. */
+int w2k_adjoint_warp(w2k_ctx *ctx, const double *f, double *e_band, double *g_warp);
+
 /* Host <-> device transfers of the reference-layout arrays (ld >= N).  */
 int w2k_get_hs(w2k_ctx *ctx, double *hs, int ld, double *hsdiag);
 int w2k_put_hs(w2k_ctx *ctx, const double *hs, int ld, const double *hsdiag);
