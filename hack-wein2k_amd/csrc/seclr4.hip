@@ -56,24 +56,25 @@ __device__ __forceinline__ double block_sum(double v, double *red)
   return red[32];
 }
 
-// 1. Sw (upper incl. diagonal) and Hf (full symmetric) from HS/HSDIAG
-__global__ void k_prep(int n, const double *__restrict__ hs, const double *__restrict__ hsdiag,
+// 1. Sw (upper incl. diagonal) and Hf (full symmetric, leading dimension ldh)
+// from HS/HSDIAG
+__global__ void k_prep(int n, int ldh, const double *__restrict__ hs, const double *__restrict__ hsdiag,
                        double *__restrict__ sw, double *__restrict__ hf)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= n) return;
-  const size_t o = (size_t)i + (size_t)n * j;
+  const size_t o = (size_t)i + (size_t)n * j, oh = (size_t)i + (size_t)ldh * j;
   const double v = hs[o];
   if (i < j) {
     sw[o] = v;
-    hf[o] = hs[(size_t)j + (size_t)n * i];
+    hf[oh] = hs[(size_t)j + (size_t)n * i];
   } else if (i == j) {
     sw[o] = hsdiag[i];
-    hf[o] = v;
+    hf[oh] = v;
   } else {
     sw[o] = 0.0;
-    hf[o] = v;
+    hf[oh] = v;
   }
 }
 
@@ -174,13 +175,13 @@ __global__ void k_fill(size_t cnt, double *p, double v)
   if (i < cnt) p[i] = v;
 }
 
-// C(j,i) = C(i,j), i > j
-__global__ void k_mirror_lower(int n, double *c)
+// C(j,i) = C(i,j), i > j (leading dimension ldc)
+__global__ void k_mirror_lower(int n, int ldc, double *c)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= n || i <= j) return;
-  c[(size_t)j + (size_t)n * i] = c[(size_t)i + (size_t)n * j];
+  c[(size_t)j + (size_t)ldc * i] = c[(size_t)i + (size_t)ldc * j];
 }
 
 // ------------------------------------------------------------------ tridiagonalisation
@@ -199,6 +200,7 @@ __global__ void k_mirror_lower(int n, double *c)
 //                partial per block.
 struct TriWork {
   int n, p0, nb;
+  int lda;            // leading dimension of a (a multiple of 16: 128-byte aligned columns)
   double *a;          // n x n, both triangles
   double *P;          // n x 3 TNB panel buffer
   double *d, *e, *tau, *scal;
@@ -222,7 +224,7 @@ __device__ __forceinline__ double tri_reflector(const TriWork &g, int i, int lan
   for (int b = lane; b < nnorm; b += 64) xs += g.norm[b];
   xs = wave_sum(xs);
   xs = __shfl(xs, 0, 64);
-  const double alpha = g.a[(size_t)n * i + i + 1];
+  const double alpha = g.a[(size_t)g.lda * i + i + 1];
   const double xnorm = sqrt(xs);
   double tau = 0.0, beta = alpha, scal = 0.0;
   if (xnorm != 0.0) {
@@ -249,17 +251,18 @@ __device__ __forceinline__ double tri_reflector(const TriWork &g, int i, int lan
 // v(i+1) = 1) and the partial dot products V^T v, W^T v of the block's rows.
 __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
 {
-  const int n = g.n, r0 = i + 1, ii = i - g.p0, nb = g.nb;
+  const int n = g.n, lda = g.lda, r0 = i + 1, ii = i - g.p0, nb = g.nb;
+  const int base = r0 & ~15;  // tile grid origin: rows on 128-byte boundaries
   const int lane = threadIdx.x;
   const int nblk = nt * (nt + 1) / 2;
   const int bid = blockIdx.x;
-  const double *ai = g.a + (size_t)n * i;
+  const double *ai = g.a + (size_t)lda * i;
   double t[SYMV_T];
   if (bid >= nblk) {
     // row block I: V column and panel dot products
     const int I = bid - nblk;
-    const int R = r0 + SYMV_T * I + lane;
-    const bool live = R < n;
+    const int R = base + SYMV_T * I + lane;
+    const bool live = R >= r0 && R < n;
     const int Rs = live ? R : r0;
     const double a_r = ai[Rs];
     if (ii > 0) {
@@ -271,9 +274,10 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
     }
     const double sc = tri_reflector(g, i, lane, false);
     const double vr = live ? (R == r0 ? 1.0 : a_r * sc) : 0.0;
-    double *V = g.P + (size_t)n * ii, *V2 = g.P + (size_t)n * (2 * nb + ii);
-    if (live) { V[R] = vr; V2[R] = vr; }
-    for (int q = I * SYMV_T + lane; q < r0; q += nt * SYMV_T) { V[q] = 0.0; V2[q] = 0.0; }
+    if (live) {
+      g.P[(size_t)n * ii + R] = vr;
+      g.P[(size_t)n * (2 * nb + ii) + R] = vr;
+    }
     if (ii > 0) {
       // t[k] = V(R,k) v(R), t[TNB+k] = W(R,k) v(R) for k < ii, summed over
       // the lanes: lane q ends with a_q (q < TNB) or b_(q-TNB)
@@ -293,50 +297,59 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
   while ((I + 1) * (I + 2) / 2 <= bid) I++;
   while (I * (I + 1) / 2 > bid) I--;
   const int J = bid - I * (I + 1) / 2;
-  const int R = r0 + SYMV_T * I + lane;
-  const int C0 = r0 + SYMV_T * J;
-  __shared__ double vs[SYMV_T];
-  const bool live = R < n;
+  const int R = base + SYMV_T * I + lane;
+  const int C0 = base + SYMV_T * J;
+  const bool live = R >= r0 && R < n;
   const bool off = (I != J);
   const int m = min(SYMV_T, n - C0);
-  const int Rs = live ? R : r0;
+  const int Rl = min(R, n - 1);  // the lane's row for the loads (in bounds)
   // every load of the tile and of v is issued before the first use: clamped
   // (in-bounds) addresses, values masked afterwards
   const double a_c = ai[min(C0 + lane, n - 1)];
-  const double a_r = ai[Rs];
-  const double *ap = g.a + (size_t)Rs + (size_t)n * C0;
+  const double a_r = ai[Rl];
+  const double *ap = g.a + (size_t)Rl + (size_t)lda * C0;
 #pragma unroll
-  for (int j = 0; j < SYMV_T; j++) t[j] = ap[(size_t)n * min(j, m - 1)];
+  for (int j = 0; j < SYMV_T; j++) t[j] = ap[(size_t)lda * min(j, m - 1)];
   const double sc = tri_reflector(g, i, lane, bid == 0);
-  vs[lane] = (C0 + lane < n) ? (C0 + lane == r0 ? 1.0 : a_c * sc) : 0.0;
+  const int c = C0 + lane;
+  const double vc = (c >= r0 && c < n) ? (c == r0 ? 1.0 : a_c * sc) : 0.0;  // v of the lane's column
   const double vr = live ? (R == r0 ? 1.0 : a_r * sc) : 0.0;
   if (m < SYMV_T || !live) {
 #pragma unroll
     for (int j = 0; j < SYMV_T; j++) t[j] = (live && j < m) ? t[j] : 0.0;
   }
-  __syncthreads();
+  // row sums: v(C0 + j) broadcast from lane j through scalar registers
+  const long long vcb = __double_as_longlong(vc);
+  const int vlo = (int)(unsigned)(vcb & 0xffffffffll), vhi = (int)(unsigned)((unsigned long long)vcb >> 32);
   double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
 #pragma unroll
   for (int j = 0; j < SYMV_T; j += 4) {
-    y0 += t[j] * vs[j];
-    y1 += t[j + 1] * vs[j + 1];
-    y2 += t[j + 2] * vs[j + 2];
-    y3 += t[j + 3] * vs[j + 3];
+    double vj[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const unsigned lo = (unsigned)__builtin_amdgcn_readlane(vlo, j + u);
+      const unsigned hi = (unsigned)__builtin_amdgcn_readlane(vhi, j + u);
+      vj[u] = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    }
+    y0 += t[j] * vj[0];
+    y1 += t[j + 1] * vj[1];
+    y2 += t[j + 2] * vj[2];
+    y3 += t[j + 3] * vj[3];
   }
   const double yrow = (y0 + y1) + (y2 + y3);
   if (live) g.part[(size_t)J * n + R] = yrow;
-  double vy = yrow * vr;
+  // v^T y over the tile: an off-diagonal tile adds v_I^T A_IJ v_J through its
+  // rows and the same number again through its columns
+  double vy = wave_sum(yrow * vr);
+  if (lane == 0) g.vy[bid] = off ? 2.0 * vy : vy;
   if (off) {
+    double *pc = g.part + (size_t)I * n + c;
+    const bool st = lane < m && c >= r0;
 #pragma unroll
     for (int j = 0; j < SYMV_T; j++) t[j] *= vr;
     const double cs = wave_transpose_sum(t, lane);  // column C0 + lane of A_IJ^T v_I
-    if (lane < m) {
-      g.part[(size_t)I * n + C0 + lane] = cs;
-      vy += cs * vs[lane];
-    }
+    if (st) *pc = cs;
   }
-  vy = wave_sum(vy);
-  if (lane == 0) g.vy[bid] = vy;
 }
 
 // sum over the 4 lanes of a quad (lanes 4q..4q+3), the same value in each
@@ -371,7 +384,7 @@ __global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, in
   __shared__ double sh[4];
   const int ii = fin >= 0 ? fin - p0 : -1;
   const double *V = g.P, *W = g.P + (size_t)n * nb;
-  double *ac = g.a + (size_t)n * (col >= 0 ? col : 0);
+  double *ac = g.a + (size_t)g.lda * (col >= 0 ? col : 0);
   const double acr = (col >= 0) ? ac[rs] : 0.0;
   double wr = 0.0, vii = 0.0;
   double vk[KQ], wk[KQ];  // V(r,k), W(r,k) for k = KQ q4 + u
@@ -451,9 +464,7 @@ __global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, in
     }
     y = quad_sum(y) - quad_sum(z);
     wr = tauf * y + alpha2 * vii;
-    double *Wi = g.P + (size_t)n * (nb + ii);
-    if (live && q4 == 0) Wi[r] = wr;
-    for (int q = blockIdx.x * 256 + t; q < rbase; q += gridDim.x * 256) Wi[q] = 0.0;
+    if (live && q4 == 0) g.P[(size_t)n * (nb + ii) + r] = wr;
   }
   if (col < 0) return;
   const int jj = col - p0;  // columns 0..jj-1 of the panel update this column
@@ -511,7 +522,7 @@ __global__ void k_tri_store(TriWork g, int ncols)
   const int i = g.p0 + k;
   const int r = i + 2 + blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ncols || r >= n) return;
-  g.a[(size_t)r + (size_t)n * i] = g.P[(size_t)n * k + r];
+  g.a[(size_t)r + (size_t)g.lda * i] = g.P[(size_t)n * k + r];
 }
 
 // ------------------------------------------------------------------ bisection
@@ -879,7 +890,7 @@ __global__ __launch_bounds__(256) void k_normsign(int n, double *z, int ldz)
 
 // ------------------------------------------------------------------ back-transforms
 // explicit panel of reflectors p0..p0+nb-1 (v(i+1)=1, v(i+2:n) from A) into vp
-__global__ void k_vpanel(int n, int p0, int nb, const double *__restrict__ a, double *__restrict__ vp)
+__global__ void k_vpanel(int n, int lda, int p0, int nb, const double *__restrict__ a, double *__restrict__ vp)
 {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y;
@@ -887,7 +898,7 @@ __global__ void k_vpanel(int n, int p0, int nb, const double *__restrict__ a, do
   const int i = p0 + k;
   double v = 0.0;
   if (r == i + 1) v = 1.0;
-  else if (r > i + 1) v = a[(size_t)r + (size_t)n * i];
+  else if (r > i + 1) v = a[(size_t)r + (size_t)lda * i];
   vp[(size_t)r + (size_t)n * k] = v;
 }
 
@@ -1150,8 +1161,11 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
   hipStream_t s = c.stream;
   int rc;
   const size_t nn = (size_t)n * n;
-  // workspaces: w_a = S/U, w_b = H full then C, w_c = W = U^-1, w_x = X
-  if ((rc = c.w_a.alloc(nn)) || (rc = c.w_b.alloc(nn)) || (rc = c.w_c.alloc(nn))) return rc;
+  // workspaces: w_a = S/U, w_b = H full then C, w_c = W = U^-1, w_x = X.
+  // H/C has its leading dimension rounded up to 16 so that its columns start
+  // on 128-byte boundaries (the tridiagonalisation SYMV reads aligned rows)
+  const int lda = (n + 15) & ~15;
+  if ((rc = c.w_a.alloc(nn)) || (rc = c.w_b.alloc((size_t)lda * n)) || (rc = c.w_c.alloc(nn))) return rc;
   double *U = c.w_a.p, *Hf = c.w_b.p, *W = c.w_c.p;
   DBuf<double> &X = c.w_x;
   if ((rc = X.alloc(nn))) return rc;
@@ -1165,7 +1179,7 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
   // ---- 1. Cholesky
   {
     dim3 g2((n + 255) / 256, n);
-    hipLaunchKernelGGL(k_prep, g2, dim3(256), 0, s, n, c.d_hs.p, c.d_hsdiag.p, U, Hf);
+    hipLaunchKernelGGL(k_prep, g2, dim3(256), 0, s, n, lda, c.d_hs.p, c.d_hsdiag.p, U, Hf);
     W2K_STAGE(s, "k_prep");
     for (int k0 = 0; k0 < n; k0 += NB) {
       const int kb = std::min(NB, n - k0);
@@ -1212,10 +1226,10 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
           return rc;
       }
     }
-    if ((rc = dgemm(s, 'N', 'N', n, n, n, 1.0, Hf, n, W, n, 0.0, X.p, n, 0, 0, 1))) return rc;
-    if ((rc = dgemm(s, 'T', 'N', n, n, n, 1.0, W, n, X.p, n, 0.0, Hf, n, 1, 2, 0))) return rc;
+    if ((rc = dgemm(s, 'N', 'N', n, n, n, 1.0, Hf, lda, W, n, 0.0, X.p, n, 0, 0, 1))) return rc;
+    if ((rc = dgemm(s, 'T', 'N', n, n, n, 1.0, W, n, X.p, n, 0.0, Hf, lda, 1, 2, 0))) return rc;
     dim3 g2((n + 255) / 256, n);
-    hipLaunchKernelGGL(k_mirror_lower, g2, dim3(256), 0, s, n, Hf);
+    hipLaunchKernelGGL(k_mirror_lower, g2, dim3(256), 0, s, n, lda, Hf);
     W2K_STAGE(s, "k_mirror_lower");
   }
   W2K_HIP(hipEventRecord(c.ev[8], s));
@@ -1231,7 +1245,7 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
       for (size_t q = old; q < c.kev.size(); q++) W2K_HIP(hipEventCreate(&c.kev[q]));
     }
     TriWork g;
-    g.n = n; g.a = A; g.P = P; g.d = d; g.e = e; g.tau = tau; g.scal = scal;
+    g.n = n; g.lda = lda; g.a = A; g.P = P; g.d = d; g.e = e; g.tau = tau; g.scal = scal;
     g.part = B.part; g.dots = B.tdots; g.vy = B.tvy; g.norm = B.tnorm; g.counter = B.ticket;
     for (int p0 = 0; p0 < n; p0 += TNB) {
       const int nb = std::min(TNB, n - p0);
@@ -1246,7 +1260,8 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
         if (i >= n - 1) break;
         nref = k + 1;
         const int rows = n - (i + 1);
-        const int nt = (rows + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
+        const int base = (i + 1) & ~15;  // the SYMV's aligned tile origin
+        const int nt = (n - base + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
         if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv], s));
         hipLaunchKernelGGL(k_tri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
         W2K_STAGE(s, "k_tri_symv");
@@ -1264,8 +1279,8 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
       if (p1 < n) {
         // A(p1:n, p1:n) -= [V W] [W V]^T
         const int r = n - p1;
-        double *A1 = A + (size_t)p1 + (size_t)n * p1;
-        if ((rc = dgemm(s, 'N', 'T', r, r, 2 * nb, -1.0, P + p1, n, P + (size_t)n * nb + p1, n, 1.0, A1, n)))
+        double *A1 = A + (size_t)p1 + (size_t)lda * p1;
+        if ((rc = dgemm(s, 'N', 'T', r, r, 2 * nb, -1.0, P + p1, n, P + (size_t)n * nb + p1, n, 1.0, A1, lda)))
           return rc;
       }
     }
@@ -1307,7 +1322,7 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
     for (int p0 = last; p0 >= 0; p0 -= TNB) {
       const int nb = std::min(TNB, nref - p0);
       if (nb <= 0) continue;
-      hipLaunchKernelGGL(k_vpanel, dim3((n + 255) / 256, nb), dim3(256), 0, s, n, p0, nb, A, vb);
+      hipLaunchKernelGGL(k_vpanel, dim3((n + 255) / 256, nb), dim3(256), 0, s, n, lda, p0, nb, A, vb);
       W2K_STAGE(s, "k_vpanel");
       if ((rc = dgemm(s, 'T', 'N', nb, nb, n, 1.0, vb, n, vb, n, 0.0, gm, nb, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
         return rc;
