@@ -90,6 +90,29 @@ __global__ void k_adj_warp_grad(int nv, const int *__restrict__ kzz, const int *
   g[2 * b + 1] = ai;
 }
 
+int adjoint_gamma(Ctx &c, const double *d_f, int rows, double *Gr, double *Gi)
+{
+  const int n = c.n, ne = c.ne;
+  const int cpx = c.run().is_complex;
+  hipStream_t s = c.stream;
+  int rc;
+  const size_t zsz = (size_t)rows * ne;
+  if ((rc = c.w_x.alloc(4 * zsz))) return rc;
+  double *zr = c.w_x.p, *zi = zr + zsz, *fzr = zi + zsz, *fzi = fzr + zsz;
+  hipLaunchKernelGGL(k_adj_zf, dim3((rows + 255) / 256, ne), dim3(256), 0, s, rows, n, ne, cpx, c.d_z.p, d_f, zr, zi,
+                     fzr, fzi);
+  W2K_STAGE(s, "k_adj_zf");
+  // real:    Gamma = (Z F) Z^T
+  // complex: Re = Zr F Zr^T + Zi F Zi^T,  Im = Zi F Zr^T - Zr F Zi^T
+  if ((rc = dgemm(s, 'N', 'T', rows, rows, ne, 1.0, fzr, rows, zr, rows, 0.0, Gr, rows, 1))) return rc;
+  if (cpx) {
+    if ((rc = dgemm(s, 'N', 'T', rows, rows, ne, 1.0, fzi, rows, zi, rows, 1.0, Gr, rows, 1))) return rc;
+    if ((rc = dgemm(s, 'N', 'T', rows, rows, ne, 1.0, fzi, rows, zr, rows, 0.0, Gi, rows, 1))) return rc;
+    if ((rc = dgemm(s, 'N', 'T', rows, rows, ne, -1.0, fzr, rows, zi, rows, 1.0, Gi, rows, 1))) return rc;
+  }
+  return 0;
+}
+
 int adjoint_warp(Ctx &c, const double *f_host, double *e_band, double *g_warp)
 {
   RunDev &R = c.run();
@@ -112,23 +135,12 @@ int adjoint_warp(Ctx &c, const double *f_host, double *e_band, double *g_warp)
     return 0;
   }
   // Gamma on the lower triangle of the plane-wave block
-  const size_t zsz = (size_t)nv * ne, gsz = (size_t)nv * nv;
-  if ((rc = c.w_x.alloc(4 * zsz)) || (rc = c.w_a.alloc(gsz)) || (cpx && (rc = c.w_b.alloc(gsz)))) return rc;
-  double *zr = c.w_x.p, *zi = zr + zsz, *fzr = zi + zsz, *fzi = fzr + zsz;
+  const size_t gsz = (size_t)nv * nv;
+  if ((rc = c.w_a.alloc(gsz)) || (cpx && (rc = c.w_b.alloc(gsz)))) return rc;
   double *Gr = c.w_a.p, *Gi = cpx ? c.w_b.p : nullptr;
   if ((rc = c.w_v.alloc((size_t)ne))) return rc;
   W2K_HIP(hipMemcpyAsync(c.w_v.p, f_host, sizeof(double) * ne, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_adj_zf, dim3((nv + 255) / 256, ne), dim3(256), 0, s, nv, n, ne, cpx, c.d_z.p, c.w_v.p, zr, zi,
-                     fzr, fzi);
-  W2K_STAGE(s, "k_adj_zf");
-  // real:    Gamma = (Z F) Z^T
-  // complex: Re = Zr F Zr^T + Zi F Zi^T,  Im = Zi F Zr^T - Zr F Zi^T
-  if ((rc = dgemm(s, 'N', 'T', nv, nv, ne, 1.0, fzr, nv, zr, nv, 0.0, Gr, nv, 1))) return rc;
-  if (cpx) {
-    if ((rc = dgemm(s, 'N', 'T', nv, nv, ne, 1.0, fzi, nv, zi, nv, 1.0, Gr, nv, 1))) return rc;
-    if ((rc = dgemm(s, 'N', 'T', nv, nv, ne, 1.0, fzi, nv, zr, nv, 0.0, Gi, nv, 1))) return rc;
-    if ((rc = dgemm(s, 'N', 'T', nv, nv, ne, -1.0, fzr, nv, zi, nv, 1.0, Gi, nv, 1))) return rc;
-  }
+  if ((rc = adjoint_gamma(c, c.w_v.p, nv, Gr, Gi))) return rc;
   // KZZ index grid
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   for (int d = 0; d < 3; d++) {

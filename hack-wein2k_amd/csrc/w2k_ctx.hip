@@ -248,6 +248,8 @@ int run_upload(Ctx &c, const w2k_run_desc *r)
   {
     R.lindex.assign(nat, 0);
     R.lm0_of.assign((size_t)nat * W2K_NSL2, 0);
+    R.hns_terms.clear();
+    R.lmmxd = r->lmmxd;
     std::vector<double> T((size_t)nat * W2K_NSL2 * W2K_NSL2 * 18, 0.0);
     const int LOP2 = W2K_LOMAX + 2;  // leading dimension of VNS*LO
     auto VNS = [&](const double *arr, int x, int lmx, int y, int a) {
@@ -271,23 +273,29 @@ int run_upload(Ctx &c, const w2k_run_desc *r)
           while (q < (int)gl.size() && gl[q].l0 == L0 && gl[q].m0 == M0 && gl[q].lp == LP && gl[q].mp == MP) {
             const int lmx = gl[q].lmx - 1;
             const double gr = gl[q].gr, gi = gl[q].gi;
-            double v;
-            v = VNS(r->vns1, LP, lmx, L0, a); t[0] += gr * v; t[1] += gi * v;   // TUU
-            v = VNS(r->vns3, L0, lmx, LP, a); t[2] += gr * v; t[3] += gi * v;   // TDU
-            v = VNS(r->vns3, LP, lmx, L0, a); t[6] += gr * v; t[7] += gi * v;   // TUD
-            v = VNS(r->vns2, LP, lmx, L0, a); t[8] += gr * v; t[9] += gi * v;   // TDD
+            // coefficient cc (0..8) += GFAC * VNS array arr at (x, lmx, y)
+            auto add = [&](int cc, int arr, const double *base, bool lo, int x, int y) {
+              const double v = lo ? VNSLO(base, x, lmx, y, a) : VNS(base, x, lmx, y, a);
+              t[2 * cc] += gr * v;
+              t[2 * cc + 1] += gi * v;
+              R.hns_terms.push_back({a, k, p, cc, arr, x, lmx, y, gr, gi});
+            };
+            add(0, 0, r->vns1, false, LP, L0);   // TUU
+            add(1, 2, r->vns3, false, L0, LP);   // TDU
+            add(3, 2, r->vns3, false, LP, L0);   // TUD
+            add(4, 1, r->vns2, false, LP, L0);   // TDD
             int loorl0 = 0;
             if (L0 <= W2K_LOMAX) {
               loorl0 = r->loor[L0 + W2K_LOMAXP * a];
               if (loorl0) {
-                v = VNSLO(r->vns1lo, L0, lmx, LP, a); t[12] += gr * v; t[13] += gi * v;  // TUU12
-                v = VNSLO(r->vns2lo, L0, lmx, LP, a); t[14] += gr * v; t[15] += gi * v;  // TDU12
+                add(6, 3, r->vns1lo, true, L0, LP);  // TUU12
+                add(7, 4, r->vns2lo, true, L0, LP);  // TDU12
               }
             }
             if (LP <= W2K_LOMAX && r->loor[LP + W2K_LOMAXP * a]) {
-              if (loorl0) { v = VNSLO(r->vns3lo, LP, lmx, L0, a); t[16] += gr * v; t[17] += gi * v; }  // TUU22
-              v = VNSLO(r->vns1lo, LP, lmx, L0, a); t[4] += gr * v; t[5] += gi * v;    // TUU21
-              v = VNSLO(r->vns2lo, LP, lmx, L0, a); t[10] += gr * v; t[11] += gi * v;  // TUD21
+              if (loorl0) add(8, 5, r->vns3lo, true, LP, L0);  // TUU22
+              add(2, 3, r->vns1lo, true, LP, L0);                // TUU21
+              add(5, 4, r->vns2lo, true, LP, L0);                // TUD21
             }
             q++;
           }
@@ -466,6 +474,7 @@ int w2k_set_basis(w2k_ctx *x, int nv, int nlo, const int *kzz, const double *xk,
   }
   int rc;
   c.h_kzz.assign(kzz, kzz + (size_t)3 * n);
+  c.have_hns_ops = 0;
   if ((rc = upload(c.d_kzz, kzz, (size_t)3 * n, c.stream))) return rc;
   if ((rc = upload(c.d_kvec, kv.data(), kv.size(), c.stream))) return rc;
   const int w = c.run().is_complex ? 2 : 1;
@@ -630,6 +639,17 @@ int w2k_adjoint_warp(w2k_ctx *x, const double *f, double *e_band, double *g_warp
   if (!c.have_basis) return W2K_ESTATE;
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
   return adjoint_warp(c, f, e_band, g_warp);
+}
+
+int w2k_adjoint_vns(w2k_ctx *x, const double *f, double *g_vns1, double *g_vns2, double *g_vns3, double *g_vns1lo,
+                    double *g_vns2lo, double *g_vns3lo)
+{
+  if (!x || (!f && x->c.ne > 0)) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  double *const g[6] = {g_vns1, g_vns2, g_vns3, g_vns1lo, g_vns2lo, g_vns3lo};
+  return adjoint_vns(c, f, g);
 }
 
 int w2k_get_hs(w2k_ctx *x, double *hs, int ld, double *hsdiag)

@@ -66,6 +66,16 @@ struct RunDev {
   DBuf<double> d_rad;     // [a][6][LMAX]: u, up, du, dup, chn, e
   DBuf<int> d_lapw;       // [a][LMAX]
   DBuf<double> d_warp;    // complex box, interleaved
+  // HNS coupling recipe: every term GFAC * VNS that run_upload adds into a
+  // coupling coefficient T_c(a,k,p), kept for the band-energy adjoint
+  // (dE/dVNS = Re(GFAC * dE/dT) summed over the terms that read it)
+  struct HnsTerm {
+    int a, k, p, c;       // atom type, unique (L0,M0) index, LP*(LP+1)+MP, coefficient 0..8
+    int arr, x, lmx, y;   // VNS array (0..2: VNS1..3, 3..5: VNS1LO..3LO) and its indices
+    double gr, gi;        // GFAC
+  };
+  std::vector<HnsTerm> hns_terms;
+  int lmmxd = 0;
   // HNS: per inequivalent atom, the sorted unique (L0,M0) list and the
   // coupling coefficients T[k][p][9] (complex), k < lindex[a], p < 25
   std::vector<int> lindex;          // LINDEX per atom type
@@ -116,6 +126,7 @@ struct Ctx {
   DBuf<double> h_small;         // LO-block operands
   DBuf<int> h_int;              // per-atom column offsets
   DBuf<double> h_re, h_im;      // complex build: planar products
+  int have_hns_ops = 0;         // L/R operands of the last HNS are valid (same basis)
   int hns_slices = 0;           // 0: FP64 rank-K update; 3..7: INT8 digit slices (split precision)
   DBuf<signed char> w_dig;      // split precision: digit planes and row exponents
   // eigen results
@@ -145,6 +156,12 @@ int hns(Ctx &c);
 int seclr4(Ctx &c, double el, double eu, int nume, int *ne, int *nbelw);
 // band-energy adjoint with respect to the warped interstitial box (after SECLR4)
 int adjoint_warp(Ctx &c, const double *f_host, double *e_band, double *g_warp);
+// Gamma = Z diag(f) Z^H on the lower triangle of the leading `rows` rows,
+// planar (Gi only for the complex build); d_f on the device
+int adjoint_gamma(Ctx &c, const double *d_f, int rows, double *Gr, double *Gi);
+// band-energy adjoint with respect to the VNS radial integrals (after a
+// CALKPT that ran HNS); g[0..5] = dE/dVNS1..3, dE/dVNS1LO..3LO (host, desc layout)
+int adjoint_vns(Ctx &c, const double *f_host, double *const g[6]);
 // elapsed milliseconds between events e0 and e1 into last_ms[slot]
 void take_ms(Ctx &c, int slot, int e0, int e1);
 

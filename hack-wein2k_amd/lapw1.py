@@ -209,6 +209,7 @@ def load_library(path: Optional[str] = None):
     lib.w2k_set_kernel_timing.argtypes = [V, ctypes.c_int]
     lib.w2k_set_split_precision.argtypes = [V, ctypes.c_int]
     lib.w2k_adjoint_warp.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p]
+    lib.w2k_adjoint_vns.argtypes = [V, _c_dbl_p] + [_c_dbl_p] * 6
     lib.w2k_calkpt_batched.argtypes = [V, ctypes.c_int, ctypes.POINTER(_BasisDesc), ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                        ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p,
                                        ctypes.c_int, ctypes.POINTER(_c_dbl_p), _c_int_p]
@@ -324,6 +325,19 @@ class Lapw1:
         g = np.zeros((2 * w1 + 1, 2 * w2 + 1, 2 * w3 + 1), np.complex128, order="F")
         self._chk("w2k_adjoint_warp", self.lib.w2k_adjoint_warp(self.ctx, _dp(f), ctypes.byref(e), _dp(g)))
         return e.value, g
+
+    def adjoint_vns(self, f) -> dict:
+        """Gradient of the band energy sum_n f_n eps_n (weights held fixed) of
+        the last solved k-point with respect to the VNS radial integrals:
+        arrays shaped like the run's VNS1..3 and VNS1LO..3LO."""
+        f = np.ascontiguousarray(np.asarray(f, np.float64))
+        if f.shape != (self.ne,):
+            raise ValueError(f"need {self.ne} weights")
+        out = {k: np.zeros(self.run.arr[k].shape, np.float64, order="F")
+               for k in ("vns1", "vns2", "vns3", "vns1lo", "vns2lo", "vns3lo")}
+        self._chk("w2k_adjoint_vns", self.lib.w2k_adjoint_vns(
+            self.ctx, _dp(f), *[_dp(out[k]) for k in ("vns1", "vns2", "vns3", "vns1lo", "vns2lo", "vns3lo")]))
+        return out
 
     def get_hs(self) -> Tuple[np.ndarray, np.ndarray]:
         n = self.basis.n

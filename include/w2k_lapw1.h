@@ -163,8 +163,22 @@ int w2k_calkpt_batched(w2k_ctx *ctx, int nk, const w2k_basis *bases, int nohns, 
  * order).  By Hellmann-Feynman g(Q) = sum over plane-wave pairs I > J with
  * KZZ_J - KZZ_I = Q of 2 conj(Gamma(I,J)), plus sum_I Gamma(I,I) at Q = 0,
  * Gamma = sum_n f_n z_n z_n^H (the real build: real parts only).  The
- * radial V_LM part of the adjoint is not provided by this entry point. */
+ * radial V_LM part is w2k_adjoint_vns. */
 int w2k_adjoint_warp(w2k_ctx *ctx, const double *f, double *e_band, double *g_warp);
+
+/* Band-energy adjoint with respect to the non-spherical radial integrals
+ * (the second half of the adjoint; no counterpart in the reference): the
+ * gradient of E_band = sum_n f[n] EIGVAL(n) (weights held fixed) with
+ * respect to every entry of VNS1..3 and VNS1LO..3LO, written in the layouts
+ * of w2k_run_desc (vns1..3: (NSLMAX, lmmxd, NSLMAX, NAT); vns1lo..3lo:
+ * (LOMAX+2, lmmxd, NSLMAX, NAT)); any output may be NULL.  Needs the last
+ * solve of this basis to have run HNS (w2k_calkpt with nohns = 0).  The
+ * integrals enter H only through the HNS coupling coefficients
+ * T = sum GFAC VNS (hns.F:266-380), so
+ * dE/dVNS = sum over the terms that read it of Re(GFAC X), X = sum_I conj
+ * XLM(I,p) [(w Gamma) Y1](I,k), Gamma = sum_n f_n z_n z_n^H. */
+int w2k_adjoint_vns(w2k_ctx *ctx, const double *f, double *g_vns1, double *g_vns2, double *g_vns3,
+                    double *g_vns1lo, double *g_vns2lo, double *g_vns3lo);
 
 /* Host <-> device transfers of the reference-layout arrays (ld >= N).  */
 int w2k_get_hs(w2k_ctx *ctx, double *hs, int ld, double *hsdiag);

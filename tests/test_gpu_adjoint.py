@@ -1,5 +1,6 @@
 """Band-energy adjoint with respect to the warped interstitial potential
-(w2k_adjoint_warp), an extended form with no counterpart in the reference.
+(w2k_adjoint_warp) and to the non-spherical radial integrals VNS
+(w2k_adjoint_vns): an extended form with no counterpart in the reference.
 
 E_band = sum_n f_n eps_n with the weights held fixed; by Hellmann-Feynman its
 gradient with respect to a box entry WARP(Q) is a sum of the eigenvector
@@ -41,18 +42,20 @@ class _Point:
         self.basis = self.fx.basis(q)
         self.el, self.eu, self.nume = float(self.kd["EL"]), float(self.kd["EU"]), int(self.kd["NUME"])
 
-    def solve(self, warp=None):
+    def solve(self, warp=None, replace=None):
         arrays = dict(self.fx.run_arrays)
         if warp is not None:
             arrays["WARP"] = warp
+        if replace:
+            arrays.update(replace)
         ctx = L.Lapw1(L.RunData(arrays, self.fx.is_complex))
         ctx.set_basis(self.basis)
         ne, _ = ctx.calkpt(self.el, self.eu, self.nume)
         ev, z = ctx.get_eig()
         return ctx, ne, ev, z
 
-    def energy(self, warp, f):
-        ctx, ne, ev, _ = self.solve(warp)
+    def energy(self, warp, f, replace=None):
+        ctx, ne, ev, _ = self.solve(warp, replace)
         ctx.close()
         assert ne == len(f), "the window lost or gained an eigenvalue"
         return float(np.dot(f, ev))
@@ -164,3 +167,57 @@ def test_adjoint_warp_torch_gradcheck():
 
     p = torch.zeros(3, dtype=torch.float64, requires_grad=True)
     assert torch.autograd.gradcheck(BandEnergy.apply, (p,), eps=1e-4, atol=1e-7, rtol=1e-5)
+
+
+VNS_KEYS = ("VNS1", "VNS2", "VNS3", "VNS1LO", "VNS2LO", "VNS3LO")
+
+
+@pytest.mark.parametrize("name,weights", [("c1_fcc", "fermi"), ("lo_mult2", "fermi"), ("lo_cplx", "ones"),
+                                          ("lo_cplx", "fermi"), ("rs8", "ones")])
+def test_adjoint_vns_matches_finite_differences(name, weights):
+    """dE_band/dVNS from the library against central differences of E_band
+    through the whole k-point (HNS included), h = 1e-5 with Richardson
+    extrapolation (near-degenerate pairs in lo_cplx give enough curvature
+    that h = 1e-4 is off by 6e-6), on the entries with the largest gradients
+    of every array (the LO arrays included where the cell has local
+    orbitals); the floor 2e-8 is the rounding of E_band over 2h."""
+    pt = _Point(name)
+    ctx, ne, ev, _ = pt.solve()
+    f = np.ones(ne) if weights == "ones" else _fermi(ev)
+    g = ctx.adjoint_vns(f)
+    ctx.close()
+    checked = 0
+    for key in VNS_KEYS:
+        gk = g[key.lower()]
+        v0 = np.asarray(pt.fx.run_arrays[key], np.float64)
+        assert gk.shape == v0.reshape(gk.shape, order="F").shape
+        if not np.any(gk):
+            continue
+        flat = np.argsort(-np.abs(gk).ravel(order="F"))[:2]
+        for k in flat:
+            idx = np.unravel_index(k, gk.shape, order="F")
+
+            def e(t):
+                v = v0.reshape(gk.shape, order="F").copy(order="F")
+                v[idx] += t
+                return pt.energy(None, f, {key: v})
+            h = 1e-5
+            d1 = (e(h) - e(-h)) / (2 * h)
+            d2 = (e(h / 2) - e(-h / 2)) / h
+            fd = (4 * d2 - d1) / 3
+            print(f"{name} {weights} {key}{tuple(int(i) for i in idx)}: adjoint {gk[idx]:.12e} fd {fd:.12e}")
+            assert abs(fd - gk[idx]) <= 1e-6 * abs(gk[idx]) + 2e-8, (key, idx, gk[idx], fd)
+            checked += 1
+    assert checked >= 2
+    if pt.fx.run_arrays.get("NLO") is not None and int(np.asarray(pt.fx.run_arrays["NLO"])) == 0:
+        assert not any(np.any(g[k.lower()]) for k in VNS_KEYS[3:])
+
+
+def test_adjoint_vns_needs_hns():
+    pt = _Point("c1_fcc")
+    ctx = L.Lapw1(pt.fx.run())
+    ctx.set_basis(pt.basis)
+    ne, _ = ctx.calkpt(pt.el, pt.eu, pt.nume, nohns=True)
+    with pytest.raises(L.Lapw1Error):
+        ctx.adjoint_vns(np.ones(ne))
+    ctx.close()
