@@ -6,7 +6,8 @@
 //
 // Stages (all on the context stream):
 //   1. Cholesky S = U^T U (POTRF 'U', seclr4.F:196): blocked right-looking,
-//      64x64 diagonal blocks factored in LDS, panel TRSM, trailing SYRK as an
+//      64x64 diagonal blocks factored and inverted by one wave in registers,
+//      the panel TRSM as a GEMM with that inverse, the trailing SYRK as an
 //      MFMA GEMM on the upper triangle.
 //   2. Reduction to standard form C = U^-T H U^-1 (DSCGST, dscgst.f:99-142):
 //      W = U^-1 by blocked triangular inversion, then X = H W and C = W^T X as
@@ -78,95 +79,113 @@ __global__ void k_prep(int n, int ldh, const double *__restrict__ hs, const doub
   }
 }
 
-// unblocked Cholesky of the kb x kb diagonal block at (k0,k0), upper
-__global__ __launch_bounds__(256) void k_potf2(int n, int k0, int kb, double *__restrict__ a, int *__restrict__ info)
+// 1/sqrt(x), x > 0: the hardware estimate refined by two Newton steps
+// y <- y + y (1/2 - (x/2) y^2) (a few ulp)
+__device__ __forceinline__ double rsqrt_nr(double x)
 {
-  __shared__ double s[NB][NB + 1];
-  __shared__ double piv;
-  const int t = threadIdx.x;
-  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
-    const int r = idx % kb, c = idx / kb;
-    s[r][c] = (r <= c) ? a[(size_t)(k0 + r) + (size_t)n * (k0 + c)] : 0.0;
-  }
-  __syncthreads();
-  for (int j = 0; j < kb; j++) {
-    if (t == 0) {
-      double ajj = s[j][j];
-      if (!(ajj > 0.0)) {
-        if (*info == 0) *info = k0 + j + 1;
-        ajj = 1.0;  // keep the arithmetic finite; the caller reports INFO
-      }
-      piv = sqrt(ajj);
-      s[j][j] = piv;
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-h * y, y, 0.5);
+  return fma(y, e, y);
+}
+
+// v of lane `src` (a wave-uniform lane index) in every lane
+__device__ __forceinline__ double lane_bcast(double v, int src)
+{
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), src);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// X = M^-1 of a 64 x 64 upper triangular M held in LDS (M(r,q) at
+// sm[q LD + r]) by one wave, with dinv[r] = 1/M(r,r): lane c ends with
+// column c of X in x[0..63] (x[r] = 0 for r > c).  Rows from the bottom:
+// X(r,c) = -dinv[r] sum_{r<q<=c} M(r,q) X(q,c), each row of M read in one
+// batch of broadcast LDS loads; the newest term (q = r+1) is added last.
+template <int LD>
+__device__ __forceinline__ void wave_trtri64(const double *sm, const double *dinv, int lane, double *x)
+{
+#pragma unroll
+  for (int r = 0; r < NB; r++) x[r] = 0.0;
+#pragma unroll
+  for (int r = NB - 1; r >= 0; r--) {
+    double ur[NB];
+#pragma unroll
+    for (int q = r + 1; q < NB; q++) ur[q] = sm[q * LD + r];
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int q = NB - 1; q > r + 1; q--) {
+      if (q & 1) s1 = fma(ur[q], x[q], s1);
+      else s0 = fma(ur[q], x[q], s0);
     }
-    __syncthreads();
-    for (int c = j + 1 + t; c < kb; c += blockDim.x) s[j][c] /= piv;
-    __syncthreads();
-    // trailing update of the block (upper): s[r][c] -= s[j][r] s[j][c], j < r <= c
-    const int m = kb - j - 1;
-    for (int idx = t; idx < m * m; idx += blockDim.x) {
-      const int r = j + 1 + idx % m, c = j + 1 + idx / m;
-      if (r <= c) s[r][c] -= s[j][r] * s[j][c];
-    }
-    __syncthreads();
-  }
-  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
-    const int r = idx % kb, c = idx / kb;
-    if (r <= c) a[(size_t)(k0 + r) + (size_t)n * (k0 + c)] = s[r][c];
+    double sum = s0 + s1;
+    if (r + 1 < NB) sum = fma(ur[r + 1], x[r + 1], sum);
+    const double d = dinv[r];
+    x[r] = (lane == r) ? d : -sum * d;
   }
 }
 
-// row panel U(k0:k0+kb, c) = U_kk^-T S(k0:k0+kb, c) for columns c >= k0+kb
-__global__ __launch_bounds__(256) void k_trsm_row(int n, int k0, int kb, double *__restrict__ a)
+// Cholesky S_kk = U^T U of the kb x kb diagonal block at (k0,k0) (upper) and
+// the inverse of its factor, by one wave: U over the block of a, U^-1 into
+// the same block of w (zeros below the diagonal).  Lane c holds column c in
+// registers; step j takes the pivot from lane j, scales row j by the inverse
+// square root of the pivot and broadcasts the row through LDS for the rank-1
+// update (rows below the diagonal of a column are carried along unused).  A
+// block smaller than 64 is padded with the identity.  A non-positive pivot
+// sets *info = k0 + j + 1 for the first such j (DPOTRF's INFO) and is taken
+// as 1 so that the arithmetic stays finite.  Loads and stores go through LDS
+// so that they are coalesced along the columns.
+__global__ __launch_bounds__(64) void k_chol_diag(int n, int k0, int kb, double *__restrict__ a,
+                                                  double *__restrict__ w, int *__restrict__ info)
 {
-  __shared__ double u[NB][NB + 1];
-  const int t = threadIdx.x;
-  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
-    const int r = idx % kb, c = idx / kb;
-    u[r][c] = (r <= c) ? a[(size_t)(k0 + r) + (size_t)n * (k0 + c)] : 0.0;
+  constexpr int LD = NB + 1;
+  __shared__ double sm[NB * LD];  // sm[c LD + r] = block element (r, c)
+  __shared__ double urow[2][NB];
+  __shared__ double dinv[NB];
+  const int lane = threadIdx.x;
+  for (int c = 0; c < NB; c++) {
+    const bool in = lane <= c && c < kb;
+    sm[c * LD + lane] = in ? a[(size_t)(k0 + lane) + (size_t)n * (k0 + c)] : (lane == c ? 1.0 : 0.0);
   }
   __syncthreads();
-  const int c = k0 + kb + blockIdx.x * blockDim.x + t;
-  if (c >= n) return;
+  double v[NB];
+#pragma unroll
+  for (int r = 0; r < NB; r++) v[r] = sm[lane * LD + r];
+  int first = 0;
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    const double ajj = lane_bcast(v[j], j);
+    const bool ok = ajj > 0.0;
+    first = (!ok && first == 0) ? j + 1 : first;
+    v[j] *= rsqrt_nr(ok ? ajj : 1.0);
+    urow[j & 1][lane] = v[j];
+    __syncthreads();
+    double u[NB];
+#pragma unroll
+    for (int r = j + 1; r < NB; r++) u[r] = urow[j & 1][r];
+#pragma unroll
+    for (int r = j + 1; r < NB; r++) v[r] = fma(-u[r], v[j], v[r]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NB; r++) sm[lane * LD + r] = (r <= lane) ? v[r] : 0.0;
+  __syncthreads();
+  for (int c = 0; c < kb; c++)
+    if (lane <= c) a[(size_t)(k0 + lane) + (size_t)n * (k0 + c)] = sm[c * LD + lane];
+  dinv[lane] = 1.0 / sm[lane * LD + lane];
+  __syncthreads();
   double x[NB];
-  double *col = a + (size_t)n * c + k0;
-  for (int r = 0; r < kb; r++) x[r] = col[r];
-  // solve U^T y = x (U^T lower): forward substitution
-  for (int r = 0; r < kb; r++) {
-    double s = x[r];
-    for (int q = 0; q < r; q++) s -= u[q][r] * x[q];
-    x[r] = s / u[r][r];
-  }
-  for (int r = 0; r < kb; r++) col[r] = x[r];
-}
-
-// inverse of the kb x kb upper block of U at (j0,j0) into W(j0,j0)
-__global__ __launch_bounds__(256) void k_trtri_diag(int n, int j0, int kb, const double *__restrict__ u,
-                                                    double *__restrict__ w)
-{
-  __shared__ double s[NB][NB + 1];
-  __shared__ double r_[NB][NB + 1];
-  const int t = threadIdx.x;
-  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
-    const int r = idx % kb, c = idx / kb;
-    s[r][c] = (r <= c) ? u[(size_t)(j0 + r) + (size_t)n * (j0 + c)] : 0.0;
-    r_[r][c] = 0.0;
-  }
+  wave_trtri64<LD>(sm, dinv, lane, x);
   __syncthreads();
-  // column c of the inverse: solve U x = e_c by back substitution (thread per column)
-  if (t < kb) {
-    const int c = t;
-    for (int r = c; r >= 0; r--) {
-      double s2 = (r == c) ? 1.0 : 0.0;
-      for (int q = r + 1; q <= c; q++) s2 -= s[r][q] * r_[q][c];
-      r_[r][c] = s2 / s[r][r];
-    }
-  }
+#pragma unroll
+  for (int r = 0; r < NB; r++) sm[lane * LD + r] = (r <= lane) ? x[r] : 0.0;
   __syncthreads();
-  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
-    const int r = idx % kb, c = idx / kb;
-    w[(size_t)(j0 + r) + (size_t)n * (j0 + c)] = (r <= c) ? r_[r][c] : 0.0;
-  }
+  if (lane < kb)
+    for (int c = 0; c < kb; c++) w[(size_t)(k0 + lane) + (size_t)n * (k0 + c)] = sm[c * LD + lane];
+  if (lane == 0 && first != 0 && *info == 0) *info = k0 + first;
 }
 
 __global__ void k_fill(size_t cnt, double *p, double v)
@@ -902,33 +921,75 @@ __global__ void k_vpanel(int n, int lda, int p0, int nb, const double *__restric
   vp[(size_t)r + (size_t)n * k] = v;
 }
 
-// T (nb x nb upper) of the compact WY form I - V T V^T from G = V^T V
-// (DLARFT 'F','C'): T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i), column by column
-__global__ __launch_bounds__(64) void k_larft(int nb, const double *__restrict__ g, const double *__restrict__ tau,
-                                              double *__restrict__ tm)
+// T (nb x nb upper, nb <= BTNB) of the compact WY form I - V T V^T of
+// H_0 H_1 ... H_(nb-1) from G = V^T V (DLARFT 'F','C').  DLARFT's recurrence
+// T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i), T(i,i) = tau_i makes T the inverse of
+// the upper triangular M = diag(1/tau) + (strict upper part of G), which is
+// formed here by blocks: X11 = M11^-1 and X22 = M22^-1 (64 x 64, one wave
+// each), then T12 = -X11 (M12 X22) (four waves).  A reflector with tau = 0
+// (H_i = I) has a zero row and column in DLARFT's T; here its row and column
+// of M are those of the identity and T(i,i) is set to 0 afterwards.
+constexpr int BTNB = 128;
+__global__ __launch_bounds__(256) void k_larft_inv(int nb, const double *__restrict__ g,
+                                                   const double *__restrict__ tau, double *__restrict__ tm)
 {
-  __shared__ double gs[TNB][TNB + 1], ts[TNB][TNB + 1];
-  const int t = threadIdx.x;
-  for (int idx = t; idx < nb * nb; idx += blockDim.x) {
-    const int r = idx % nb, c = idx / nb;
-    gs[r][c] = g[r + nb * c];
-    ts[r][c] = 0.0;
+  constexpr int LDM = BTNB + 1;
+  __shared__ double ms[BTNB * LDM];  // ms[c LDM + r] = M(r, c), then T(r, c)
+  __shared__ double dinv[BTNB];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  for (int idx = t; idx < BTNB * BTNB; idx += blockDim.x) {
+    const int r = idx & (BTNB - 1), c = idx / BTNB;
+    double v = 0.0;
+    if (r == c) {
+      v = (r < nb && tau[r] != 0.0) ? 1.0 / tau[r] : 1.0;
+    } else if (r < c && c < nb) {
+      v = (tau[r] != 0.0 && tau[c] != 0.0) ? g[r + (size_t)nb * c] : 0.0;
+    }
+    ms[c * LDM + r] = v;
+  }
+  if (t < BTNB) dinv[t] = (t < nb && tau[t] != 0.0) ? tau[t] : 1.0;
+  __syncthreads();
+  double x[NB];
+  if (wid < 2 && (wid == 0 || nb > NB)) wave_trtri64<LDM>(ms + wid * NB * (LDM + 1), dinv + wid * NB, lane, x);
+  __syncthreads();
+  if (wid < 2 && (wid == 0 || nb > NB)) {
+    double *blk = ms + wid * NB * (LDM + 1);
+#pragma unroll
+    for (int r = 0; r < NB; r++) blk[lane * LDM + r] = (r <= lane) ? x[r] : 0.0;
   }
   __syncthreads();
-  for (int i = 0; i < nb; i++) {
-    const double ti = tau[i];
-    if (t < i) {
-      double sum = 0.0;
-      for (int q = t; q < i; q++) sum += ts[t][q] * gs[q][i];
-      ts[t][i] = -ti * sum;
-    } else if (t == i) {
-      ts[i][i] = ti;
+  if (nb > NB) {
+    // thread (c = lane, rows 16 wid .. 16 wid + 15): P = M12 X22, then T12 = -X11 P
+    const int c = lane, r0 = 16 * wid;
+    double acc[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) acc[u] = 0.0;
+    for (int q = 0; q < NB; q++) {
+      const double xq = ms[(NB + c) * LDM + NB + q];  // X22(q, c)
+#pragma unroll
+      for (int u = 0; u < 16; u++) acc[u] = fma(ms[(NB + q) * LDM + r0 + u], xq, acc[u]);  // M12(r, q)
     }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; u++) ms[(NB + c) * LDM + r0 + u] = acc[u];  // P(r, c) over M12
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; u++) acc[u] = 0.0;
+    for (int q = 0; q < NB; q++) {
+      const double pq = ms[(NB + c) * LDM + q];  // P(q, c)
+#pragma unroll
+      for (int u = 0; u < 16; u++) acc[u] = fma(ms[q * LDM + r0 + u], pq, acc[u]);  // X11(r, q)
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; u++) ms[(NB + c) * LDM + r0 + u] = -acc[u];
     __syncthreads();
   }
   for (int idx = t; idx < nb * nb; idx += blockDim.x) {
-    const int r = idx % nb, c = idx / nb;
-    tm[r + nb * c] = ts[r][c];
+    const int q = idx % nb, c = idx / nb;
+    double v = (q <= c) ? ms[c * LDM + q] : 0.0;
+    if (q == c && tau[q] == 0.0) v = 0.0;
+    tm[idx] = v;
   }
 }
 
@@ -1176,21 +1237,21 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
   int *info = B.info;
 
   W2K_HIP(hipEventRecord(c.ev[6], s));
-  // ---- 1. Cholesky
+  // ---- 1. Cholesky, with U_kk^-1 of every diagonal block into W (zero
+  // elsewhere; the reduction fills in the blocks above the diagonal)
   {
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nn, W, 0.0);
+    W2K_STAGE(s, "k_fill");
     dim3 g2((n + 255) / 256, n);
     hipLaunchKernelGGL(k_prep, g2, dim3(256), 0, s, n, lda, c.d_hs.p, c.d_hsdiag.p, U, Hf);
     W2K_STAGE(s, "k_prep");
     for (int k0 = 0; k0 < n; k0 += NB) {
       const int kb = std::min(NB, n - k0);
-      hipLaunchKernelGGL(k_potf2, dim3(1), dim3(256), 0, s, n, k0, kb, U, info);
-      W2K_STAGE(s, "k_potf2");
+      hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(64), 0, s, n, k0, kb, U, W, info);
+      W2K_STAGE(s, "k_chol_diag");
       const int rest = n - k0 - kb;
       if (rest > 0) {
-        // U(k0:k0+kb, k0+kb:n) = U_kk^-T S(k0:k0+kb, k0+kb:n) with U_kk^-1 from
-        // the diagonal-block inversion (into W, free until the reduction)
-        hipLaunchKernelGGL(k_trtri_diag, dim3(1), dim3(256), 0, s, n, k0, kb, U, W);
-        W2K_STAGE(s, "k_trtri_diag");
+        // U(k0:k0+kb, k0+kb:n) = U_kk^-T S(k0:k0+kb, k0+kb:n)
         double *S12 = U + (size_t)k0 + (size_t)n * (k0 + kb);
         if ((rc = dgemm(s, 'T', 'N', kb, rest, kb, 1.0, W + (size_t)k0 + (size_t)n * k0, n, S12, n, 0.0, X.p, kb, 0,
                         2, 0)))
@@ -1209,22 +1270,16 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
   if (hinfo[0] != 0) return hinfo[0];  // POTRF INFO, seclr4.F:200-205
   // ---- 2. reduction: W = U^-1, X = H W, C = W^T X (lower), mirrored
   {
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nn, W, 0.0);
-    W2K_STAGE(s, "k_fill");
-    for (int j0 = 0; j0 < n; j0 += NB) {
+    for (int j0 = NB; j0 < n; j0 += NB) {
       const int kb = std::min(NB, n - j0);
-      hipLaunchKernelGGL(k_trtri_diag, dim3(1), dim3(256), 0, s, n, j0, kb, U, W);
-      W2K_STAGE(s, "k_trtri_diag");
-      if (j0 > 0) {
-        // T = W(0:j0,0:j0) U(0:j0, j0:j0+kb); W(0:j0, j0:) = -T W_jj
-        double *T = X.p;
-        if ((rc = dgemm(s, 'N', 'N', j0, kb, j0, 1.0, W, n, U + (size_t)n * j0, n, 0.0, T, n, 0, 1, 0, c.w_gk.p,
-                        c.w_gk.n)))
-          return rc;
-        if ((rc = dgemm(s, 'N', 'N', j0, kb, kb, -1.0, T, n, W + (size_t)j0 + (size_t)n * j0, n, 0.0,
-                        W + (size_t)n * j0, n, 0, 0, 1)))
-          return rc;
-      }
+      // T = W(0:j0,0:j0) U(0:j0, j0:j0+kb); W(0:j0, j0:) = -T W_jj
+      double *T = X.p;
+      if ((rc = dgemm(s, 'N', 'N', j0, kb, j0, 1.0, W, n, U + (size_t)n * j0, n, 0.0, T, n, 0, 1, 0, c.w_gk.p,
+                      c.w_gk.n)))
+        return rc;
+      if ((rc = dgemm(s, 'N', 'N', j0, kb, kb, -1.0, T, n, W + (size_t)j0 + (size_t)n * j0, n, 0.0,
+                      W + (size_t)n * j0, n, 0, 0, 1)))
+        return rc;
     }
     if ((rc = dgemm(s, 'N', 'N', n, n, n, 1.0, Hf, lda, W, n, 0.0, X.p, n, 0, 0, 1))) return rc;
     if ((rc = dgemm(s, 'T', 'N', n, n, n, 1.0, W, n, X.p, n, 0.0, Hf, lda, 1, 2, 0))) return rc;
@@ -1313,21 +1368,22 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
   W2K_HIP(hipEventRecord(c.ev[10], s));
   if (m > 0) {
     // ---- 6. Z = Q Zt (reflectors of step 3, last panel first), sort, Z = W Z
-    if ((rc = c.w_bt.alloc((size_t)(n + 2 * m + 2 * TNB) * TNB))) return rc;
-    // w_bt: V panel (n x TNB), V^T V and T (TNB x TNB each), Y1 and Y2 (TNB x m each)
-    double *vb = c.w_bt.p, *gm = vb + (size_t)n * TNB, *tm = gm + TNB * TNB, *y1 = tm + TNB * TNB;
-    double *y2 = y1 + (size_t)TNB * m;
+    if ((rc = c.w_bt.alloc((size_t)(n + 2 * m + 2 * BTNB) * BTNB))) return rc;
+    // w_bt: V block (n x BTNB), V^T V and T (BTNB x BTNB each), Y1 and Y2
+    // (BTNB x m each); blocks of BTNB reflectors, last block first
+    double *vb = c.w_bt.p, *gm = vb + (size_t)n * BTNB, *tm = gm + BTNB * BTNB, *y1 = tm + BTNB * BTNB;
+    double *y2 = y1 + (size_t)BTNB * m;
     const int nref = n - 1;  // reflectors 0..n-2
-    const int last = nref > 0 ? ((nref - 1) / TNB) * TNB : -1;
-    for (int p0 = last; p0 >= 0; p0 -= TNB) {
-      const int nb = std::min(TNB, nref - p0);
+    const int last = nref > 0 ? ((nref - 1) / BTNB) * BTNB : -1;
+    for (int p0 = last; p0 >= 0; p0 -= BTNB) {
+      const int nb = std::min(BTNB, nref - p0);
       if (nb <= 0) continue;
       hipLaunchKernelGGL(k_vpanel, dim3((n + 255) / 256, nb), dim3(256), 0, s, n, lda, p0, nb, A, vb);
       W2K_STAGE(s, "k_vpanel");
       if ((rc = dgemm(s, 'T', 'N', nb, nb, n, 1.0, vb, n, vb, n, 0.0, gm, nb, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
         return rc;
-      hipLaunchKernelGGL(k_larft, dim3(1), dim3(64), 0, s, nb, gm, tau + p0, tm);
-      W2K_STAGE(s, "k_larft");
+      hipLaunchKernelGGL(k_larft_inv, dim3(1), dim3(256), 0, s, nb, gm, tau + p0, tm);
+      W2K_STAGE(s, "k_larft_inv");
       if ((rc = dgemm(s, 'T', 'N', nb, m, n, 1.0, vb, n, Zt, n, 0.0, y1, nb, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
         return rc;
       if ((rc = dgemm(s, 'N', 'N', nb, m, nb, 1.0, tm, nb, y1, nb, 0.0, y2, nb, 0, 1, 0))) return rc;
