@@ -226,24 +226,38 @@ struct TriWork {
   double *part;       // nchunk x n SYMV partial sums
   double *dots;       // tiles x 2 TNB partial V^T v, W^T v
   double *vy;         // tiles x chunks partial v^T y
-  double *norm;       // blocks: partial squared norms
+  double *norm;       // blocks of the column update: partial squared norms
   int *counter;       // unused by the real kernels
 };
 
 
+constexpr int COL_ROWS = 32;  // rows per block of the column update (8 threads per row)
+
 // DLARFG of column i on (A(i+1,i), A(i+2:n,i)), from the squared norm
 // partials of the column-update kernel; every wave of the SYMV evaluates it
 // with identical arithmetic (fixed-order sums), so no grid-wide step is
-// needed between the two kernels.  Returns scal = 1/(alpha - beta).
-__device__ __forceinline__ double tri_reflector(const TriWork &g, int i, int lane, bool store)
+// needed between the two kernels.  tri_norm_load issues the loads (first in
+// the wave, so that the reflector is formed while the tile is still on its
+// way); tri_reflector returns scal = 1/(alpha - beta).
+constexpr int NORM_B = 3;  // partials per lane loaded up front (n <= 64 NORM_B COL_ROWS)
+__device__ __forceinline__ void tri_norm_load(const TriWork &g, int i, int lane, double *x, double &alpha)
 {
-  const int n = g.n;
-  const int nnorm = (n - i + 63) / 64;  // blocks of the update of column i
+  const int nnorm = (g.n - i + COL_ROWS - 1) / COL_ROWS;  // blocks of the update of column i
+#pragma unroll
+  for (int u = 0; u < NORM_B; u++) x[u] = g.norm[min(lane + 64 * u, nnorm - 1)];
+  alpha = g.a[(size_t)g.lda * i + i + 1];
+}
+
+__device__ __forceinline__ double tri_reflector(const TriWork &g, int i, int lane, bool store, const double *x,
+                                                double alpha)
+{
+  const int nnorm = (g.n - i + COL_ROWS - 1) / COL_ROWS;
   double xs = 0.0;
-  for (int b = lane; b < nnorm; b += 64) xs += g.norm[b];
+#pragma unroll
+  for (int u = 0; u < NORM_B; u++) xs += (lane + 64 * u < nnorm) ? x[u] : 0.0;
+  for (int b = lane + 64 * NORM_B; b < nnorm; b += 64) xs += g.norm[b];
   xs = wave_sum(xs);
   xs = __shfl(xs, 0, 64);
-  const double alpha = g.a[(size_t)g.lda * i + i + 1];
   const double xnorm = sqrt(xs);
   double tau = 0.0, beta = alpha, scal = 0.0;
   if (xnorm != 0.0) {
@@ -277,6 +291,8 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
   const int bid = blockIdx.x;
   const double *ai = g.a + (size_t)lda * i;
   double t[SYMV_T];
+  double xn[NORM_B], alpha;
+  tri_norm_load(g, i, lane, xn, alpha);
   if (bid >= nblk) {
     // row block I: V column and panel dot products
     const int I = bid - nblk;
@@ -291,7 +307,7 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
         t[TNB + k] = g.P[(size_t)n * (nb + k) + Rs];
       }
     }
-    const double sc = tri_reflector(g, i, lane, false);
+    const double sc = tri_reflector(g, i, lane, false, xn, alpha);
     const double vr = live ? (R == r0 ? 1.0 : a_r * sc) : 0.0;
     if (live) {
       g.P[(size_t)n * ii + R] = vr;
@@ -329,7 +345,7 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
   const double *ap = g.a + (size_t)Rl + (size_t)lda * C0;
 #pragma unroll
   for (int j = 0; j < SYMV_T; j++) t[j] = ap[(size_t)lda * min(j, m - 1)];
-  const double sc = tri_reflector(g, i, lane, bid == 0);
+  const double sc = tri_reflector(g, i, lane, bid == 0, xn, alpha);
   const int c = C0 + lane;
   const double vc = (c >= r0 && c < n) ? (c == r0 ? 1.0 : a_c * sc) : 0.0;  // v of the lane's column
   const double vr = live ? (R == r0 ? 1.0 : a_r * sc) : 0.0;
@@ -371,151 +387,179 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
   }
 }
 
-// sum over the 4 lanes of a quad (lanes 4q..4q+3), the same value in each
-__device__ __forceinline__ double quad_sum(double v)
+// sum over the 8 lanes 8q..8q+7, the same value in each
+__device__ __forceinline__ double oct_sum(double v)
 {
   v += bfly_partner<1>(v);
   v += bfly_partner<2>(v);
+  v += bfly_partner<4>(v);
   return v;
 }
 
 // Finish W(:, fin - p0) (fin >= 0) and update column col (col >= 0) with the
 // panel so far; write the squared-norm partial of its sub-subdiagonal part
-// per block (the reflector itself is formed by the next SYMV).  Four threads
-// per row (a quad), each with a quarter of the row's SYMV partials and of its
-// panel columns.  Every sum over a runtime count is read in unconditional
-// batches (clamped index, masked value) so that its loads are in flight
-// together.
+// per block (the reflector itself is formed by the next SYMV).  Eight
+// threads per row, COL_ROWS rows per block; each thread takes an eighth of
+// its row's SYMV partials and of the panel columns.  Every load that does
+// not depend on a value formed here is issued before any of them is used
+// (fewer than 64 per thread: one round of memory latency): the first batch
+// of every sum is unconditional, with clamped indices and masked values, and
+// the tile partials are read in pairs.  Three barriers per block.
 __global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, int ntile, int nvy)
 {
-  constexpr int BT = 32;          // loads per batch
-  constexpr int KQ = TNB / 4;     // panel columns per quad lane
+  constexpr int BD = 16;       // first batch of V^T v, W^T v pairs (ntile <= 8 BD)
+  constexpr int BV = 12;       // first batch of v^T y pairs (nvy <= 512 BV)
+  constexpr int BP = 12;       // first batch of SYMV partials of a row (ntile <= 8 BP)
+  constexpr int KO = TNB / 8;  // panel columns per thread
   const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  const int q4 = t & 3;
+  const int q8 = t & 7;
   const int nch = ntile;
   const int rbase = fin >= 0 ? fin + 1 : col;
-  const int r = rbase + blockIdx.x * 64 + (t >> 2);
+  const int r = rbase + blockIdx.x * COL_ROWS + (t >> 3);
   const bool live = r < n;
   const int rs = live ? r : rbase;  // rbase < n: a safe row for dead lanes
-  __shared__ double ab[2 * TNB];  // a_k = V_k^T v, b_k = W_k^T v
+  __shared__ double ab[2 * TNB];    // a_k = V_k^T v, b_k = W_k^T v
   __shared__ double wc[TNB], vc[TNB];
-  __shared__ double red[4][2 * TNB];
-  __shared__ double sh[4];
+  __shared__ double redd[8][2 * TNB];
+  __shared__ double sh[2][4];
   const int ii = fin >= 0 ? fin - p0 : -1;
+  const int jj = col - p0;  // columns 0..jj-1 of the panel update column col
   const double *V = g.P, *W = g.P + (size_t)n * nb;
   double *ac = g.a + (size_t)g.lda * (col >= 0 ? col : 0);
+  // ---- loads
   const double acr = (col >= 0) ? ac[rs] : 0.0;
-  double wr = 0.0, vii = 0.0;
-  double vk[KQ], wk[KQ];  // V(r,k), W(r,k) for k = KQ q4 + u
-  double tauf = 0.0, alpha2 = 0.0;
+  double vcol = 0.0, wcol = 0.0, pcol = 0.0;  // row col of the panel, SYMV partial slot t of row col
+  if (col >= 0 && t < jj) {
+    vcol = V[(size_t)n * t + col];
+    if (t < ii || fin < 0) wcol = W[(size_t)n * t + col];
+  }
+  if (fin >= 0 && col >= 0 && t < nch) pcol = g.part[(size_t)t * n + col];
+  double wr = 0.0, vii = 0.0, tauf = 0.0, alpha2 = 0.0;
+  double vk[KO], wk[KO];  // V(r,k), W(r,k) for k = KO q8 + u
+  double y = 0.0, vs = 0.0;
+  const int dq = t & 31, dp = t >> 5;  // V^T v, W^T v: columns 2dq, 2dq+1 of tiles dp + 8u
   if (fin >= 0) {
     tauf = g.tau[fin];
 #pragma unroll
-    for (int u = 0; u < KQ; u++) {
-      vk[u] = V[(size_t)n * (KQ * q4 + u) + rs];
-      wk[u] = W[(size_t)n * (KQ * q4 + u) + rs];
+    for (int u = 0; u < KO; u++) {
+      vk[u] = V[(size_t)n * (KO * q8 + u) + rs];
+      wk[u] = W[(size_t)n * (KO * q8 + u) + rs];
     }
     vii = V[(size_t)n * ii + rs];
-    // a_k, b_k: lane q of wave wid sums the tile partials b = wid (mod 4)
+    double2 xd[BD], xv[BV];
+    double xp[BP];
+    const int vlast = (nvy - 1) & ~1;  // the last pair (vy has one spare entry)
+#pragma unroll
+    for (int u = 0; u < BD; u++)
+      xd[u] = *reinterpret_cast<const double2 *>(g.dots + (size_t)min(dp + 8 * u, ntile - 1) * 2 * TNB + 2 * dq);
+#pragma unroll
+    for (int u = 0; u < BV; u++) xv[u] = *reinterpret_cast<const double2 *>(g.vy + min(2 * t + 512 * u, vlast));
+#pragma unroll
+    for (int u = 0; u < BP; u++) xp[u] = g.part[(size_t)min(q8 + 8 * u, nch - 1) * n + rs];
+    // V^T v, W^T v: partials of tiles dp (mod 8)
+    {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+      for (int u = 0; u < BD; u += 2) {
+        const bool k0 = dp + 8 * u < ntile, k1 = dp + 8 * u + 8 < ntile;
+        s0 += k0 ? xd[u].x : 0.0;
+        s1 += k0 ? xd[u].y : 0.0;
+        s2 += k1 ? xd[u + 1].x : 0.0;
+        s3 += k1 ? xd[u + 1].y : 0.0;
+      }
+      for (int b = dp + 8 * BD; b < ntile; b += 8) {
+        const double2 x = *reinterpret_cast<const double2 *>(g.dots + (size_t)b * 2 * TNB + 2 * dq);
+        s0 += x.x;
+        s1 += x.y;
+      }
+      redd[dp][2 * dq] = s0 + s2;
+      redd[dp][2 * dq + 1] = s1 + s3;
+    }
+    // v^T y: the tile partials, in pairs
     {
       double s0 = 0.0, s1 = 0.0;
-      for (int b0 = wid; b0 < ntile; b0 += 4 * BT) {
-        double x[BT];
 #pragma unroll
-        for (int u = 0; u < BT; u++) x[u] = g.dots[(size_t)min(b0 + 4 * u, ntile - 1) * 2 * TNB + lane];
-#pragma unroll
-        for (int u = 0; u < BT; u += 2) {
-          s0 += (b0 + 4 * u < ntile) ? x[u] : 0.0;
-          s1 += (b0 + 4 * u + 4 < ntile) ? x[u + 1] : 0.0;
-        }
+      for (int u = 0; u < BV; u++) {
+        const int j = 2 * t + 512 * u;
+        s0 += (j < nvy) ? xv[u].x : 0.0;
+        s1 += (j + 1 < nvy) ? xv[u].y : 0.0;
       }
-      red[wid][lane] = s0 + s1;
-    }
-    // v^T y: the tile partials over the block, fixed order
-    double vsum;
-    {
-      double s0 = 0.0, s1 = 0.0;
-      for (int b0 = t; b0 < nvy; b0 += 256 * BT) {
-        double x[BT];
-#pragma unroll
-        for (int u = 0; u < BT; u++) x[u] = g.vy[min(b0 + 256 * u, nvy - 1)];
-#pragma unroll
-        for (int u = 0; u < BT; u += 2) {
-          s0 += (b0 + 256 * u < nvy) ? x[u] : 0.0;
-          s1 += (b0 + 256 * u + 256 < nvy) ? x[u + 1] : 0.0;
-        }
+      for (int j = 2 * t + 512 * BV; j < nvy; j += 512) {
+        s0 += g.vy[j];
+        s1 += (j + 1 < nvy) ? g.vy[j + 1] : 0.0;
       }
-      vsum = wave_sum(s0 + s1);
+      vs = s0 + s1;
     }
-    // y(r): this lane's quarter of the row's SYMV partials (slots q4 + 4j)
-    double y;
+    // y(r): this thread's eighth of the row's SYMV partials (slots q8 + 8j)
     {
       double y0 = 0.0, y1 = 0.0;
-      for (int c0 = q4; c0 < nch; c0 += 4 * BT) {
-        double x[BT];
 #pragma unroll
-        for (int u = 0; u < BT; u++) x[u] = g.part[(size_t)min(c0 + 4 * u, nch - 1) * n + rs];
-#pragma unroll
-        for (int u = 0; u < BT; u += 2) {
-          y0 += (c0 + 4 * u < nch) ? x[u] : 0.0;
-          y1 += (c0 + 4 * u + 4 < nch) ? x[u + 1] : 0.0;
-        }
+      for (int u = 0; u < BP; u += 2) {
+        y0 += (q8 + 8 * u < nch) ? xp[u] : 0.0;
+        y1 += (q8 + 8 * u + 8 < nch) ? xp[u + 1] : 0.0;
       }
+      for (int c0 = q8 + 8 * BP; c0 < nch; c0 += 8) y0 += g.part[(size_t)c0 * n + rs];
       y = y0 + y1;
     }
-    __syncthreads();
+    for (int c = t + blockDim.x; c < nch && col >= 0; c += blockDim.x) pcol += g.part[(size_t)c * n + col];
+    vs = wave_sum(vs);
+    const double ps = wave_sum(pcol);
+    if (lane == 0) {
+      sh[0][wid] = vs;
+      sh[1][wid] = ps;
+    }
+  }
+  if (col >= 0 && t < jj) {
+    vc[t] = vcol;
+    if (t < ii || fin < 0) wc[t] = wcol;
+  }
+  __syncthreads();
+  if (fin >= 0) {
     if (t < 2 * TNB) {
-      const double q = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+      const double q = ((redd[0][t] + redd[1][t]) + (redd[2][t] + redd[3][t])) +
+                       ((redd[4][t] + redd[5][t]) + (redd[6][t] + redd[7][t]));
       ab[t] = ((t & (TNB - 1)) < ii) ? q : 0.0;
     }
-    if (lane == 0) sh[wid] = vsum;
-    __syncthreads();
-    const double vty = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  }
+  __syncthreads();
+  double wcii = 0.0;  // W(col, ii)
+  if (fin >= 0) {
+    const double vty = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+    const double scol = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
     double ssum = 0.0;
 #pragma unroll
     for (int k = 0; k < TNB; k++) ssum += ab[k] * ab[TNB + k];
     alpha2 = -0.5 * tauf * tauf * (vty - 2.0 * ssum);
     double z = 0.0;
 #pragma unroll
-    for (int u = 0; u < KQ; u++) {
-      const int k = KQ * q4 + u;
+    for (int u = 0; u < KO; u++) {
+      const int k = KO * q8 + u;
       z += (k < ii ? wk[u] : 0.0) * ab[k] + (k < ii ? vk[u] : 0.0) * ab[TNB + k];
     }
-    y = quad_sum(y) - quad_sum(z);
+    y = oct_sum(y) - oct_sum(z);
     wr = tauf * y + alpha2 * vii;
-    if (live && q4 == 0) g.P[(size_t)n * (nb + ii) + r] = wr;
+    if (live && q8 == 0) g.P[(size_t)n * (nb + ii) + r] = wr;
+    if (col >= 0) {
+      // row col = fin + 1 of the same formula (v(col) = 1), in every thread
+      double corr = 0.0;
+      for (int k = 0; k < ii; k++) corr += wc[k] * ab[k] + vc[k] * ab[TNB + k];
+      wcii = tauf * (scol - corr) + alpha2;
+    }
   }
   if (col < 0) return;
-  const int jj = col - p0;  // columns 0..jj-1 of the panel update this column
-  if (t < jj) {
-    vc[t] = V[(size_t)n * t + col];
-    if (t < ii || fin < 0) wc[t] = W[(size_t)n * t + col];
-  }
-  if (fin >= 0) {
-    // W(col, ii): every block recomputes the value of row col = fin + 1
-    // (sum of the SYMV partials of that row minus the panel corrections)
-    double s = 0.0;
-    for (int c = t; c < nch; c += blockDim.x) s += g.part[(size_t)c * n + col];
-    if (t < ii) s -= W[(size_t)n * t + col] * ab[t] + V[(size_t)n * t + col] * ab[TNB + t];
-    s = wave_sum(s);
-    if (lane == 0) red[wid][0] = s;  // red[][] was last read before the previous barrier
-    __syncthreads();
-    if (t == 0) wc[ii] = tauf * ((red[0][0] + red[1][0]) + (red[2][0] + red[3][0])) + alpha2 * 1.0;
-  }
-  __syncthreads();
   double a = acr;
   if (fin >= 0) {
     double a0 = 0.0;
 #pragma unroll
-    for (int u = 0; u < KQ; u++) {
-      const int k = KQ * q4 + u;
+    for (int u = 0; u < KO; u++) {
+      const int k = KO * q8 + u;
       const bool use = k < ii;
       a0 += (use ? vk[u] : 0.0) * (use ? wc[k] : 0.0) + (use ? wk[u] : 0.0) * (use ? vc[k] : 0.0);
     }
-    a -= quad_sum(a0) + (vii * wc[ii] + wr * vc[ii]);
+    a -= oct_sum(a0) + (vii * wcii + wr * vc[ii]);
   }
   double nrm = 0.0;
-  if (live && q4 == 0) {
+  if (live && q8 == 0) {
     ac[r] = a;
     if (r == col) {
       g.d[col] = a;
@@ -527,10 +571,9 @@ __global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, in
     if (r >= col + 2) nrm = a * a;
   }
   nrm = wave_sum(nrm);
-  __syncthreads();  // red[][] reuse
-  if (lane == 0) red[wid][1] = nrm;
+  if (lane == 0) redd[0][wid] = nrm;  // redd was last read before the second barrier
   __syncthreads();
-  if (t == 0) g.norm[blockIdx.x] = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
+  if (t == 0) g.norm[blockIdx.x] = (redd[0][0] + redd[0][1]) + (redd[0][2] + redd[0][3]);
 }
 
 // reflectors of panel columns into A below the subdiagonal (read by the back-transform)
@@ -1060,23 +1103,28 @@ int seclr4_buffers(Ctx &c, int n, int cplx, Seclr4Bufs &B)
   const int nt = (n + SYMV_T - 1) / SYMV_T;
   const size_t z = cplx ? 2 : 1;
   const size_t nts = (size_t)nt;
+  const size_t nnorm = nts + (size_t)n / COL_ROWS + 64;  // partial norms per block of 64 (complex) or 32 rows
   const size_t small = (size_t)12 * n + z * 2 * n + z * 3 * (size_t)n * TNB + z * nts * n + z * nts * 2 * TNB +
-                       nts * (nts + 1) / 2 + nts + 64;
+                       nts * (nts + 1) / 2 + 2 + nnorm + 64 + 32 * 24;
   if ((rc = c.w_small.alloc(small))) return rc;
   if ((rc = c.w_int.alloc((size_t)6 * n + 32))) return rc;
   B.n = n;
   B.nt = nt;
   B.cplx = cplx;
   double *p = c.w_small.p;
-  auto take = [&](size_t cnt) { double *q = p; p += cnt; return q; };
+  auto take = [&](size_t cnt) {  // 256-byte aligned pieces
+    double *q = p;
+    p += (cnt + 31) & ~(size_t)31;
+    return q;
+  };
   B.d = take(n); B.e = take(n); B.e2 = take(n);
   B.tau = take(z * n); B.scal = take(z * n);
   B.wv = take(n);
   B.P = take(z * 3 * (size_t)n * TNB);
   B.part = take(z * nts * n);
   B.tdots = take(z * nts * 2 * TNB);
-  B.tvy = take(nts * (nts + 1) / 2);
-  B.tnorm = take(nts);
+  B.tvy = take(nts * (nts + 1) / 2 + 2);
+  B.tnorm = take(nnorm);
   B.bgl = take(n); B.bgu = take(n); B.batol = take(n); B.bnrm = take(n); B.xs = take(n);
   B.pivmin = take(1);
   // ints: [0,4) INFO, [4,12) counts, [12,16) grid-reduction tickets, then task arrays
@@ -1307,7 +1355,7 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
       g.p0 = p0;
       g.nb = nb;
       // reflector of the panel's first column (already updated by earlier panels)
-      hipLaunchKernelGGL(k_tri_col, dim3((n - p0 + 63) / 64), dim3(256), 0, s, g, -1, p0, 0, 0);
+      hipLaunchKernelGGL(k_tri_col, dim3((n - p0 + COL_ROWS - 1) / COL_ROWS), dim3(256), 0, s, g, -1, p0, 0, 0);
       W2K_STAGE(s, "k_tri_col");
       int nref = 0;  // panel columns that carry a reflector
       for (int k = 0; k < nb; k++) {
@@ -1323,7 +1371,7 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
         if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv + 1], s));
         nsymv++;
         const int next = (k + 1 < nb) ? i + 1 : -1;
-        hipLaunchKernelGGL(k_tri_col, dim3((rows + 63) / 64), dim3(256), 0, s, g, i, next, nt, nblk);
+        hipLaunchKernelGGL(k_tri_col, dim3((rows + COL_ROWS - 1) / COL_ROWS), dim3(256), 0, s, g, i, next, nt, nblk);
         W2K_STAGE(s, "k_tri_col");
       }
       if (nref > 0) {
