@@ -100,29 +100,29 @@ __device__ __forceinline__ double lane_bcast(double v, int src)
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// X = M^-1 of a 64 x 64 upper triangular M held in LDS (M(r,q) at
-// sm[q LD + r]) by one wave, with dinv[r] = 1/M(r,r): lane c ends with
-// column c of X in x[0..63] (x[r] = 0 for r > c).  Rows from the bottom:
+// X = M^-1 of an N x N (N <= 64) upper triangular M held in LDS (M(r,q) at
+// sm[q LD + r]) by one wave, with dinv[r] = 1/M(r,r): lane c < N ends with
+// column c of X in x[0..N-1] (x[r] = 0 for r > c).  Rows from the bottom:
 // X(r,c) = -dinv[r] sum_{r<q<=c} M(r,q) X(q,c), each row of M read in one
 // batch of broadcast LDS loads; the newest term (q = r+1) is added last.
-template <int LD>
-__device__ __forceinline__ void wave_trtri64(const double *sm, const double *dinv, int lane, double *x)
+template <int N, int LD>
+__device__ __forceinline__ void wave_trtri(const double *sm, const double *dinv, int lane, double *x)
 {
 #pragma unroll
-  for (int r = 0; r < NB; r++) x[r] = 0.0;
+  for (int r = 0; r < N; r++) x[r] = 0.0;
 #pragma unroll
-  for (int r = NB - 1; r >= 0; r--) {
-    double ur[NB];
+  for (int r = N - 1; r >= 0; r--) {
+    double ur[N];
 #pragma unroll
-    for (int q = r + 1; q < NB; q++) ur[q] = sm[q * LD + r];
+    for (int q = r + 1; q < N; q++) ur[q] = sm[q * LD + r];
     double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-    for (int q = NB - 1; q > r + 1; q--) {
+    for (int q = N - 1; q > r + 1; q--) {
       if (q & 1) s1 = fma(ur[q], x[q], s1);
       else s0 = fma(ur[q], x[q], s0);
     }
     double sum = s0 + s1;
-    if (r + 1 < NB) sum = fma(ur[r + 1], x[r + 1], sum);
+    if (r + 1 < N) sum = fma(ur[r + 1], x[r + 1], sum);
     const double d = dinv[r];
     x[r] = (lane == r) ? d : -sum * d;
   }
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(64) void k_chol_diag(int n, int k0, int kb, double 
   dinv[lane] = 1.0 / sm[lane * LD + lane];
   __syncthreads();
   double x[NB];
-  wave_trtri64<LD>(sm, dinv, lane, x);
+  wave_trtri<NB, LD>(sm, dinv, lane, x);
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < NB; r++) sm[lane * LD + r] = (r <= lane) ? x[r] : 0.0;
@@ -603,98 +603,179 @@ struct BisArgs {
   double *pivmin;
 };
 
-__device__ int sturm_count(int n, const double *d, const double *e2, double x, double pivmin)
+constexpr int SB = 16;  // rows per batch of loads ahead of a serial recurrence
+
+// One row of the Sturm recurrence t <- (d - e^2 / t) - x with PIVMIN
+// (dlaebz); the quotient through the hardware reciprocal refined by two
+// Newton steps (|t| >= PIVMIN >= DBL_MIN and e^2 / PIVMIN <= 1/DBL_MIN, so
+// neither the reciprocal nor the product leaves the normal range).
+__device__ __forceinline__ void sturm_row(double &t, int &c, double dj, double e2, double x, double pivmin)
+{
+  double r = __builtin_amdgcn_rcp(t);
+  r = fma(r, fma(-t, r, 1.0), r);
+  r = fma(r, fma(-t, r, 1.0), r);
+  t = (dj - e2 * r) - x;
+  if (fabs(t) < pivmin) t = -pivmin;
+  c += (t <= 0.0);
+}
+
+// Number of eigenvalues <= x (the lane's own shift) of an unreduced block
+// (d, e) of order n, by one wave (dlaebz's Sturm count), the squared
+// off-diagonals formed as e*e (inside a block they equal the E2 of the
+// splitting step).  The recurrence is serial; d and e are read 64 rows at a
+// time, one row per lane and a chunk ahead, and handed to the recurrence by
+// readlane, four rows per step.
+__device__ __forceinline__ int sturm_count_wave(int n, const double *d, const double *e, double x, double pivmin,
+                                                int lane)
 {
   double t = d[0] - x;
   if (fabs(t) < pivmin) t = -pivmin;
   int c = (t <= 0.0);
-  for (int j = 1; j < n; j++) {
-    t = d[j] - e2[j - 1] / t - x;
-    if (fabs(t) < pivmin) t = -pivmin;
-    if (t <= 0.0) c++;
+  const int elast = max(n - 2, 0);
+  double dc = d[min(1 + lane, n - 1)];  // chunk row j0 + lane: d(j), e(j-1)
+  double ec = e[min(lane, elast)];
+  for (int j0 = 1; j0 < n; j0 += 64) {
+    const double dx = d[min(j0 + 64 + lane, n - 1)];
+    const double ex = e[min(j0 + 63 + lane, elast)];
+    const int rows = min(64, n - j0);
+    int u = 0;
+    for (; u + 4 <= rows; u += 4) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const double ej = lane_bcast(ec, u + v);
+        sturm_row(t, c, lane_bcast(dc, u + v), ej * ej, x, pivmin);
+      }
+    }
+    for (; u < rows; u++) {
+      const double ej = lane_bcast(ec, u);
+      sturm_row(t, c, lane_bcast(dc, u), ej * ej, x, pivmin);
+    }
+    dc = dx;
+    ec = ex;
   }
   return c;
 }
 
-// single thread: split points, blocks, counts, task list (dstebz2.f:301-540)
-__global__ void k_bisect_prep(BisArgs g)
+__device__ __forceinline__ double wave_max(double v)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int n = g.n;
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v)
+{
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// one wave: split points, blocks, counts, task list (dstebz2.f:301-540).
+// Pass 1 forms E2 and PIVMIN over all rows; pass 2 finds the split points
+// again in order and handles each block as it closes (one-norm, Gershgorin
+// interval, Sturm counts at its ends, its tasks).  Rows are spread over the
+// lanes; the block list and the running counts are wave-uniform and lane 0
+// writes them.
+__global__ __launch_bounds__(64) void k_bisect_prep(BisArgs g)
+{
+  if (blockIdx.x != 0) return;
+  const int n = g.n, lane = threadIdx.x;
   const double safemn = DBL_MIN, ulp = DBL_EPSILON, fudge = 2.1;
-  double pivmin = 1.0;
-  int nb = 0;
-  int start = 0;
-  for (int j = 1; j < n; j++) {
-    const double t = g.e[j - 1] * g.e[j - 1];
-    if (fabs(g.d[j] * g.d[j - 1]) * ulp * ulp + safemn > t) {
-      g.e2[j - 1] = 0.0;
-      g.bstart[nb] = start;
-      g.bend[nb] = j;
-      nb++;
-      start = j;
-    } else {
-      g.e2[j - 1] = t;
-      pivmin = fmax(pivmin, t);
+  auto split_at = [&](int j) {  // E(j-1) negligible: a block ends before row j
+    const double ej = g.e[j - 1];
+    const double t = ej * ej;
+    return fabs(g.d[j] * g.d[j - 1]) * ulp * ulp + safemn > t;
+  };
+  double pm = 1.0;
+  for (int j0 = 1; j0 < n; j0 += 64) {
+    const int j = j0 + lane;
+    if (j < n) {
+      const double ej = g.e[j - 1];
+      const double t = ej * ej;
+      const bool sp = split_at(j);
+      g.e2[j - 1] = sp ? 0.0 : t;
+      if (!sp) pm = fmax(pm, t);
     }
   }
-  g.bstart[nb] = start;
-  g.bend[nb] = n;
-  nb++;
-  pivmin *= safemn;
-  *g.pivmin = pivmin;
-  int m = 0, nwl = 0;
-  for (int b = 0; b < nb; b++) {
-    const int ib = g.bstart[b], ie = g.bend[b], in = ie - ib;
-    double onenrm = fabs(g.d[ib]) + (in > 1 ? fabs(g.e[ib]) : 0.0);
-    for (int i = ib + 1; i < ie; i++)
-      onenrm = fmax(onenrm, fabs(g.d[i]) + fabs(g.e[i - 1]) + (i < ie - 1 ? fabs(g.e[i]) : 0.0));
-    g.bnrm[b] = onenrm;
+  const double pivmin = wave_max(pm) * safemn;
+  if (lane == 0) *g.pivmin = pivmin;
+  int nb = 0, m = 0, nwl = 0;
+  auto block = [&](int ib, int ie) {
+    const int b = nb++;
+    const int in = ie - ib;
+    if (lane == 0) {
+      g.bstart[b] = ib;
+      g.bend[b] = ie;
+    }
+    double on = 0.0, gu = -DBL_MAX, gl = DBL_MAX;
+    for (int i0 = ib; i0 < ie; i0 += 64) {
+      const int i = i0 + lane;
+      if (i < ie) {
+        const double di = g.d[i];
+        const double em = (i > ib) ? fabs(g.e[i - 1]) : 0.0;
+        const double ep = (i < ie - 1) ? fabs(g.e[i]) : 0.0;
+        on = fmax(on, fabs(di) + em + ep);
+        gu = fmax(gu, di + em + ep);
+        gl = fmin(gl, di - em - ep);
+      }
+    }
+    on = wave_max(on);
+    if (lane == 0) g.bnrm[b] = on;
+    const double dib = g.d[ib];
     if (in == 1) {
-      if (g.el >= g.d[ib] - pivmin) nwl++;
-      if (g.el < g.d[ib] - pivmin && g.eu >= g.d[ib] - pivmin) {
-        g.tblk[m] = b;
-        g.tidx[m] = -1;
+      if (g.el >= dib - pivmin) nwl++;
+      if (g.el < dib - pivmin && g.eu >= dib - pivmin) {
+        if (lane == 0) {
+          g.tblk[m] = b;
+          g.tidx[m] = -1;
+        }
         m++;
       }
-      continue;
+      return;
     }
-    double gu = g.d[ib], gl = g.d[ib], t1 = 0.0;
-    for (int j = ib; j < ie - 1; j++) {
-      const double t2 = fabs(g.e[j]);
-      gu = fmax(gu, g.d[j] + t1 + t2);
-      gl = fmin(gl, g.d[j] - t1 - t2);
-      t1 = t2;
-    }
-    gu = fmax(gu, g.d[ie - 1] + t1);
-    gl = fmin(gl, g.d[ie - 1] - t1);
+    gu = wave_max(gu);
+    gl = wave_min(gl);
     const double bnorm = fmax(fabs(gl), fabs(gu));
     gl = gl - fudge * bnorm * ulp * in - fudge * pivmin;
     gu = gu + fudge * bnorm * ulp * in + fudge * pivmin;
-    g.batol[b] = ulp * fmax(fabs(gl), fabs(gu));
+    if (lane == 0) g.batol[b] = ulp * fmax(fabs(gl), fabs(gu));
     if (gu < g.el) {
       nwl += in;
-      continue;
+      return;
     }
     gl = fmax(gl, g.el);
     gu = fmin(gu, g.eu);
-    g.bgl[b] = gl;
-    g.bgu[b] = gu;
-    if (gl >= gu) continue;
-    const int cl = sturm_count(in, g.d + ib, g.e2 + ib, gl, pivmin);
-    const int cu = sturm_count(in, g.d + ib, g.e2 + ib, gu, pivmin);
+    if (lane == 0) {
+      g.bgl[b] = gl;
+      g.bgu[b] = gu;
+    }
+    if (gl >= gu) return;
+    const int cl = sturm_count_wave(in, g.d + ib, g.e + ib, gl, pivmin, lane);
+    const int cu = sturm_count_wave(in, g.d + ib, g.e + ib, gu, pivmin, lane);
     nwl += cl;
-    for (int k = cl; k < cu; k++) {
-      if (m < n) {
-        g.tblk[m] = b;
-        g.tidx[m] = k;
+    for (int k = cl + lane; k < cu; k += 64) {
+      const int q = m + (k - cl);
+      if (q < n) {
+        g.tblk[q] = b;
+        g.tidx[q] = k;
       }
-      m++;
+    }
+    m += cu - cl;
+  };
+  int start = 0;
+  for (int j0 = 1; j0 < n; j0 += 64) {
+    const int j = j0 + lane;
+    unsigned long long mask = __ballot(j < n && split_at(j));
+    while (mask) {
+      const int bit = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      block(start, j0 + bit);
+      start = j0 + bit;
     }
   }
-  g.cnt[0] = m < g.nume ? m : g.nume;
-  g.cnt[1] = nwl;
-  g.cnt[2] = nb;
+  block(start, n);
+  if (lane == 0) {
+    g.cnt[0] = m < g.nume ? m : g.nume;
+    g.cnt[1] = nwl;
+    g.cnt[2] = nb;
+  }
 }
 
 // multisection: one wave per eigenvalue; each round the 64 lanes take the
@@ -720,7 +801,7 @@ __global__ __launch_bounds__(256) void k_bisect(BisArgs g, int m, double *__rest
     const double h = (hi - lo) / 65.0;
     double x = lo + h * (double)(lane + 1);
     if (lane == 63 || !(x < hi)) x = hi;  // the last point is hi itself
-    const int cnt = sturm_count(in, g.d + ib, g.e2 + ib, x, pivmin);
+    const int cnt = sturm_count_wave(in, g.d + ib, g.e + ib, x, pivmin, lane);
     const unsigned long long above = __ballot(cnt > k);
     if (above == 0ull) {
       lo = hi;  // cannot happen for a valid bracket; stop narrowing
@@ -786,26 +867,37 @@ __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__
   const double tol = fmax(fabs(lam), onenrm) * DBL_EPSILON;
   // factorisation; small pivots of U perturbed to +-tol for the solves
   double a0 = d[0] - lam, a1 = e[0];
-  for (int k = 0; k < bs - 1; k++) {
-    const double sub = e[k];
-    const double dn = d[k + 1] - lam, en = (k + 1 < bs - 1) ? e[k + 1] : 0.0;
-    const size_t o = (size_t)k * M + t;
-    double p0, p1, p2, mm;
-    int pv;
-    if (fabs(a0) >= fabs(sub)) {
-      pv = 0; p0 = a0; p1 = a1; p2 = 0.0;
-      mm = (a0 != 0.0) ? sub / a0 : 0.0;
-      a0 = dn - mm * a1;
-      a1 = en;
-    } else {
-      pv = 1; p0 = sub; p1 = dn; p2 = en;
-      mm = a0 / sub;
-      const double na0 = a1 - mm * dn, na1 = -mm * en;
-      a0 = na0;
-      a1 = na1;
+  for (int k0 = 0; k0 < bs - 1; k0 += SB) {
+    double dbt[SB], ebt[SB + 1];  // d(k0+1 ..), e(k0 ..) of this batch of rows
+#pragma unroll
+    for (int u = 0; u < SB; u++) dbt[u] = d[min(k0 + 1 + u, bs - 1)];
+#pragma unroll
+    for (int u = 0; u <= SB; u++) ebt[u] = e[min(k0 + u, bs - 2)];
+#pragma unroll
+    for (int kq = 0; kq < SB; kq++) {
+      const int k = k0 + kq;
+      if (k < bs - 1) {
+        const double sub = ebt[kq];
+        const double dn = dbt[kq] - lam, en = (k + 1 < bs - 1) ? ebt[kq + 1] : 0.0;
+        const size_t o = (size_t)k * M + t;
+        double p0, p1, p2, mm;
+        int pv;
+        if (fabs(a0) >= fabs(sub)) {
+          pv = 0; p0 = a0; p1 = a1; p2 = 0.0;
+          mm = (a0 != 0.0) ? sub / a0 : 0.0;
+          a0 = dn - mm * a1;
+          a1 = en;
+        } else {
+          pv = 1; p0 = sub; p1 = dn; p2 = en;
+          mm = a0 / sub;
+          const double na0 = a1 - mm * dn, na1 = -mm * en;
+          a0 = na0;
+          a1 = na1;
+        }
+        if (fabs(p0) < tol) p0 = (p0 < 0 ? -tol : tol);
+        piv[o] = pv; u0[o] = p0; u1[o] = p1; u2[o] = p2; mul[o] = mm;
+      }
     }
-    if (fabs(p0) < tol) p0 = (p0 < 0 ? -tol : tol);
-    piv[o] = pv; u0[o] = p0; u1[o] = p1; u2[o] = p2; mul[o] = mm;
   }
   {
     const size_t o = (size_t)(bs - 1) * M + t;
@@ -820,7 +912,7 @@ __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__
     return ((double)(seed >> 11) / 9007199254740992.0) * 2.0 - 1.0;
   };
   double sc = 1.0;
-  constexpr int G = 4;  // rows per group: loads of a group are issued before its arithmetic
+  constexpr int G = 16;  // rows per group: loads of a group are issued before its arithmetic
   for (int it = 0; it < 3; it++) {
     // forward: L^-1 P x, x scaled by 1/max|x| of the previous solve
     double xk = (it == 0) ? rnd() : sc * x[t];
@@ -993,7 +1085,7 @@ __global__ __launch_bounds__(256) void k_larft_inv(int nb, const double *__restr
   if (t < BTNB) dinv[t] = (t < nb && tau[t] != 0.0) ? tau[t] : 1.0;
   __syncthreads();
   double x[NB];
-  if (wid < 2 && (wid == 0 || nb > NB)) wave_trtri64<LDM>(ms + wid * NB * (LDM + 1), dinv + wid * NB, lane, x);
+  if (wid < 2 && (wid == 0 || nb > NB)) wave_trtri<NB, LDM>(ms + wid * NB * (LDM + 1), dinv + wid * NB, lane, x);
   __syncthreads();
   if (wid < 2 && (wid == 0 || nb > NB)) {
     double *blk = ms + wid * NB * (LDM + 1);
@@ -1036,55 +1128,51 @@ __global__ __launch_bounds__(256) void k_larft_inv(int nb, const double *__restr
   }
 }
 
-// Cholesky G = R^T R of a small Gram matrix (cw <= 32) and R^-1 (upper);
-// *fail set when G is not numerically positive definite
+// Cholesky G = R^T R of a small Gram matrix (cw <= TNB) and R^-1 (upper),
+// by one wave in registers as k_chol_diag (lane c holds column c; columns
+// past cw and the lanes above TNB carry the identity); *fail set when G is
+// not numerically positive definite (a non-positive or non-finite pivot,
+// taken as 1 so that the arithmetic stays finite)
 __global__ __launch_bounds__(64) void k_chol_rinv(int cw, const double *__restrict__ g, double *__restrict__ rinv,
                                                   int *__restrict__ fail)
 {
-  __shared__ double a[TNB][TNB + 1], ri[TNB][TNB + 1];
-  __shared__ int bad;
-  const int t = threadIdx.x;
-  if (t == 0) bad = 0;
-  for (int idx = t; idx < cw * cw; idx += blockDim.x) {
-    const int r = idx % cw, c = idx / cw;
-    a[r][c] = (r <= c) ? g[r + cw * c] : 0.0;
-    ri[r][c] = 0.0;
+  constexpr int N = TNB, LD = N + 1;
+  __shared__ double sm[64 * LD];  // sm[c LD + r] = R(r, c), then R^-1(r, c)
+  __shared__ double urow[2][64];
+  __shared__ double dinv[64];
+  const int c = threadIdx.x;
+  double v[N];
+#pragma unroll
+  for (int r = 0; r < N; r++) v[r] = (r <= c && c < cw) ? g[r + (size_t)cw * c] : (r == c ? 1.0 : 0.0);
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < N; j++) {
+    const double ajj = lane_bcast(v[j], j);
+    const bool ok = ajj > 0.0 && ajj <= DBL_MAX;
+    bad = bad || !ok;
+    v[j] *= rsqrt_nr(ok ? ajj : 1.0);
+    urow[j & 1][c] = v[j];
+    __syncthreads();
+    double u[N];
+#pragma unroll
+    for (int r = j + 1; r < N; r++) u[r] = urow[j & 1][r];
+#pragma unroll
+    for (int r = j + 1; r < N; r++) v[r] = fma(-u[r], v[j], v[r]);
   }
   __syncthreads();
-  for (int j = 0; j < cw; j++) {
-    if (t == 0) {
-      const double dj = a[j][j];
-      if (!(dj > 0.0) || !isfinite(dj)) {
-        bad = 1;
-        a[j][j] = 1.0;
-      } else {
-        a[j][j] = sqrt(dj);
-      }
-    }
-    __syncthreads();
-    if (t > j && t < cw) a[j][t] /= a[j][j];
-    __syncthreads();
-    const int m = cw - j - 1;
-    for (int idx = t; idx < m * m; idx += blockDim.x) {
-      const int r = j + 1 + idx % m, c = j + 1 + idx / m;
-      if (r <= c) a[r][c] -= a[j][r] * a[j][c];
-    }
-    __syncthreads();
-  }
-  if (t < cw) {  // column t of R^-1 by back substitution
-    const int c = t;
-    for (int r = c; r >= 0; r--) {
-      double v = (r == c) ? 1.0 : 0.0;
-      for (int q = r + 1; q <= c; q++) v -= a[r][q] * ri[q][c];
-      ri[r][c] = v / a[r][r];
-    }
-  }
+#pragma unroll
+  for (int r = 0; r < N; r++) sm[c * LD + r] = (r <= c) ? v[r] : 0.0;
   __syncthreads();
-  for (int idx = t; idx < cw * cw; idx += blockDim.x) {
-    const int r = idx % cw, c = idx / cw;
-    rinv[r + cw * c] = (r <= c) ? ri[r][c] : 0.0;
+  dinv[c] = (c < N) ? 1.0 / sm[c * LD + c] : 1.0;
+  __syncthreads();
+  double x[N];
+  wave_trtri<N, LD>(sm, dinv, c, x);
+  if (c < cw) {
+#pragma unroll
+    for (int r = 0; r < N; r++)
+      if (r < cw) rinv[r + (size_t)cw * c] = (r <= c) ? x[r] : 0.0;
   }
-  if (t == 0 && bad) *fail = 1;
+  if (c == 0 && bad) *fail = 1;
 }
 
 __global__ void k_gather_cols(int n, int m, const int *__restrict__ perm, const double *__restrict__ src,
