@@ -11,7 +11,8 @@
 //      MFMA GEMM on the upper triangle.
 //   2. Reduction to standard form C = U^-T H U^-1 (DSCGST, dscgst.f:99-142):
 //      W = U^-1 by blocked triangular inversion, then X = H W and C = W^T X as
-//      MFMA GEMMs (lower triangle of C, mirrored).
+//      MFMA GEMMs (lower triangle of C; the tridiagonalisation reads and
+//      updates only the lower triangle).
 //   3. Householder tridiagonalisation (DSYTRD2, dsytrd2.f, lower; DLATRD
 //      panels of 32 columns, trailing rank-64 update on MFMA), DLARFG signs.
 //   4. Bisection (DSTEBZ2, RANGE='V', ORDER='B', ABSTOL=0, dstebz2.f:301-652):
@@ -194,15 +195,6 @@ __global__ void k_fill(size_t cnt, double *p, double v)
   if (i < cnt) p[i] = v;
 }
 
-// C(j,i) = C(i,j), i > j (leading dimension ldc)
-__global__ void k_mirror_lower(int n, int ldc, double *c)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= n || i <= j) return;
-  c[(size_t)j + (size_t)ldc * i] = c[(size_t)i + (size_t)ldc * j];
-}
-
 // ------------------------------------------------------------------ tridiagonalisation
 // Panel of TNB columns (DLATRD, lower): the panel buffer P holds, for panel
 // column k, the reflector V(:,k) at column k, W(:,k) at column nb+k and a
@@ -273,10 +265,11 @@ __device__ __forceinline__ double tri_reflector(const TriWork &g, int i, int lan
   return scal;
 }
 
-// y = A(r0:n, r0:n) v from the tiles on and below the diagonal only (A is
-// symmetric): one wave per 64 x 64 tile (I, J), J <= I, adds A_IJ v_J to the
-// rows of tile I and, for J < I, A_IJ^T v_I to the rows of tile J (column
-// sums by a wave-wide reduce-scatter, no LDS).  Row r of tile R receives
+// y = A(r0:n, r0:n) v from the lower triangle only (A is symmetric): one
+// wave per 64 x 64 tile (I, J), J <= I, adds A_IJ v_J to the rows of tile I
+// and, for J < I, A_IJ^T v_I to the rows of tile J (column sums by a
+// wave-wide reduce-scatter, no LDS); a diagonal tile adds its lower part by
+// rows and its strict lower part by columns.  Row r of tile R receives
 // exactly one partial per tile index: from (R, J) at slot J <= R and from
 // (I, R) at slot I > R, so y(r) = sum over slots of part[slot][r] in a fixed
 // order.  Every tile wave writes its partial v^T y.  The nt waves after the
@@ -353,6 +346,13 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
 #pragma unroll
     for (int j = 0; j < SYMV_T; j++) t[j] = (live && j < m) ? t[j] : 0.0;
   }
+  // only the lower triangle of A is kept current (the trailing updates skip
+  // the upper one): a diagonal tile uses its lower part, by rows and (strictly
+  // below the diagonal) by columns
+  if (!off) {
+#pragma unroll
+    for (int j = 0; j < SYMV_T; j++) t[j] = (j <= lane) ? t[j] : 0.0;
+  }
   // row sums: v(C0 + j) broadcast from lane j through scalar registers
   const long long vcb = __double_as_longlong(vc);
   const int vlo = (int)(unsigned)(vcb & 0xffffffffll), vhi = (int)(unsigned)((unsigned long long)vcb >> 32);
@@ -371,7 +371,12 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
     y2 += t[j + 2] * vj[2];
     y3 += t[j + 3] * vj[3];
   }
-  const double yrow = (y0 + y1) + (y2 + y3);
+  double yrow = (y0 + y1) + (y2 + y3);
+  if (!off) {
+#pragma unroll
+    for (int j = 0; j < SYMV_T; j++) t[j] = (j < lane) ? t[j] * vr : 0.0;
+    yrow += wave_transpose_sum(t, lane);  // column C0 + lane = row R of the strict lower part
+  }
   if (live) g.part[(size_t)J * n + R] = yrow;
   // v^T y over the tile: an off-diagonal tile adds v_I^T A_IJ v_J through its
   // rows and the same number again through its columns
@@ -1404,7 +1409,7 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
   W2K_HIP(hipMemcpyAsync(hinfo, info, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
   W2K_HIP(hipStreamSynchronize(s));
   if (hinfo[0] != 0) return hinfo[0];  // POTRF INFO, seclr4.F:200-205
-  // ---- 2. reduction: W = U^-1, X = H W, C = W^T X (lower), mirrored
+  // ---- 2. reduction: W = U^-1, X = H W, C = W^T X (lower triangle only)
   {
     for (int j0 = NB; j0 < n; j0 += NB) {
       const int kb = std::min(NB, n - j0);
@@ -1419,9 +1424,6 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
     }
     if ((rc = dgemm(s, 'N', 'N', n, n, n, 1.0, Hf, lda, W, n, 0.0, X.p, n, 0, 0, 1))) return rc;
     if ((rc = dgemm(s, 'T', 'N', n, n, n, 1.0, W, n, X.p, n, 0.0, Hf, lda, 1, 2, 0))) return rc;
-    dim3 g2((n + 255) / 256, n);
-    hipLaunchKernelGGL(k_mirror_lower, g2, dim3(256), 0, s, n, lda, Hf);
-    W2K_STAGE(s, "k_mirror_lower");
   }
   W2K_HIP(hipEventRecord(c.ev[8], s));
   // ---- 3. tridiagonalisation of C (in Hf)
@@ -1468,10 +1470,10 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
       }
       const int p1 = p0 + nb;
       if (p1 < n) {
-        // A(p1:n, p1:n) -= [V W] [W V]^T
+        // A(p1:n, p1:n) -= [V W] [W V]^T, lower triangle
         const int r = n - p1;
         double *A1 = A + (size_t)p1 + (size_t)lda * p1;
-        if ((rc = dgemm(s, 'N', 'T', r, r, 2 * nb, -1.0, P + p1, n, P + (size_t)n * nb + p1, n, 1.0, A1, lda)))
+        if ((rc = dgemm(s, 'N', 'T', r, r, 2 * nb, -1.0, P + p1, n, P + (size_t)n * nb + p1, n, 1.0, A1, lda, 1)))
           return rc;
       }
     }
