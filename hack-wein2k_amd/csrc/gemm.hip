@@ -162,6 +162,129 @@ __global__ __launch_bounds__(256) void k_dgemm(GemmArgs g)
   }
 }
 
+// Large products: tile 128x128x16, 256 threads = 4 wave64 in a 2x2
+// arrangement, each wave a 64x64 block of sixteen 16x16 MFMA accumulators
+// (eight operand reads from LDS per sixteen MFMAs).  The transposes are
+// template parameters, and every global load is unconditional (indices
+// clamped into the matrix, the value masked afterwards) so that a slab's
+// loads are issued together; LDS double-buffered as above.
+constexpr int LM = 128, LN = 128, LK = 16;
+
+template <int TA, int TB>
+__global__ __launch_bounds__(256, 2) void k_dgemm_big(GemmArgs g)
+{
+  __shared__ double As[2][LK][LM + PAD];
+  __shared__ double Bs[2][LK][LN + PAD];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * LM, j0 = blockIdx.y * LN;
+  if (g.tri == 1 && i0 + LM - 1 < j0) return;
+  if (g.tri == 2 && j0 + LN - 1 < i0) return;
+  int kbeg = 0, kend = g.k;
+  if (g.atri == 1) kbeg = max(kbeg, i0);
+  if (g.atri == 2) kend = min(kend, i0 + LM);
+  if (g.btri == 1) kend = min(kend, j0 + LN);
+  if (g.btri == 2) kbeg = max(kbeg, j0);
+  kbeg = (kbeg / LK) * LK;
+
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wi = (wave & 1) * 64, wj = (wave >> 1) * 64;
+  const int li = lane & 15, lk = lane >> 4;
+  d4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+
+  // this thread's eight elements of each operand per slab: op(A)(ia, ka + step),
+  // op(B)(kb + step, jb) with the step along k (no transpose) or the row / column
+  const int ia = TA ? i0 + (tid >> 4) : i0 + (tid & 127);
+  const int ka = TA ? (tid & 15) : (tid >> 7);
+  const int jb = TB ? j0 + (tid & 127) : j0 + (tid >> 4);
+  const int kb = TB ? (tid >> 7) : (tid & 15);
+  double ra[8], rb[8];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int i = TA ? ia + 16 * r : ia;
+      const int kk = TA ? k0 + ka : k0 + ka + 2 * r;
+      bool ok = i < g.m && kk < kend;
+      if (g.atri == 1) ok = ok && kk >= i;
+      if (g.atri == 2) ok = ok && kk <= i;
+      const int ic = min(i, g.m - 1), kc = min(kk, g.k - 1);
+      const double v = TA ? g.A[(size_t)kc + (size_t)g.lda * ic] : g.A[(size_t)ic + (size_t)g.lda * kc];
+      ra[r] = ok ? v : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int j = TB ? jb : jb + 16 * r;
+      const int kk = TB ? k0 + kb + 2 * r : k0 + kb;
+      bool ok = j < g.n && kk < kend;
+      if (g.btri == 1) ok = ok && kk <= j;
+      if (g.btri == 2) ok = ok && kk >= j;
+      const int jc = min(j, g.n - 1), kc = min(kk, g.k - 1);
+      const double v = TB ? g.B[(size_t)jc + (size_t)g.ldb * kc] : g.B[(size_t)kc + (size_t)g.ldb * jc];
+      rb[r] = ok ? v : 0.0;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      if (TA) As[buf][ka][(tid >> 4) + 16 * r] = ra[r];
+      else As[buf][ka + 2 * r][tid & 127] = ra[r];
+      if (TB) Bs[buf][kb + 2 * r][tid & 127] = rb[r];
+      else Bs[buf][kb][(tid >> 4) + 16 * r] = rb[r];
+    }
+  };
+
+  if (kbeg < kend) {
+    gload(kbeg);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += LK) {
+      const bool more = (k0 + LK < kend);
+      if (more) gload(k0 + LK);
+#pragma unroll
+      for (int kq = 0; kq < LK; kq += 4) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) av[a] = As[buf][kq + lk][wi + 16 * a + li];
+#pragma unroll
+        for (int b = 0; b < 4; b++) bv[b] = Bs[buf][kq + lk][wj + 16 * b + li];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++)
+            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+      }
+      if (more) {
+        sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+      }
+    }
+  }
+  // epilogue: as in k_dgemm (lane holds rows lane/16 + 4r, column lane%16)
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int j = j0 + wj + 16 * b + li;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = i0 + wi + 16 * a + lk + 4 * r;
+        if (i >= g.m || j >= g.n) continue;
+        if (g.tri == 1 && i < j) continue;
+        if (g.tri == 2 && i > j) continue;
+        double *cp = g.C + (size_t)i + (size_t)g.ldc * j;
+        double v = g.alpha * acc[a][b][r];
+        if (g.beta != 0.0) v += g.beta * *cp;
+        *cp = v;
+      }
+    }
+  }
+}
+
 // C = alpha * sum_z ws[z] + beta * C, fixed summation order
 __global__ void k_splitk_reduce(int m, int n, int nz, const double *__restrict__ ws, double alpha, double beta,
                                 double *__restrict__ C, int ldc)
@@ -191,6 +314,17 @@ int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
   dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
   if (k <= 0) {
     g.k = 0;  // still applies beta
+  }
+  // large outputs with a long K: the 128x128 tile kernel (a short K, such as
+  // a rank-64 update, runs faster on the 64x64 tiles)
+  if (m >= 256 && n >= 256 && k >= 256) {
+    dim3 gb((m + LM - 1) / LM, (n + LN - 1) / LN);
+    if (!g.ta && !g.tb) hipLaunchKernelGGL((k_dgemm_big<0, 0>), gb, dim3(256), 0, s, g);
+    else if (g.ta && !g.tb) hipLaunchKernelGGL((k_dgemm_big<1, 0>), gb, dim3(256), 0, s, g);
+    else if (!g.ta && g.tb) hipLaunchKernelGGL((k_dgemm_big<0, 1>), gb, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((k_dgemm_big<1, 1>), gb, dim3(256), 0, s, g);
+    W2K_STAGE(s, "k_dgemm_big");
+    return 0;
   }
   // few output tiles and a long K: split K over z-slices so that the whole
   // chip works, then reduce the slices in a fixed order

@@ -64,6 +64,43 @@ def test_dgemm_split_k_matches_numpy(ta, tb, m, n, k):
     assert np.max(np.abs(out - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(256, 256, 256), (300, 257, 270), (513, 384, 401), (259, 700, 131)])
+def test_dgemm_large_tiles_match_numpy(ta, tb, m, n, k):
+    """Outputs of at least 256 x 256 with K >= 256 take the 128 x 128 tile
+    kernel; ragged edges in every dimension (and one short-K case for the
+    64 x 64 tiles)."""
+    test_dgemm_matches_numpy(ta, tb, m, n, k)
+
+
+@pytest.mark.parametrize("tri", [1, 2])
+@pytest.mark.parametrize("n,k", [(150, 40), (389, 64), (389, 300)])
+def test_dgemm_triangular_output_sizes(tri, n, k):
+    rng = np.random.default_rng(n + k + tri)
+    A = rng.standard_normal((n, k)); B = rng.standard_normal((n, k)); C = rng.standard_normal((n, n))
+    out = _dgemm("N", "T", n, n, k, 1.0, A, B, 1.0, C, tri=tri)
+    full = C + A @ B.T
+    mask = np.tril(np.ones((n, n), bool)) if tri == 1 else np.triu(np.ones((n, n), bool))
+    assert np.max(np.abs(out[mask] - full[mask])) < 1e-11
+    assert np.array_equal(out[~mask], C[~mask])
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N")])
+@pytest.mark.parametrize("atri,btri", [(1, 0), (2, 0), (0, 1), (0, 2)])
+def test_dgemm_triangular_operands_large(ta, tb, atri, btri):
+    """Triangular op(A) / op(B) in the 128 x 128 tile kernel (k range clipped
+    per tile, elements outside the triangle masked; the storage there holds
+    values that must not be read)."""
+    rng = np.random.default_rng(31 + atri + 3 * btri)
+    m = n = k = 301
+    A = rng.standard_normal((m, k)); B = rng.standard_normal((k, n)); C = np.zeros((m, n))
+    opA = A if ta == "N" else A.T  # square: op(A) has the shape of A
+    Am = np.triu(opA) if atri == 1 else (np.tril(opA) if atri == 2 else opA)
+    Bm = np.triu(B) if btri == 1 else (np.tril(B) if btri == 2 else B)
+    out = _dgemm(ta, tb, m, n, k, 1.0, A, B, 0.0, C, atri=atri, btri=btri)
+    assert np.max(np.abs(out - Am @ Bm)) < 1e-11
+
+
 @pytest.mark.parametrize("tri", [1, 2])
 def test_dgemm_triangular_output(tri):
     rng = np.random.default_rng(5)
