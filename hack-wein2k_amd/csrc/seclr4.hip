@@ -147,9 +147,14 @@ __global__ __launch_bounds__(64) void k_chol_diag(int n, int k0, int kb, double 
   __shared__ double urow[2][NB];
   __shared__ double dinv[NB];
   const int lane = threadIdx.x;
-  for (int c = 0; c < NB; c++) {
-    const bool in = lane <= c && c < kb;
-    sm[c * LD + lane] = in ? a[(size_t)(k0 + lane) + (size_t)n * (k0 + c)] : (lane == c ? 1.0 : 0.0);
+  {
+    // all 64 loads in flight together (clamped addresses, masked values)
+    double ld[NB];
+    const int rl = min(lane, kb - 1);
+#pragma unroll
+    for (int c = 0; c < NB; c++) ld[c] = a[(size_t)(k0 + rl) + (size_t)n * (k0 + min(c, kb - 1))];
+#pragma unroll
+    for (int c = 0; c < NB; c++) sm[c * LD + lane] = (lane <= c && c < kb) ? ld[c] : (lane == c ? 1.0 : 0.0);
   }
   __syncthreads();
   double v[NB];
@@ -829,20 +834,34 @@ __global__ void k_stein_prep(BisArgs g, int m, const double *__restrict__ w, dou
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int prevb = -1, gp = 0;
   double xjm = 0.0;
-  for (int t = 0; t < m; t++) {
-    const int b = g.tblk[t];
-    double xj = w[t];
-    if (b != prevb) {
-      gp = t;
-    } else {
-      const double pertol = 10.0 * fabs(DBL_EPSILON * xj);
-      if (xj - xjm < pertol) xj = xjm + pertol;
-      if (xj - xjm > 1.0e-3 * g.bnrm[b]) gp = t;
+  for (int t0 = 0; t0 < m; t0 += SB) {
+    int bb[SB];
+    double wb[SB], nb[SB];
+#pragma unroll
+    for (int u = 0; u < SB; u++) {
+      bb[u] = g.tblk[min(t0 + u, m - 1)];
+      wb[u] = w[min(t0 + u, m - 1)];
     }
-    xs[t] = xj;
-    cstart[t] = gp;
-    xjm = xj;
-    prevb = b;
+#pragma unroll
+    for (int u = 0; u < SB; u++) nb[u] = g.bnrm[bb[u]];
+#pragma unroll
+    for (int u = 0; u < SB; u++) {
+      const int t = t0 + u;
+      if (t >= m) continue;
+      const int b = bb[u];
+      double xj = wb[u];
+      if (b != prevb) {
+        gp = t;
+      } else {
+        const double pertol = 10.0 * fabs(DBL_EPSILON * xj);
+        if (xj - xjm < pertol) xj = xjm + pertol;
+        if (xj - xjm > 1.0e-3 * nb[u]) gp = t;
+      }
+      xs[t] = xj;
+      cstart[t] = gp;
+      xjm = xj;
+      prevb = b;
+    }
   }
 }
 
@@ -1075,19 +1094,34 @@ __global__ __launch_bounds__(256) void k_larft_inv(int nb, const double *__restr
 {
   constexpr int LDM = BTNB + 1;
   __shared__ double ms[BTNB * LDM];  // ms[c LDM + r] = M(r, c), then T(r, c)
-  __shared__ double dinv[BTNB];
+  __shared__ double dinv[BTNB], taus[BTNB];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  for (int idx = t; idx < BTNB * BTNB; idx += blockDim.x) {
-    const int r = idx & (BTNB - 1), c = idx / BTNB;
-    double v = 0.0;
-    if (r == c) {
-      v = (r < nb && tau[r] != 0.0) ? 1.0 / tau[r] : 1.0;
-    } else if (r < c && c < nb) {
-      v = (tau[r] != 0.0 && tau[c] != 0.0) ? g[r + (size_t)nb * c] : 0.0;
-    }
-    ms[c * LDM + r] = v;
+  if (t < BTNB) {
+    const double tt = (t < nb) ? tau[t] : 0.0;
+    taus[t] = tt;
+    dinv[t] = (tt != 0.0) ? tt : 1.0;
   }
-  if (t < BTNB) dinv[t] = (t < nb && tau[t] != 0.0) ? tau[t] : 1.0;
+  __syncthreads();
+  {
+    // thread t: row r = t % 128 of columns c = t / 128 + 2u; every load of G
+    // in flight together (clamped addresses, masked values)
+    const int r = t & (BTNB - 1), c0 = t >> 7;
+    double gv[BTNB / 2];
+#pragma unroll
+    for (int u = 0; u < BTNB / 2; u++) gv[u] = g[min(r, nb - 1) + (size_t)nb * min(c0 + 2 * u, nb - 1)];
+    const double taur = taus[r];
+#pragma unroll
+    for (int u = 0; u < BTNB / 2; u++) {
+      const int c = c0 + 2 * u;
+      double v = 0.0;
+      if (r == c) {
+        v = (taur != 0.0) ? 1.0 / taur : 1.0;
+      } else if (r < c && c < nb) {
+        v = (taur != 0.0 && taus[c] != 0.0) ? gv[u] : 0.0;
+      }
+      ms[c * LDM + r] = v;
+    }
+  }
   __syncthreads();
   double x[NB];
   if (wid < 2 && (wid == 0 || nb > NB)) wave_trtri<NB, LDM>(ms + wid * NB * (LDM + 1), dinv + wid * NB, lane, x);
@@ -1128,7 +1162,7 @@ __global__ __launch_bounds__(256) void k_larft_inv(int nb, const double *__restr
   for (int idx = t; idx < nb * nb; idx += blockDim.x) {
     const int q = idx % nb, c = idx / nb;
     double v = (q <= c) ? ms[c * LDM + q] : 0.0;
-    if (q == c && tau[q] == 0.0) v = 0.0;
+    if (q == c && taus[q] == 0.0) v = 0.0;
     tm[idx] = v;
   }
 }
@@ -1147,8 +1181,11 @@ __global__ __launch_bounds__(64) void k_chol_rinv(int cw, const double *__restri
   __shared__ double dinv[64];
   const int c = threadIdx.x;
   double v[N];
+  const int cl = min(c, cw - 1);
 #pragma unroll
-  for (int r = 0; r < N; r++) v[r] = (r <= c && c < cw) ? g[r + (size_t)cw * c] : (r == c ? 1.0 : 0.0);
+  for (int r = 0; r < N; r++) v[r] = g[min(r, cw - 1) + (size_t)cw * cl];  // clamped: all loads in flight
+#pragma unroll
+  for (int r = 0; r < N; r++) v[r] = (r <= c && c < cw) ? v[r] : (r == c ? 1.0 : 0.0);
   bool bad = false;
 #pragma unroll
   for (int j = 0; j < N; j++) {
