@@ -280,7 +280,8 @@ def main():
                      "work_per_launch": work / nl, "ms_per_launch": round(kms / nl, 4), "launches": nl,
                      "other_kernels": others},
         "cpu_baseline": cpu,
-        "phases_ms": {k: round(v, 2) for k, v in t.items()},
+        "phases_ms": {k: round(v, 2) for k, v in t.items() if k != "tri_launch_mode"},
+        "tri_launch_mode": {0: "direct", 1: "hip graph replay", 2: "hip graph capture"}.get(int(t.get("tri_launch_mode", 0)), "?"),
         "check": {"ne": ne, "nbelw": nb, "ne_ref": int(kd["NE"]), "max_abs_eig_dev_ry": dmax},
         "fp64": {"value": round(ms_fp64 / 1e3, 4), "ms_per_step": round(ms_fp64, 2), "max_abs_eig_dev_ry": dmax},
         "split_precision": split,

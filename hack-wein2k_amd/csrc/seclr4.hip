@@ -1394,6 +1394,65 @@ int sort_eigen(Ctx &c, Seclr4Bufs &B, int m)
   return 0;
 }
 
+// Launch mode of the tridiagonalisation (2n kernel launches and n/32 GEMMs
+// with no host read-back): direct launches, or one HIP graph captured from
+// them and replayed.  The choice is made per shape by measurement: the first
+// call with a given key (order, leading dimension and the buffers the
+// launches address) launches directly, the second captures the launches into
+// a graph and replays it, and later calls replay it only if the first replay
+// was faster than the direct launches (the phase time of the previous call
+// decides).  Kernel timing, W2K_SYNC_CHECK or a failed capture keep the
+// direct launches.
+template <class F>
+static int tri_run(Ctx &c, hipStream_t s, int n, int lda, const double *a, bool kt, F &launch)
+{
+  Ctx::TriGraph &tg = c.tri_graph;
+  const bool same = tg.n == n && tg.lda == lda && tg.a == a && tg.small == c.w_small.p && tg.ints == c.w_int.p;
+  if (same) {
+    // the phase time of the previous call, by the mode it used
+    if (tg.mode == 0) tg.direct_ms = c.last_ms[5];
+    if (tg.mode == 1) {
+      tg.graph_ms = c.last_ms[5];
+      if (tg.direct_ms > 0.0 && tg.graph_ms >= tg.direct_ms) tg.use = 0;
+    }
+  } else {
+    if (tg.exec) (void)hipGraphExecDestroy(tg.exec);
+    tg = Ctx::TriGraph();
+    tg.n = n; tg.lda = lda; tg.a = a; tg.small = c.w_small.p; tg.ints = c.w_int.p;
+  }
+  tg.seen++;
+  tg.mode = -1;
+  const bool allowed = c.graphs && !kt && !sync_checks_enabled() && tg.use;
+  if (allowed && tg.exec) {
+    W2K_HIP(hipGraphLaunch(tg.exec, s));
+    tg.mode = c.last_ms[12] = 1;
+    return 0;
+  }
+  if (allowed && tg.seen >= 2) {
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      hipGraph_t graph = nullptr;
+      const int rc = launch();
+      const hipError_t e = hipStreamEndCapture(s, &graph);
+      hipGraphExec_t ex = nullptr;
+      if (rc == 0 && e == hipSuccess && hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0) == hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        tg.exec = ex;
+        W2K_HIP(hipGraphLaunch(ex, s));
+        tg.mode = c.last_ms[12] = 2;
+        return 0;
+      }
+      if (graph) (void)hipGraphDestroy(graph);
+    }
+    // nothing of a failed capture ran: clear the error and launch directly
+    (void)hipGetLastError();
+    tg.use = 0;
+  }
+  const int rc = launch();
+  if (rc) return rc;
+  tg.mode = c.last_ms[12] = 0;
+  return 0;
+}
+
 static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbelw_out)
 {
   const int n = c.n;
@@ -1477,43 +1536,51 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
     TriWork g;
     g.n = n; g.lda = lda; g.a = A; g.P = P; g.d = d; g.e = e; g.tau = tau; g.scal = scal;
     g.part = B.part; g.dots = B.tdots; g.vy = B.tvy; g.norm = B.tnorm; g.counter = B.ticket;
-    for (int p0 = 0; p0 < n; p0 += TNB) {
-      const int nb = std::min(TNB, n - p0);
-      g.p0 = p0;
-      g.nb = nb;
-      // reflector of the panel's first column (already updated by earlier panels)
-      hipLaunchKernelGGL(k_tri_col, dim3((n - p0 + COL_ROWS - 1) / COL_ROWS), dim3(256), 0, s, g, -1, p0, 0, 0);
-      W2K_STAGE(s, "k_tri_col");
-      int nref = 0;  // panel columns that carry a reflector
-      for (int k = 0; k < nb; k++) {
-        const int i = p0 + k;
-        if (i >= n - 1) break;
-        nref = k + 1;
-        const int rows = n - (i + 1);
-        const int base = (i + 1) & ~15;  // the SYMV's aligned tile origin
-        const int nt = (n - base + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
-        if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv], s));
-        hipLaunchKernelGGL(k_tri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
-        W2K_STAGE(s, "k_tri_symv");
-        if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv + 1], s));
-        nsymv++;
-        const int next = (k + 1 < nb) ? i + 1 : -1;
-        hipLaunchKernelGGL(k_tri_col, dim3((rows + COL_ROWS - 1) / COL_ROWS), dim3(256), 0, s, g, i, next, nt, nblk);
+    // every launch of the phase (no host read-back in between)
+    auto launches = [&]() -> int {
+      nsymv = 0;
+      for (int p0 = 0; p0 < n; p0 += TNB) {
+        const int nb = std::min(TNB, n - p0);
+        g.p0 = p0;
+        g.nb = nb;
+        // reflector of the panel's first column (already updated by earlier panels)
+        hipLaunchKernelGGL(k_tri_col, dim3((n - p0 + COL_ROWS - 1) / COL_ROWS), dim3(256), 0, s, g, -1, p0, 0, 0);
         W2K_STAGE(s, "k_tri_col");
+        int nref = 0;  // panel columns that carry a reflector
+        for (int k = 0; k < nb; k++) {
+          const int i = p0 + k;
+          if (i >= n - 1) break;
+          nref = k + 1;
+          const int rows = n - (i + 1);
+          const int base = (i + 1) & ~15;  // the SYMV's aligned tile origin
+          const int nt = (n - base + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
+          if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv], s));
+          hipLaunchKernelGGL(k_tri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
+          W2K_STAGE(s, "k_tri_symv");
+          if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv + 1], s));
+          nsymv++;
+          const int next = (k + 1 < nb) ? i + 1 : -1;
+          hipLaunchKernelGGL(k_tri_col, dim3((rows + COL_ROWS - 1) / COL_ROWS), dim3(256), 0, s, g, i, next, nt,
+                             nblk);
+          W2K_STAGE(s, "k_tri_col");
+        }
+        if (nref > 0) {
+          hipLaunchKernelGGL(k_tri_store, dim3((n + 255) / 256, nref), dim3(256), 0, s, g, nref);
+          W2K_STAGE(s, "k_tri_store");
+        }
+        const int p1 = p0 + nb;
+        if (p1 < n) {
+          // A(p1:n, p1:n) -= [V W] [W V]^T, lower triangle
+          const int r = n - p1;
+          double *A1 = A + (size_t)p1 + (size_t)lda * p1;
+          int r2;
+          if ((r2 = dgemm(s, 'N', 'T', r, r, 2 * nb, -1.0, P + p1, n, P + (size_t)n * nb + p1, n, 1.0, A1, lda, 1)))
+            return r2;
+        }
       }
-      if (nref > 0) {
-        hipLaunchKernelGGL(k_tri_store, dim3((n + 255) / 256, nref), dim3(256), 0, s, g, nref);
-        W2K_STAGE(s, "k_tri_store");
-      }
-      const int p1 = p0 + nb;
-      if (p1 < n) {
-        // A(p1:n, p1:n) -= [V W] [W V]^T, lower triangle
-        const int r = n - p1;
-        double *A1 = A + (size_t)p1 + (size_t)lda * p1;
-        if ((rc = dgemm(s, 'N', 'T', r, r, 2 * nb, -1.0, P + p1, n, P + (size_t)n * nb + p1, n, 1.0, A1, lda, 1)))
-          return rc;
-      }
-    }
+      return 0;
+    };
+    if ((rc = tri_run(c, s, n, lda, A, kt, launches))) return rc;
   }
   W2K_HIP(hipEventRecord(c.ev[9], s));
   if (c.ktime) {

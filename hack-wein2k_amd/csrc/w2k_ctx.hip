@@ -31,6 +31,7 @@ static int sync_checks()
   static const int on = getenv("W2K_SYNC_CHECK") != nullptr && atoi(getenv("W2K_SYNC_CHECK")) != 0;
   return on;
 }
+int sync_checks_enabled() { return sync_checks(); }
 
 int stage_check(hipStream_t s, const char *name, const char *file, int line)
 {
@@ -380,6 +381,14 @@ int w2k_set_kernel_timing(w2k_ctx *x, int on)
   return 0;
 }
 
+int w2k_set_graphs(w2k_ctx *x, int on)
+{
+  if (!x) return W2K_EINVAL;
+  x->c.graphs = on ? 1 : 0;
+  for (w2k_ctx *w : x->c.workers) w->c.graphs = x->c.graphs;
+  return 0;
+}
+
 int w2k_set_split_precision(w2k_ctx *x, int slices)
 {
   if (!x) return W2K_EINVAL;
@@ -445,6 +454,7 @@ void w2k_destroy(w2k_ctx *x)
   drop_workers(x->c);
   (void)hipSetDevice(x->c.device);
   if (x->c.stream) (void)hipStreamSynchronize(x->c.stream);
+  if (x->c.tri_graph.exec) (void)hipGraphExecDestroy(x->c.tri_graph.exec);
   for (int k = 0; k < W2K_NEVENTS; k++) (void)hipEventDestroy(x->c.ev[k]);
   for (hipEvent_t e : x->c.kev) (void)hipEventDestroy(e);
   if (x->c.stream) (void)hipStreamDestroy(x->c.stream);
@@ -585,7 +595,10 @@ int w2k_calkpt_batched(w2k_ctx *x, int nk, const w2k_basis *bases, int nohns, co
     w->c.runp = &c.own_run;
     c.workers.push_back(w);
   }
-  for (int w = 0; w < P; w++) c.workers[w]->c.hns_slices = c.hns_slices;
+  for (int w = 0; w < P; w++) {
+    c.workers[w]->c.hns_slices = c.hns_slices;
+    c.workers[w]->c.graphs = c.graphs;
+  }
   // longest first: k-points in decreasing N, each to the least loaded stream
   // (load ~ N^3); a stream's list is then in decreasing N, so its buffers
   // are sized by its first k-point

@@ -12,7 +12,7 @@
 #define W2K_LOMAXP (W2K_LOMAX + 1)
 #define W2K_NSL2 (W2K_NSLMAX * W2K_NSLMAX)
 #define W2K_NEVENTS 16
-#define W2K_NTIMINGS 12
+#define W2K_NTIMINGS 13
 
 namespace w2k {
 
@@ -28,6 +28,8 @@ struct Status {
   } while (0)
 
 int w2k_hip_fail(hipError_t e, const char *what, const char *file, int line);
+// W2K_SYNC_CHECK=1 in the environment: synchronise after every kernel
+int sync_checks_enabled();
 
 // launch check after kernel `name` (and a stream synchronisation when the
 // environment sets W2K_SYNC_CHECK=1)
@@ -141,6 +143,18 @@ struct Ctx {
   DBuf<double> w_bt;    // back-transform panel work
   DBuf<double> w_cl;    // cluster orthogonalisation work
   DBuf<double> w_gk;    // split-K GEMM partial products
+  // launch mode of the tridiagonalisation, chosen per shape by measurement:
+  // direct launches or one captured HIP graph replayed (seclr4.hip)
+  int graphs = 0;  // 1: capture and replay where it measures faster (off by default)
+  struct TriGraph {
+    int n = -1, lda = 0;  // key: order, leading dimension and the buffers the launches address
+    const void *a = nullptr, *small = nullptr, *ints = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int seen = 0;      // calls with this key
+    int mode = -1;     // last call: 0 direct, 1 replay, 2 capture and replay, -1 none
+    int use = 1;       // 0 once a replay was slower than direct launches (or capture failed)
+    double direct_ms = 0.0, graph_ms = 0.0;
+  } tri_graph;
   // timing
   hipEvent_t ev[W2K_NEVENTS];
   double last_ms[W2K_NTIMINGS] = {0};

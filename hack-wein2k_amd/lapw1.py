@@ -208,6 +208,7 @@ def load_library(path: Optional[str] = None):
     lib.w2k_last_error.restype = ctypes.c_char_p
     lib.w2k_set_kernel_timing.argtypes = [V, ctypes.c_int]
     lib.w2k_set_split_precision.argtypes = [V, ctypes.c_int]
+    lib.w2k_set_graphs.argtypes = [V, ctypes.c_int]
     lib.w2k_adjoint_warp.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p]
     lib.w2k_adjoint_vns.argtypes = [V, _c_dbl_p] + [_c_dbl_p] * 6
     lib.w2k_calkpt_batched.argtypes = [V, ctypes.c_int, ctypes.POINTER(_BasisDesc), ctypes.c_int, _c_dbl_p, _c_dbl_p,
@@ -366,11 +367,18 @@ class Lapw1:
         return ev[:ne], z[:, :ne]
 
     TIMING_NAMES = ("hamilt", "hns", "seclr4", "cholesky", "reduction", "tridiag", "bisect_invit",
-                    "backtransform", "hamilt_pw_kernel", "hns_gemm", "tri_symv_total", "tri_symv_launches")
+                    "backtransform", "hamilt_pw_kernel", "hns_gemm", "tri_symv_total", "tri_symv_launches",
+                    "tri_launch_mode")
 
     def set_kernel_timing(self, on: bool):
         """Per-launch events around the tridiagonalisation SYMV kernels."""
         self._chk("w2k_set_kernel_timing", self.lib.w2k_set_kernel_timing(self.ctx, int(bool(on))))
+
+    def set_graphs(self, on: bool):
+        """Tridiagonalisation launches captured into a HIP graph and replayed
+        where that measures faster (on), or always launched directly (off, the
+        default)."""
+        self._chk("w2k_set_graphs", self.lib.w2k_set_graphs(self.ctx, int(bool(on))))
 
     def set_split_precision(self, slices: int):
         """HNS rank-K update in split precision: 0 = FP64 (default), 3..7 = INT8

@@ -206,7 +206,9 @@ int w2k_sync(w2k_ctx *ctx);
  * [6] bisection + inverse iteration, [7] back-transforms, [8] the HAMILT
  * plane-wave pair kernel alone, [9] the HNS rank-K GEMM alone, [10] the
  * summed time of the tridiagonalisation SYMV kernels and [11] their number of
- * launches (both only while per-launch timing is on).  At most n values are
+ * launches (both only while per-launch timing is on), [12] the launch mode of
+ * the last tridiagonalisation (0 direct launches, 1 replay of a captured HIP
+ * graph, 2 capture and replay; see w2k_set_graphs).  At most n values are
  * written. */
 int w2k_last_timings(w2k_ctx *ctx, double *ms, int n);
 
@@ -214,6 +216,16 @@ int w2k_last_timings(w2k_ctx *ctx, double *ms, int n);
  * [10], [11] of w2k_last_timings); off by default since the events add a
  * little time between launches.  No counterpart in the reference. */
 int w2k_set_kernel_timing(w2k_ctx *ctx, int on);
+
+/* Launch mode of the tridiagonalisation (off by default): with on = 1 the
+ * library captures the phase's launches (two kernels per column and one GEMM
+ * per 32-column panel, no host read-back) into a HIP graph on the second
+ * call with the same order and buffers, replays it on the third, and keeps
+ * replaying only if that replay measured faster than the direct launches;
+ * on = 0 always launches directly.  Results are the same either way.  The
+ * setting is passed on to the streams of w2k_calkpt_batched.  No counterpart
+ * in the reference. */
+int w2k_set_graphs(w2k_ctx *ctx, int on);
 
 /* Split precision for the HNS rank-K update (hns.F:443-488, which the
  * reference does in FP64 with DSYR2M / ZHER2M): slices = 0 keeps the FP64

@@ -1,0 +1,67 @@
+"""Launch mode of the tridiagonalisation (w2k_set_graphs): an extended form
+with no counterpart in the reference.  With graphs on, the phase's launches
+are captured into a HIP graph on the second call with the same order and
+buffers and replayed on the third; the results must be bit-identical to
+direct launches across a capture, replays, a change of basis (a new capture)
+and a return to the first basis."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import Fixture, PKG
+
+pytestmark = pytest.mark.gpu
+L = PKG.lapw1
+MODE = L.Lapw1.TIMING_NAMES.index("tri_launch_mode")
+
+
+def _run(ctx, fx, q):
+    kd = fx.k(q)
+    ctx.set_basis(fx.basis(q))
+    ne, nb = ctx.calkpt(float(kd["EL"]), float(kd["EU"]), int(kd["NUME"]))
+    ev, z = ctx.get_eig()
+    return ne, nb, ev, z, int(ctx.timings()[MODE])
+
+
+# (MgO: one k-point repeated; a basis that grows within one context with
+# local orbitals is the case kept off the device, see test_gpu_seclr4.py)
+@pytest.mark.parametrize("name,nk", [("mgo", 1), ("rs8", 1), ("c1_fcc", 2)])
+def test_graph_replay_matches_direct_launches(name, nk):
+    fx = Fixture(name)
+    direct = {}
+    for q in range(nk):  # a fresh context per k-point, direct launches
+        ref = L.Lapw1(fx.run())
+        ref.set_graphs(False)
+        direct[q] = _run(ref, fx, q)
+        ref.close()
+    assert all(r[4] == 0 for r in direct.values())
+    ctx = L.Lapw1(fx.run())
+    ctx.set_graphs(True)
+    seq = [0, 0, 0, 0]
+    if nk > 1:
+        seq += [1, 1, 1, 0, 0]
+    modes = []
+    for q in seq:
+        ne, nb, ev, z, mode = _run(ctx, fx, q)
+        modes.append(mode)
+        d = direct[q]
+        assert (ne, nb) == (d[0], d[1])
+        assert np.array_equal(ev, d[2]), f"k{q} mode {mode}: eigenvalues differ from direct launches"
+        assert np.array_equal(z, d[3]), f"k{q} mode {mode}: eigenvectors differ from direct launches"
+        if ne:
+            assert np.max(np.abs(ev - fx.k(q)["EIGVAL"])) < 1e-10
+    ctx.close()
+    print(f"{name}: launch modes {modes}")
+    # first call direct, second captures, third replays; the fourth replays
+    # only if that replay measured faster than the direct launches
+    assert modes[:3] == [0, 2, 1]
+    assert modes[3] in (0, 1)
+
+
+def test_graphs_off_stays_direct():
+    fx = Fixture("c1_fcc")
+    ctx = L.Lapw1(fx.run())  # off by default
+    for _ in range(3):
+        assert _run(ctx, fx, 0)[4] == 0
+    ctx.close()
