@@ -13,6 +13,9 @@
 // bank-conflict free; LDS double-buffered so the next k-slab's global loads
 // overlap the current slab's MFMAs.
 #include <algorithm>
+#include <cstdlib>
+#include <map>
+#include <mutex>
 
 #include "w2k_internal.h"
 
@@ -300,25 +303,49 @@ __global__ void k_splitk_reduce(int m, int n, int nz, const double *__restrict__
   *cp = v;
 }
 
-int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
-          const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri, double *ws,
-          size_t wsn)
-{
-  if (m <= 0 || n <= 0) return 0;
-  GemmArgs g;
-  g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
-  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
-  g.ta = (ta == 'T' || ta == 't'); g.tb = (tb == 'T' || tb == 't');
-  g.tri = tri; g.atri = atri; g.btri = btri;
-  g.kslice = 0; g.ws = nullptr;
-  dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
-  if (k <= 0) {
-    g.k = 0;  // still applies beta
+// ------------------------------------------------------------------ tile choice
+// Products that both tile kernels can serve (at least 128 x 128 outputs and
+// K >= 64) get their kernel from a per-device table keyed by the shape class
+// (transposes, structure flags, floor(log2) of m, n and k): the first call of
+// a class runs the heuristic's kernel between two events, the next call of
+// the class the other kernel, and once both durations are in (read with
+// hipEventQuery, never waited for) the faster one is kept.  The two kernels
+// accumulate every product in the same order (MFMA steps of four along k,
+// zero products only added for elements outside a triangle), so their
+// results are bit-identical and the choice never changes a result.
+// W2K_GEMM_TUNE=0 keeps the heuristic (128x128 tiles for outputs of at least
+// 256 x 256 with K >= 256).
+namespace {
+struct TuneKey {
+  int dev, ta, tb, tri, atri, btri, lm, ln, lk;
+  bool operator<(const TuneKey &o) const
+  {
+    const int a[9] = {dev, ta, tb, tri, atri, btri, lm, ln, lk};
+    const int b[9] = {o.dev, o.ta, o.tb, o.tri, o.atri, o.btri, o.lm, o.ln, o.lk};
+    return std::lexicographical_compare(a, a + 9, b, b + 9);
   }
-  // large outputs with a long K: the 128x128 tile kernel (a short K, such as
-  // a rank-64 update, runs faster on the 64x64 tiles)
-  if (m >= 256 && n >= 256 && k >= 256) {
-    dim3 gb((m + LM - 1) / LM, (n + LN - 1) / LN);
+};
+struct TuneEntry {
+  int choice = -1;
+  float ms[2] = {-1.f, -1.f};
+  int pending[2] = {0, 0};
+  hipEvent_t ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+};
+std::mutex tune_mu;
+std::map<TuneKey, TuneEntry> tune_table;
+int ilog2(int x) { return 31 - __builtin_clz((unsigned)x); }
+bool tuning_on()
+{
+  static const bool on = !(getenv("W2K_GEMM_TUNE") && atoi(getenv("W2K_GEMM_TUNE")) == 0);
+  return on;
+}
+}  // namespace
+
+// variant 0: 64x64 tiles, 1: 128x128 tiles
+static int launch_tiles(hipStream_t s, const GemmArgs &g, int variant)
+{
+  if (variant == 1) {
+    dim3 gb((g.m + LM - 1) / LM, (g.n + LN - 1) / LN);
     if (!g.ta && !g.tb) hipLaunchKernelGGL((k_dgemm_big<0, 0>), gb, dim3(256), 0, s, g);
     else if (g.ta && !g.tb) hipLaunchKernelGGL((k_dgemm_big<1, 0>), gb, dim3(256), 0, s, g);
     else if (!g.ta && g.tb) hipLaunchKernelGGL((k_dgemm_big<0, 1>), gb, dim3(256), 0, s, g);
@@ -326,8 +353,41 @@ int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
     W2K_STAGE(s, "k_dgemm_big");
     return 0;
   }
+  dim3 grid((g.m + BM - 1) / BM, (g.n + BN - 1) / BN);
+  hipLaunchKernelGGL(k_dgemm<false>, grid, dim3(256), 0, s, g);
+  W2K_STAGE(s, "k_dgemm");
+  return 0;
+}
+
+static GemmArgs gemm_args(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+                          const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri)
+{
+  GemmArgs g;
+  g.m = m; g.n = n; g.k = k > 0 ? k : 0; g.alpha = alpha; g.beta = beta;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.ta = (ta == 'T' || ta == 't'); g.tb = (tb == 'T' || tb == 't');
+  g.tri = tri; g.atri = atri; g.btri = btri;
+  g.kslice = 0; g.ws = nullptr;
+  return g;
+}
+
+int dgemm_tiles(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+                const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri, int variant)
+{
+  if (m <= 0 || n <= 0) return 0;
+  const GemmArgs g = gemm_args(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, atri, btri);
+  return launch_tiles(s, g, (variant == 1 && g.k > 0) ? 1 : 0);
+}
+
+int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+          const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri, double *ws,
+          size_t wsn)
+{
+  if (m <= 0 || n <= 0) return 0;
+  GemmArgs g = gemm_args(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, atri, btri);
   // few output tiles and a long K: split K over z-slices so that the whole
   // chip works, then reduce the slices in a fixed order
+  dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
   const long tiles = (long)grid.x * grid.y;
   if (ws && tri == 0 && tiles < 256 && k >= 4 * 256) {
     int nz = (int)std::min<long>((k + 255) / 256, std::max<long>(1, 1024 / tiles));
@@ -345,9 +405,43 @@ int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
       return 0;
     }
   }
-  hipLaunchKernelGGL(k_dgemm<false>, grid, dim3(256), 0, s, g);
-  W2K_STAGE(s, "k_dgemm");
-  return 0;
+  int variant = (m >= 256 && n >= 256 && k >= 256) ? 1 : 0;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (m >= 128 && n >= 128 && k >= 64 && tuning_on() && hipStreamIsCapturing(s, &cap) == hipSuccess &&
+      cap == hipStreamCaptureStatusNone) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const TuneKey key{dev, g.ta, g.tb, tri, atri, btri, ilog2(m), ilog2(n), ilog2(k)};
+    std::unique_lock<std::mutex> lk(tune_mu);
+    TuneEntry &e = tune_table[key];
+    for (int v = 0; v < 2; v++) {
+      if (e.pending[v] && hipEventQuery(e.ev[v][1]) == hipSuccess) {
+        float ms = -1.f;
+        if (hipEventElapsedTime(&ms, e.ev[v][0], e.ev[v][1]) == hipSuccess) e.ms[v] = ms;
+        e.pending[v] = 0;
+      }
+    }
+    (void)hipGetLastError();  // a query of an unfinished event is not an error here
+    if (e.choice < 0 && e.ms[0] >= 0.f && e.ms[1] >= 0.f) e.choice = e.ms[1] < e.ms[0] ? 1 : 0;
+    if (e.choice >= 0) {
+      variant = e.choice;
+    } else {
+      int v = -1;
+      for (int u : {variant, 1 - variant})
+        if (v < 0 && e.ms[u] < 0.f && !e.pending[u]) v = u;
+      if (v >= 0) {
+        for (int q = 0; q < 2; q++)
+          if (!e.ev[v][q]) W2K_HIP(hipEventCreate(&e.ev[v][q]));
+        W2K_HIP(hipEventRecord(e.ev[v][0], s));
+        int rc;
+        if ((rc = launch_tiles(s, g, v))) return rc;
+        W2K_HIP(hipEventRecord(e.ev[v][1], s));
+        e.pending[v] = 1;
+        return 0;
+      }
+    }
+  }
+  return launch_tiles(s, g, g.k > 0 ? variant : 0);
 }
 
 }  // namespace w2k

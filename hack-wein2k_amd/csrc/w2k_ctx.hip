@@ -797,7 +797,8 @@ int w2k_last_timings(w2k_ctx *x, double *ms, int n)
 }  // extern "C"
 
 static int test_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda, const double *B,
-                      int ldb, double beta, double *C, int ldc, int tri, int atri, int btri, size_t wsn)
+                      int ldb, double beta, double *C, int ldc, int tri, int atri, int btri, size_t wsn,
+                      int variant = -1)
 {
   if (hipSetDevice(0) != hipSuccess) return W2K_ENODEV;
   const size_t na = (size_t)lda * ((ta == 'N' || ta == 'n') ? k : m);
@@ -810,8 +811,11 @@ static int test_dgemm(char ta, char tb, int m, int n, int k, double alpha, const
   W2K_HIP(hipMemcpy(dA.p, A, sizeof(double) * na, hipMemcpyHostToDevice));
   W2K_HIP(hipMemcpy(dB.p, B, sizeof(double) * nb, hipMemcpyHostToDevice));
   W2K_HIP(hipMemcpy(dC.p, C, sizeof(double) * nc, hipMemcpyHostToDevice));
-  rc = dgemm(nullptr, ta, tb, m, n, k, alpha, dA.p, lda, dB.p, ldb, beta, dC.p, ldc, tri, atri, btri,
-             wsn ? dW.p : nullptr, wsn);
+  if (variant >= 0)
+    rc = dgemm_tiles(nullptr, ta, tb, m, n, k, alpha, dA.p, lda, dB.p, ldb, beta, dC.p, ldc, tri, atri, btri, variant);
+  else
+    rc = dgemm(nullptr, ta, tb, m, n, k, alpha, dA.p, lda, dB.p, ldb, beta, dC.p, ldc, tri, atri, btri,
+               wsn ? dW.p : nullptr, wsn);
   if (rc) return rc;
   W2K_HIP(hipDeviceSynchronize());
   W2K_HIP(hipMemcpy(C, dC.p, sizeof(double) * nc, hipMemcpyDeviceToHost));
@@ -822,6 +826,14 @@ extern "C" int w2k_test_dgemm(char ta, char tb, int m, int n, int k, double alph
                               const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri)
 {
   return test_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, atri, btri, 0);
+}
+
+extern "C" int w2k_test_dgemm_tiles(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+                                    const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri,
+                                    int btri, int variant)
+{
+  if (variant != 0 && variant != 1) return W2K_EINVAL;
+  return test_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, atri, btri, 0, variant);
 }
 
 extern "C" int w2k_test_dgemm_ws(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,

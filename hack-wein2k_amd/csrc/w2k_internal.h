@@ -195,6 +195,9 @@ int split_gemm_nt(hipStream_t s, int m, int n, int k, const double *A, int lda, 
 int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
           const double *B, int ldb, double beta, double *C, int ldc, int tri = 0, int atri = 0, int btri = 0,
           double *ws = nullptr, size_t wsn = 0);
+// the same product on a given tile kernel (0: 64x64, 1: 128x128; no split K)
+int dgemm_tiles(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+                const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri, int variant);
 
 }  // namespace w2k
 

@@ -16,6 +16,13 @@ extern "C" {
 int w2k_test_dgemm(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
                    const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri);
 
+/* The same product on a given tile kernel (variant 0: 64x64 tiles, 1:
+ * 128x128 tiles), the two choices of the library's GEMM tile autotuner,
+ * which must give bit-identical results. */
+int w2k_test_dgemm_tiles(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
+                         const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri,
+                         int variant);
+
 /* The same dense product with a device workspace of wsn doubles, which lets
  * a product with few output tiles and a long K split K across the chip. */
 int w2k_test_dgemm_ws(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,

@@ -101,6 +101,59 @@ def test_dgemm_triangular_operands_large(ta, tb, atri, btri):
     assert np.max(np.abs(out - Am @ Bm)) < 1e-11
 
 
+def _dgemm_tiles(ta, tb, m, n, k, alpha, A, B, beta, C, tri, atri, btri, variant):
+    lib = L.load_library()
+    fn = lib.w2k_test_dgemm_tiles
+    dp = ctypes.POINTER(ctypes.c_double)
+    fn.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                   dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    A = np.asfortranarray(A); B = np.asfortranarray(B); C = np.asfortranarray(C.copy())
+    rc = fn(ta.encode(), tb.encode(), m, n, k, alpha, A.ctypes.data_as(dp), A.shape[0], B.ctypes.data_as(dp),
+            B.shape[0], beta, C.ctypes.data_as(dp), C.shape[0], tri, atri, btri, variant)
+    assert rc == 0
+    return C
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k,tri,atri,btri", [(300, 257, 64, 0, 0, 0), (389, 389, 300, 1, 0, 0),
+                                                 (389, 389, 200, 2, 0, 0), (301, 301, 301, 0, 1, 0),
+                                                 (301, 301, 301, 0, 2, 0), (301, 301, 301, 0, 0, 1),
+                                                 (301, 301, 301, 0, 0, 2), (130, 700, 999, 0, 0, 0)])
+def test_dgemm_tile_kernels_bit_identical(ta, tb, m, n, k, tri, atri, btri):
+    """The GEMM tile autotuner switches between the 64x64 and the 128x128 tile
+    kernels by measured time; both accumulate in the same order, so their
+    results must agree bit for bit (triangular outputs and operands
+    included)."""
+    rng = np.random.default_rng(m + 7 * n + 13 * k + tri + 3 * atri + 5 * btri)
+    A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+    C = rng.standard_normal((m, n))
+    c0 = _dgemm_tiles(ta, tb, m, n, k, 0.7, A, B, -1.3, C, tri, atri, btri, 0)
+    c1 = _dgemm_tiles(ta, tb, m, n, k, 0.7, A, B, -1.3, C, tri, atri, btri, 1)
+    assert np.array_equal(c0, c1)
+    opA = A if ta == "N" else A.T
+    opB = B if tb == "N" else B.T
+    Am = np.triu(opA) if atri == 1 else (np.tril(opA) if atri == 2 else opA)
+    Bm = np.triu(opB) if btri == 1 else (np.tril(opB) if btri == 2 else opB)
+    ref = 0.7 * Am @ Bm - 1.3 * C
+    mask = np.ones((m, n), bool) if tri == 0 else (np.tril(np.ones((m, n), bool)) if tri == 1
+                                                   else np.triu(np.ones((m, n), bool)))
+    assert np.max(np.abs(c1[mask] - ref[mask])) < 1e-11 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_dgemm_autotuned_calls_match():
+    """Repeated calls of one shape class while the autotuner measures both
+    kernels and then keeps one: every call gives the same bits."""
+    rng = np.random.default_rng(77)
+    m, n, k = 520, 530, 300
+    A = rng.standard_normal((m, k)); B = rng.standard_normal((k, n)); C = rng.standard_normal((m, n))
+    outs = [_dgemm("N", "N", m, n, k, 1.0, A, B, 0.5, C) for _ in range(5)]
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    assert np.max(np.abs(outs[0] - (A @ B + 0.5 * C))) < 1e-10
+
+
 @pytest.mark.parametrize("tri", [1, 2])
 def test_dgemm_triangular_output(tri):
     rng = np.random.default_rng(5)
