@@ -209,111 +209,210 @@ struct ZTriWork {
   int *counter;
 };
 
-__global__ __launch_bounds__(64) void k_ztri_symv(ZTriWork g, int i, int nt)
+// ZLARFG of column i on (A(i+1,i), A(i+2:n,i)) from the squared-norm
+// partials of the column kernel (one per block of 256 rows), evaluated by
+// every SYMV wave with the same fixed-order arithmetic (so no grid-wide step
+// separates the two kernels); the wave with store set writes E, TAU and the
+// scale 1/(alpha - beta) of v.
+__device__ __forceinline__ void ztri_reflector(const ZTriWork &g, int i, int lane, bool store, double &scr,
+                                               double &sci)
+{
+  const int n = g.n;
+  const int nnorm = (n - i + 255) / 256;
+  double x[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) x[u] = g.norm[min(lane + 64 * u, nnorm - 1)];
+  const double alr = g.ar[(size_t)n * i + i + 1], ali = g.ai[(size_t)n * i + i + 1];
+  double xs = 0.0;
+#pragma unroll
+  for (int u = 0; u < 4; u++) xs += (lane + 64 * u < nnorm) ? x[u] : 0.0;
+  for (int b = lane + 256; b < nnorm; b += 64) xs += g.norm[b];
+  xs = wave_sum(xs);
+  xs = __shfl(xs, 0, 64);
+  const double xnorm = sqrt(xs);
+  double taur = 0.0, taui = 0.0, beta = alr;
+  scr = 0.0;
+  sci = 0.0;
+  if (!(xnorm == 0.0 && ali == 0.0)) {
+    beta = -copysign(sqrt(alr * alr + ali * ali + xnorm * xnorm), alr);
+    taur = (beta - alr) / beta;
+    taui = -ali / beta;
+    const double zr = alr - beta, zi = ali, den = zr * zr + zi * zi;
+    scr = zr / den;
+    sci = -zi / den;
+  }
+  if (store && lane == 0) {
+    g.e[i] = beta;
+    g.taur[i] = taur;
+    g.taui[i] = taui;
+    g.scr[i] = scr;
+    g.sci[i] = sci;
+  }
+}
+
+// y = A(r0:n, r0:n) v for Hermitian A from the tiles on and below the
+// diagonal (one wave per 64 x 64 tile (I, J), J <= I): A_IJ v_J to the rows
+// of tile I and, for J < I, A_IJ^H v_I to the rows of tile J.  A tile is
+// taken in four quarters of 16 columns: the real and imaginary parts of a
+// quarter-row fill 32 of the lane's registers, row sums stay in the lane and
+// column sums use a wave-wide reduce-scatter (no LDS); v(C0 + j) reaches the
+// row sums from lane j by readlane.  The nt waves after the tiles each
+// take one row block: they write V(:,k) and the partial dot products V^H v,
+// W^H v of the block's rows.
+__device__ __forceinline__ double dbl_readlane(double v, int src)
+{
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), src);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// t[0..31] per lane -> returns, in lane l, the sum over the 64 lanes of
+// t[l % 32] (five halving steps within each 32-lane half, then the halves
+// added).  Overwrites t.
+__device__ __forceinline__ double wave_transpose_sum32(double *t, int lane)
+{
+  bfly_step<16>(t, lane);
+  bfly_step<8>(t, lane);
+  bfly_step<4>(t, lane);
+  bfly_step<2>(t, lane);
+  bfly_step<1>(t, lane);
+  return t[0] + __shfl_xor(t[0], 32, 64);
+}
+
+__global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
 {
   const int n = g.n, r0 = i + 1, ii = i - g.p0, nb = g.nb;
   const int lane = threadIdx.x;
+  const int nblk = nt * (nt + 1) / 2;
   const int bid = blockIdx.x;
+  const double *air = g.ar + (size_t)n * i, *aii = g.ai + (size_t)n * i;
+  double sr, si;
+  ztri_reflector(g, i, lane, bid == 0, sr, si);
+  // v(x): 1 at r0, A(x, i) (sr + i si) below it, 0 outside [r0, n)
+  auto vof = [&](int x, double &vr, double &vi) {
+    const int xc = min(max(x, r0), n - 1);
+    const double ar = air[xc], ai = aii[xc];
+    vr = 0.0;
+    vi = 0.0;
+    if (x == r0) {
+      vr = 1.0;
+    } else if (x > r0 && x < n) {
+      vr = ar * sr - ai * si;
+      vi = ar * si + ai * sr;
+    }
+  };
+  double t[SYMV_T];
+  if (bid >= nblk) {
+    const int I = bid - nblk;
+    const int R = r0 + SYMV_T * I + lane;
+    const bool live = R < n;
+    const int Rs = live ? R : r0;
+    double vrr, vri;
+    vof(R, vrr, vri);
+    double *Vr = g.Pr + (size_t)n * ii, *Vi = g.Pi + (size_t)n * ii;
+    double *V2r = g.Pr + (size_t)n * (2 * nb + ii), *V2i = g.Pi + (size_t)n * (2 * nb + ii);
+    if (live) { Vr[R] = vrr; Vi[R] = vri; V2r[R] = vrr; V2i[R] = vri; }
+    for (int q = I * SYMV_T + lane; q < r0; q += nt * SYMV_T) { Vr[q] = 0.0; Vi[q] = 0.0; V2r[q] = 0.0; V2i[q] = 0.0; }
+    if (ii > 0) {
+      double *dr = g.dotr + (size_t)I * 2 * TNB, *di = g.doti + (size_t)I * 2 * TNB;
+#pragma unroll
+      for (int half = 0; half < 2; half++) {  // 0: V, 1: W
+        const double *Xr = g.Pr + (size_t)n * (half * nb), *Xi = g.Pi + (size_t)n * (half * nb);
+#pragma unroll
+        for (int k = 0; k < TNB; k++) {
+          t[k] = Xr[(size_t)n * k + Rs];
+          t[TNB + k] = Xi[(size_t)n * k + Rs];
+        }
+#pragma unroll
+        for (int k = 0; k < TNB; k++) {  // conj(X(R,k)) v(R)
+          const bool use = live && k < ii;
+          const double ur = t[k], ui = t[TNB + k];
+          t[k] = use ? ur * vrr + ui * vri : 0.0;
+          t[TNB + k] = use ? ur * vri - ui * vrr : 0.0;
+        }
+        const double sq = wave_transpose_sum(t, lane);  // lane l < 32: Re of column l, 32 + l: Im
+        if (lane < TNB) dr[half * TNB + lane] = sq;
+        else di[half * TNB + lane - TNB] = sq;
+      }
+    }
+    return;
+  }
   int I = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
   while ((I + 1) * (I + 2) / 2 <= bid) I++;
   while (I * (I + 1) / 2 > bid) I--;
   const int J = bid - I * (I + 1) / 2;
   const int R = r0 + SYMV_T * I + lane;
   const int C0 = r0 + SYMV_T * J;
-  const double sr = g.scr[i], si = g.sci[i];
-  const double *air = g.ar + (size_t)n * i, *aii = g.ai + (size_t)n * i;
-  __shared__ double vsr[SYMV_T], vsi[SYMV_T];
-  __shared__ double br[SYMV_T][SYMV_T + 1], bi[SYMV_T][SYMV_T + 1];
-  {
-    const int c = C0 + lane;
-    double xr = 0.0, xi = 0.0;
-    if (c < n) {
-      if (c == r0) {
-        xr = 1.0;
-      } else {
-        xr = air[c] * sr - aii[c] * si;
-        xi = air[c] * si + aii[c] * sr;
-      }
-    }
-    vsr[lane] = xr;
-    vsi[lane] = xi;
-  }
   const bool live = R < n;
-  double vrr = 0.0, vri = 0.0;
-  if (live) {
-    if (R == r0) {
-      vrr = 1.0;
-    } else {
-      vrr = air[R] * sr - aii[R] * si;
-      vri = air[R] * si + aii[R] * sr;
-    }
-  }
-  __syncthreads();
   const bool off = (I != J);
   const int m = min(SYMV_T, n - C0);
-  double yr = 0.0, yi = 0.0;
-  if (live) {
-    const double *apr = g.ar + (size_t)R + (size_t)n * C0, *api = g.ai + (size_t)R + (size_t)n * C0;
-    for (int j = 0; j < m; j++) {
-      const double xr = apr[(size_t)n * j], xi = api[(size_t)n * j];
-      cmad(xr, xi, vsr[j], vsi[j], yr, yi);
-      if (off) {  // conj(A(R,c)) v(R)
-        br[j][lane] = xr * vrr + xi * vri;
-        bi[j][lane] = xr * vri - xi * vrr;
+  const int Rl = min(R, n - 1);
+  double vcr, vci, vrr, vri;
+  vof(C0 + lane, vcr, vci);  // v of the lane's column
+  vof(R, vrr, vri);          // v of the lane's row
+  double yr = 0.0, yi = 0.0, vyc = 0.0;
+  const double *apr = g.ar + (size_t)Rl + (size_t)n * C0, *api = g.ai + (size_t)Rl + (size_t)n * C0;
+#pragma unroll 1
+  for (int c0 = 0; c0 < m; c0 += 16) {  // four quarters of 16 columns (wave-uniform)
+    double q[32];  // q[j] = Re A(R, C0 + c0 + j), q[16 + j] = Im
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const int jj = min(c0 + j, m - 1);
+      q[j] = apr[(size_t)n * jj];
+      q[16 + j] = api[(size_t)n * jj];
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const bool ok = live && c0 + j < m;
+      q[j] = ok ? q[j] : 0.0;
+      q[16 + j] = ok ? q[16 + j] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) cmad(q[j], q[16 + j], dbl_readlane(vcr, c0 + j), dbl_readlane(vci, c0 + j), yr, yi);
+    if (off) {
+#pragma unroll
+      for (int j = 0; j < 16; j++) {  // conj(A(R,c)) v(R)
+        const double ar = q[j], ai = q[16 + j];
+        q[j] = ar * vrr + ai * vri;
+        q[16 + j] = ar * vri - ai * vrr;
       }
+      // lane l: entry l % 32 summed over the wave (Re of column l % 16 for
+      // l % 32 < 16, Im of column l % 16 above)
+      const double cs = wave_transpose_sum32(q, lane);
+      const double co = __shfl_xor(cs, 16, 64);
+      const int cl = c0 + (lane & 15);
+      if (lane < 32 && cl < m) ((lane & 16) ? g.parti : g.partr)[(size_t)I * n + C0 + cl] = cs;
+      const double vcl_r = __shfl(vcr, cl, 64), vcl_i = __shfl(vci, cl, 64);
+      if (lane < 16 && cl < m) vyc += vcl_r * cs + vcl_i * co;  // Re(conj(v_c) (A^H v)_c)
     }
   }
-  if (off && !live)
-    for (int j = 0; j < m; j++) { br[j][lane] = 0.0; bi[j][lane] = 0.0; }
   if (live) {
     g.partr[(size_t)J * n + R] = yr;
     g.parti[(size_t)J * n + R] = yi;
   }
-  double vy = vrr * yr + vri * yi;  // Re(conj(v) y)
-  if (off) {
-    __syncthreads();
-    if (lane < m) {
-      double cr = 0.0, ci = 0.0;
-      for (int rr = 0; rr < SYMV_T; rr++) { cr += br[lane][rr]; ci += bi[lane][rr]; }
-      g.partr[(size_t)I * n + C0 + lane] = cr;
-      g.parti[(size_t)I * n + C0 + lane] = ci;
-      vy += vsr[lane] * cr + vsi[lane] * ci;
-    }
-  }
+  double vy = vrr * yr + vri * yi + vyc;  // Re(conj(v) y)
   vy = wave_sum(vy);
   if (lane == 0) g.vy[bid] = vy;
-  if (J == 0) {
-    double *Vr = g.Pr + (size_t)n * ii, *Vi = g.Pi + (size_t)n * ii;
-    double *V2r = g.Pr + (size_t)n * (2 * nb + ii), *V2i = g.Pi + (size_t)n * (2 * nb + ii);
-    if (live) { Vr[R] = vrr; Vi[R] = vri; V2r[R] = vrr; V2i[R] = vri; }
-    for (int q = I * SYMV_T + lane; q < r0; q += nt * SYMV_T) { Vr[q] = 0.0; Vi[q] = 0.0; V2r[q] = 0.0; V2i[q] = 0.0; }
-    double *dr = g.dotr + (size_t)I * 2 * TNB, *di = g.doti + (size_t)I * 2 * TNB;
-    for (int k = 0; k < ii; k++) {
-      double pr = 0.0, pi = 0.0, qr = 0.0, qi = 0.0;
-      if (live) {
-        // conj(V(R,k)) v(R) and conj(W(R,k)) v(R)
-        const double ur = g.Pr[(size_t)n * k + R], ui = g.Pi[(size_t)n * k + R];
-        const double wr = g.Pr[(size_t)n * (nb + k) + R], wi = g.Pi[(size_t)n * (nb + k) + R];
-        pr = ur * vrr + ui * vri; pi = ur * vri - ui * vrr;
-        qr = wr * vrr + wi * vri; qi = wr * vri - wi * vrr;
-      }
-      pr = wave_sum(pr); pi = wave_sum(pi); qr = wave_sum(qr); qi = wave_sum(qi);
-      if (lane == 0) { dr[k] = pr; di[k] = pi; dr[TNB + k] = qr; di[TNB + k] = qi; }
-    }
-  }
 }
 
-// finish W(:, fin - p0) and update + reflect column col (ZLATRD / ZLARFG)
+// Finish W(:, fin - p0) (fin >= 0) and update column col (col >= 0) with the
+// panel so far (ZLATRD); write the squared-norm partial of its
+// sub-subdiagonal part per block (ZLARFG itself is evaluated by the next
+// SYMV's waves).  One thread per row; every runtime-length sum reads its
+// operands in batches of ZB (clamped indices, masked values) so that the
+// loads of a batch are in flight together.
+constexpr int ZB = 8;
 __global__ __launch_bounds__(256) void k_ztri_col(ZTriWork g, int fin, int col, int ntile, int nch, int nvy)
 {
   const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x;
   const int rbase = fin >= 0 ? fin + 1 : col;
   const int r = rbase + blockIdx.x * 256 + t;
   const bool live = r < n;
+  const int rs = live ? r : rbase;
   __shared__ double abr[2 * TNB], abi[2 * TNB];  // a_k = V_k^H v, b_k = W_k^H v
   __shared__ double wcr[TNB], wci[TNB], vcr[TNB], vci[TNB];
   __shared__ double sh[4], red[4], red2[4];
-  __shared__ int last;
   const int ii = fin >= 0 ? fin - p0 : -1;
   double wrr = 0.0, wri = 0.0;
   double tr = 0.0, ti = 0.0, alpha2 = 0.0;
@@ -321,10 +420,25 @@ __global__ __launch_bounds__(256) void k_ztri_col(ZTriWork g, int fin, int col, 
   if (fin >= 0) {
     tr = g.taur[fin];
     ti = g.taui[fin];
-    for (int q = t; q < 2 * TNB; q += blockDim.x) {
+    if (t < 2 * TNB) {
+      const int q = t;
       double sr = 0.0, si = 0.0;
-      if ((q < TNB && q < ii) || (q >= TNB && q - TNB < ii))
-        for (int b = 0; b < ntile; b++) { sr += g.dotr[(size_t)b * 2 * TNB + q]; si += g.doti[(size_t)b * 2 * TNB + q]; }
+      if ((q & (TNB - 1)) < ii) {
+        for (int b0 = 0; b0 < ntile; b0 += ZB) {
+          double xr[ZB], xi[ZB];
+#pragma unroll
+          for (int u = 0; u < ZB; u++) {
+            const size_t o = (size_t)min(b0 + u, ntile - 1) * 2 * TNB + q;
+            xr[u] = g.dotr[o];
+            xi[u] = g.doti[o];
+          }
+#pragma unroll
+          for (int u = 0; u < ZB; u++) {
+            sr += (b0 + u < ntile) ? xr[u] : 0.0;
+            si += (b0 + u < ntile) ? xi[u] : 0.0;
+          }
+        }
+      }
       abr[q] = sr;
       abi[q] = si;
     }
@@ -334,6 +448,28 @@ __global__ __launch_bounds__(256) void k_ztri_col(ZTriWork g, int fin, int col, 
       s = wave_sum(s);
       if ((t & 63) == 0) red[t >> 6] = s;
     }
+    // y(r) from the SYMV partials, minus the panel terms
+    double yr = 0.0, yi = 0.0;
+    for (int c0 = 0; c0 < nch; c0 += ZB) {
+      double xr[ZB], xi[ZB];
+#pragma unroll
+      for (int u = 0; u < ZB; u++) {
+        const size_t o = (size_t)min(c0 + u, nch - 1) * n + rs;
+        xr[u] = g.partr[o];
+        xi[u] = g.parti[o];
+      }
+#pragma unroll
+      for (int u = 0; u < ZB; u++) {
+        yr += (c0 + u < nch) ? xr[u] : 0.0;
+        yi += (c0 + u < nch) ? xi[u] : 0.0;
+      }
+    }
+    double vkr[TNB], vki[TNB], wkr[TNB], wki[TNB];  // V(r,k), W(r,k), k < ii
+#pragma unroll
+    for (int k = 0; k < TNB; k++) {
+      const size_t o = (size_t)n * k + rs;
+      vkr[k] = Vr[o]; vki[k] = Vi[o]; wkr[k] = Wr[o]; wki[k] = Wi[o];
+    }
     __syncthreads();
     if (t == 0) sh[0] = (red[0] + red[1]) + (red[2] + red[3]);
     __syncthreads();
@@ -342,16 +478,15 @@ __global__ __launch_bounds__(256) void k_ztri_col(ZTriWork g, int fin, int col, 
     for (int k = 0; k < ii; k++) ssum += abr[k] * abr[TNB + k] + abi[k] * abi[TNB + k];
     alpha2 = -0.5 * (tr * tr + ti * ti) * (sh[0] - 2.0 * ssum);
     double *Wir = g.Pr + (size_t)n * (nb + ii), *Wii = g.Pi + (size_t)n * (nb + ii);
-    if (live) {
-      double yr = 0.0, yi = 0.0;
-      for (int c = 0; c < nch; c++) { yr += g.partr[(size_t)c * n + r]; yi += g.parti[(size_t)c * n + r]; }
-      for (int k = 0; k < ii; k++) {
+#pragma unroll
+    for (int k = 0; k < TNB; k++) {
+      if (k < ii) {
         // y -= W(r,k) a_k + V(r,k) b_k
-        const double w_r = Wr[(size_t)n * k + r], w_i = Wi[(size_t)n * k + r];
-        const double v_r = Vr[(size_t)n * k + r], v_i = Vi[(size_t)n * k + r];
-        yr -= w_r * abr[k] - w_i * abi[k] + v_r * abr[TNB + k] - v_i * abi[TNB + k];
-        yi -= w_r * abi[k] + w_i * abr[k] + v_r * abi[TNB + k] + v_i * abr[TNB + k];
+        yr -= wkr[k] * abr[k] - wki[k] * abi[k] + vkr[k] * abr[TNB + k] - vki[k] * abi[TNB + k];
+        yi -= wkr[k] * abi[k] + wki[k] * abr[k] + vkr[k] * abi[TNB + k] + vki[k] * abr[TNB + k];
       }
+    }
+    if (live) {
       const double vr_ = Vr[(size_t)n * ii + r], vi_ = Vi[(size_t)n * ii + r];
       wrr = tr * yr - ti * yi + alpha2 * vr_;
       wri = tr * yi + ti * yr + alpha2 * vi_;
@@ -394,55 +529,42 @@ __global__ __launch_bounds__(256) void k_ztri_col(ZTriWork g, int fin, int col, 
   __syncthreads();
   double *acr = g.ar + (size_t)n * col, *aci = g.ai + (size_t)n * col;
   double nrm = 0.0;
-  if (live && r >= col) {
-    double xr = acr[r], xi = (r == col) ? 0.0 : aci[r];
-    for (int k = 0; k < jj; k++) {
-      const double w_r = (k == ii) ? wrr : Wr[(size_t)n * k + r], w_i = (k == ii) ? wri : Wi[(size_t)n * k + r];
-      const double v_r = Vr[(size_t)n * k + r], v_i = Vi[(size_t)n * k + r];
-      // -= V(r,k) conj(W(col,k)) + W(r,k) conj(V(col,k))
-      xr -= v_r * wcr[k] + v_i * wci[k] + w_r * vcr[k] + w_i * vci[k];
-      xi -= v_i * wcr[k] - v_r * wci[k] + w_i * vcr[k] - w_r * vci[k];
+  {
+    double ukr[TNB], uki[TNB], pkr[TNB], pki[TNB];  // V(r,k), W(r,k), k < jj
+#pragma unroll
+    for (int k = 0; k < TNB; k++) {
+      const size_t o = (size_t)n * k + rs;
+      ukr[k] = Vr[o]; uki[k] = Vi[o]; pkr[k] = Wr[o]; pki[k] = Wi[o];
     }
-    if (r == col) xi = 0.0;
-    acr[r] = xr;
-    aci[r] = xi;
-    if (r >= col + 2) nrm = xr * xr + xi * xi;
+    if (live && r >= col) {
+      double xr = acr[r], xi = (r == col) ? 0.0 : aci[r];
+#pragma unroll
+      for (int k = 0; k < TNB; k++) {
+        if (k < jj) {
+          const double w_r = (k == ii) ? wrr : pkr[k], w_i = (k == ii) ? wri : pki[k];
+          // -= V(r,k) conj(W(col,k)) + W(r,k) conj(V(col,k))
+          xr -= ukr[k] * wcr[k] + uki[k] * wci[k] + w_r * vcr[k] + w_i * vci[k];
+          xi -= uki[k] * wcr[k] - ukr[k] * wci[k] + w_i * vcr[k] - w_r * vci[k];
+        }
+      }
+      if (r == col) xi = 0.0;
+      acr[r] = xr;
+      aci[r] = xi;
+      if (r == col) {
+        g.d[col] = xr;
+        if (col >= n - 1) {
+          g.taur[col] = 0.0; g.taui[col] = 0.0;
+          g.scr[col] = 0.0; g.sci[col] = 0.0;
+        }
+      }
+      if (r >= col + 2) nrm = xr * xr + xi * xi;
+    }
   }
   nrm = wave_sum(nrm);
+  __syncthreads();
   if ((t & 63) == 0) red[t >> 6] = nrm;
   __syncthreads();
-  if (t == 0) {
-    g.norm[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
-    __threadfence();
-    last = (atomicAdd(g.counter, 1) == (int)gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last || t != 0) return;
-  __threadfence();
-  *g.counter = 0;
-  double xs = 0.0;
-  for (int b = 0; b < (int)gridDim.x; b++) xs += g.norm[b];
-  g.d[col] = acr[col];
-  if (col >= n - 1) {
-    g.taur[col] = 0.0; g.taui[col] = 0.0;
-    g.scr[col] = 0.0; g.sci[col] = 0.0;
-    return;
-  }
-  // ZLARFG on (A(col+1,col), A(col+2:n,col))
-  const double xnorm = sqrt(xs);
-  const double alr = acr[col + 1], ali = aci[col + 1];
-  double taur = 0.0, taui = 0.0, beta = alr, scr = 0.0, sci = 0.0;
-  if (!(xnorm == 0.0 && ali == 0.0)) {
-    beta = -copysign(sqrt(alr * alr + ali * ali + xnorm * xnorm), alr);
-    taur = (beta - alr) / beta;
-    taui = -ali / beta;
-    const double zr = alr - beta, zi = ali, den = zr * zr + zi * zi;
-    scr = zr / den;
-    sci = -zi / den;
-  }
-  g.e[col] = beta;
-  g.taur[col] = taur; g.taui[col] = taui;
-  g.scr[col] = scr; g.sci[col] = sci;
+  if (t == 0) g.norm[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 __global__ void k_ztri_store(ZTriWork g, int ncols)
@@ -627,7 +749,7 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
       nref = k + 1;
       const int rows = n - (i + 1);
       const int nt = (rows + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
-      hipLaunchKernelGGL(k_ztri_symv, dim3(nblk), dim3(SYMV_T), 0, s, g, i, nt);
+      hipLaunchKernelGGL(k_ztri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
       W2K_STAGE(s, "k_ztri_symv");
       const int next = (k + 1 < nb) ? i + 1 : -1;
       hipLaunchKernelGGL(k_ztri_col, dim3((rows + 255) / 256), dim3(256), 0, s, g, i, next, nt, nt, nblk);
