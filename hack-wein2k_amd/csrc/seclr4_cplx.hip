@@ -209,8 +209,10 @@ struct ZTriWork {
   int *counter;
 };
 
+constexpr int ZCOL_ROWS = 32;  // rows per block of the column kernel (8 threads per row)
+
 // ZLARFG of column i on (A(i+1,i), A(i+2:n,i)) from the squared-norm
-// partials of the column kernel (one per block of 256 rows), evaluated by
+// partials of the column kernel (one per block of its rows), evaluated by
 // every SYMV wave with the same fixed-order arithmetic (so no grid-wide step
 // separates the two kernels); the wave with store set writes E, TAU and the
 // scale 1/(alpha - beta) of v.
@@ -218,7 +220,7 @@ __device__ __forceinline__ void ztri_reflector(const ZTriWork &g, int i, int lan
                                                double &sci)
 {
   const int n = g.n;
-  const int nnorm = (n - i + 255) / 256;
+  const int nnorm = (n - i + ZCOL_ROWS - 1) / ZCOL_ROWS;  // blocks of the column kernel
   double x[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) x[u] = g.norm[min(lane + 64 * u, nnorm - 1)];
@@ -396,175 +398,234 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
   if (lane == 0) g.vy[bid] = vy;
 }
 
+// sum over the 8 lanes 8q..8q+7, the same value in each
+__device__ __forceinline__ double oct_sum(double v)
+{
+  v += bfly_partner<1>(v);
+  v += bfly_partner<2>(v);
+  v += bfly_partner<4>(v);
+  return v;
+}
+
 // Finish W(:, fin - p0) (fin >= 0) and update column col (col >= 0) with the
 // panel so far (ZLATRD); write the squared-norm partial of its
 // sub-subdiagonal part per block (ZLARFG itself is evaluated by the next
-// SYMV's waves).  One thread per row; every runtime-length sum reads its
-// operands in batches of ZB (clamped indices, masked values) so that the
-// loads of a batch are in flight together.
-constexpr int ZB = 8;
+// SYMV's waves).  Eight threads per row, ZCOL_ROWS rows per block; each
+// thread takes an eighth of its row's SYMV partials and of the panel
+// columns.  Every load that does not depend on a value formed here is issued
+// before any of them is used: the first batch of every sum is unconditional
+// (clamped indices, masked values), the tile partials are read in pairs.
 __global__ __launch_bounds__(256) void k_ztri_col(ZTriWork g, int fin, int col, int ntile, int nch, int nvy)
 {
-  const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x;
+  constexpr int BD = 4;        // first batch of V^H v, W^H v pairs per array (ntile <= 8 BD)
+  constexpr int BV = 6;        // first batch of v^H y pairs (nvy <= 512 BV)
+  constexpr int BP = 8;        // first batch of SYMV partials of a row (nch <= 8 BP)
+  constexpr int KO = TNB / 8;  // panel columns per thread
+  const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int q8 = t & 7;
   const int rbase = fin >= 0 ? fin + 1 : col;
-  const int r = rbase + blockIdx.x * 256 + t;
+  const int r = rbase + blockIdx.x * ZCOL_ROWS + (t >> 3);
   const bool live = r < n;
   const int rs = live ? r : rbase;
   __shared__ double abr[2 * TNB], abi[2 * TNB];  // a_k = V_k^H v, b_k = W_k^H v
   __shared__ double wcr[TNB], wci[TNB], vcr[TNB], vci[TNB];
-  __shared__ double sh[4], red[4], red2[4];
+  __shared__ double reddr[8][2 * TNB], reddi[8][2 * TNB];
+  __shared__ double sh[3][4];
   const int ii = fin >= 0 ? fin - p0 : -1;
-  double wrr = 0.0, wri = 0.0;
-  double tr = 0.0, ti = 0.0, alpha2 = 0.0;
+  const int jj = col - p0;
   const double *Vr = g.Pr, *Vi = g.Pi, *Wr = g.Pr + (size_t)n * nb, *Wi = g.Pi + (size_t)n * nb;
+  double *acr = g.ar + (size_t)n * (col >= 0 ? col : 0), *aci = g.ai + (size_t)n * (col >= 0 ? col : 0);
+  // ---- loads
+  const double ar0 = (col >= 0) ? acr[rs] : 0.0, ai0 = (col >= 0) ? aci[rs] : 0.0;
+  double vc_r = 0.0, vc_i = 0.0, wc_r = 0.0, wc_i = 0.0, pc_r = 0.0, pc_i = 0.0;
+  if (col >= 0 && t < jj) {
+    vc_r = Vr[(size_t)n * t + col];
+    vc_i = Vi[(size_t)n * t + col];
+    if (t < ii || fin < 0) {
+      wc_r = Wr[(size_t)n * t + col];
+      wc_i = Wi[(size_t)n * t + col];
+    }
+  }
+  if (fin >= 0 && col >= 0 && t < nch) {
+    pc_r = g.partr[(size_t)t * n + col];
+    pc_i = g.parti[(size_t)t * n + col];
+  }
+  double vkr[KO], vki[KO], wkr[KO], wki[KO];  // V(r,k), W(r,k) for k = KO q8 + u
+#pragma unroll
+  for (int u = 0; u < KO; u++) {
+    const size_t o = (size_t)n * (KO * q8 + u) + rs;
+    vkr[u] = Vr[o]; vki[u] = Vi[o]; wkr[u] = Wr[o]; wki[u] = Wi[o];
+  }
+  double tr = 0.0, ti = 0.0, alpha2 = 0.0, wrr = 0.0, wri = 0.0, viir = 0.0, viii = 0.0;
+  double yr = 0.0, yi = 0.0, vs = 0.0;
+  const int dq = t & 31, dp = t >> 5;  // dots: columns 2dq, 2dq+1 of tiles dp + 8u
   if (fin >= 0) {
     tr = g.taur[fin];
     ti = g.taui[fin];
-    if (t < 2 * TNB) {
-      const int q = t;
-      double sr = 0.0, si = 0.0;
-      if ((q & (TNB - 1)) < ii) {
-        for (int b0 = 0; b0 < ntile; b0 += ZB) {
-          double xr[ZB], xi[ZB];
+    viir = Vr[(size_t)n * ii + rs];
+    viii = Vi[(size_t)n * ii + rs];
+    double2 xdr[BD], xdi[BD], xv[BV];
+    double xpr[BP], xpi[BP];
+    const int vlast = (nvy - 1) & ~1;
 #pragma unroll
-          for (int u = 0; u < ZB; u++) {
-            const size_t o = (size_t)min(b0 + u, ntile - 1) * 2 * TNB + q;
-            xr[u] = g.dotr[o];
-            xi[u] = g.doti[o];
-          }
+    for (int u = 0; u < BD; u++) {
+      const size_t o = (size_t)min(dp + 8 * u, ntile - 1) * 2 * TNB + 2 * dq;
+      xdr[u] = *reinterpret_cast<const double2 *>(g.dotr + o);
+      xdi[u] = *reinterpret_cast<const double2 *>(g.doti + o);
+    }
 #pragma unroll
-          for (int u = 0; u < ZB; u++) {
-            sr += (b0 + u < ntile) ? xr[u] : 0.0;
-            si += (b0 + u < ntile) ? xi[u] : 0.0;
-          }
-        }
-      }
-      abr[q] = sr;
-      abi[q] = si;
+    for (int u = 0; u < BV; u++) xv[u] = *reinterpret_cast<const double2 *>(g.vy + min(2 * t + 512 * u, vlast));
+#pragma unroll
+    for (int u = 0; u < BP; u++) {
+      const size_t o = (size_t)min(q8 + 8 * u, nch - 1) * n + rs;
+      xpr[u] = g.partr[o];
+      xpi[u] = g.parti[o];
     }
     {
-      double s = 0.0;
-      for (int b = t; b < nvy; b += blockDim.x) s += g.vy[b];
-      s = wave_sum(s);
-      if ((t & 63) == 0) red[t >> 6] = s;
-    }
-    // y(r) from the SYMV partials, minus the panel terms
-    double yr = 0.0, yi = 0.0;
-    for (int c0 = 0; c0 < nch; c0 += ZB) {
-      double xr[ZB], xi[ZB];
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-      for (int u = 0; u < ZB; u++) {
-        const size_t o = (size_t)min(c0 + u, nch - 1) * n + rs;
-        xr[u] = g.partr[o];
-        xi[u] = g.parti[o];
+      for (int u = 0; u < BD; u++) {
+        const bool k = dp + 8 * u < ntile;
+        s0 += k ? xdr[u].x : 0.0;
+        s1 += k ? xdr[u].y : 0.0;
+        s2 += k ? xdi[u].x : 0.0;
+        s3 += k ? xdi[u].y : 0.0;
       }
+      for (int b = dp + 8 * BD; b < ntile; b += 8) {
+        const size_t o = (size_t)b * 2 * TNB + 2 * dq;
+        const double2 x = *reinterpret_cast<const double2 *>(g.dotr + o);
+        const double2 z = *reinterpret_cast<const double2 *>(g.doti + o);
+        s0 += x.x; s1 += x.y; s2 += z.x; s3 += z.y;
+      }
+      reddr[dp][2 * dq] = s0; reddr[dp][2 * dq + 1] = s1;
+      reddi[dp][2 * dq] = s2; reddi[dp][2 * dq + 1] = s3;
+    }
+    {
+      double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-      for (int u = 0; u < ZB; u++) {
-        yr += (c0 + u < nch) ? xr[u] : 0.0;
-        yi += (c0 + u < nch) ? xi[u] : 0.0;
+      for (int u = 0; u < BV; u++) {
+        const int j = 2 * t + 512 * u;
+        s0 += (j < nvy) ? xv[u].x : 0.0;
+        s1 += (j + 1 < nvy) ? xv[u].y : 0.0;
+      }
+      for (int j = 2 * t + 512 * BV; j < nvy; j += 512) {
+        s0 += g.vy[j];
+        s1 += (j + 1 < nvy) ? g.vy[j + 1] : 0.0;
+      }
+      vs = s0 + s1;
+    }
+    {
+#pragma unroll
+      for (int u = 0; u < BP; u++) {
+        yr += (q8 + 8 * u < nch) ? xpr[u] : 0.0;
+        yi += (q8 + 8 * u < nch) ? xpi[u] : 0.0;
+      }
+      for (int c0 = q8 + 8 * BP; c0 < nch; c0 += 8) {
+        yr += g.partr[(size_t)c0 * n + rs];
+        yi += g.parti[(size_t)c0 * n + rs];
       }
     }
-    double vkr[TNB], vki[TNB], wkr[TNB], wki[TNB];  // V(r,k), W(r,k), k < ii
-#pragma unroll
-    for (int k = 0; k < TNB; k++) {
-      const size_t o = (size_t)n * k + rs;
-      vkr[k] = Vr[o]; vki[k] = Vi[o]; wkr[k] = Wr[o]; wki[k] = Wi[o];
+    for (int c = t + blockDim.x; c < nch && col >= 0; c += blockDim.x) {
+      pc_r += g.partr[(size_t)c * n + col];
+      pc_i += g.parti[(size_t)c * n + col];
     }
-    __syncthreads();
-    if (t == 0) sh[0] = (red[0] + red[1]) + (red[2] + red[3]);
-    __syncthreads();
-    // alpha2 = -1/2 |tau|^2 (v^H A v - 2 Re sum conj(a_k) b_k)
+    vs = wave_sum(vs);
+    const double psr = wave_sum(pc_r), psi = wave_sum(pc_i);
+    if (lane == 0) { sh[0][wid] = vs; sh[1][wid] = psr; sh[2][wid] = psi; }
+  }
+  if (col >= 0 && t < jj) {
+    vcr[t] = vc_r; vci[t] = vc_i;
+    if (t < ii || fin < 0) { wcr[t] = wc_r; wci[t] = wc_i; }
+  }
+  __syncthreads();
+  if (fin >= 0 && t < 2 * TNB) {
+    double qr = 0.0, qi = 0.0;
+#pragma unroll
+    for (int p = 0; p < 8; p++) { qr += reddr[p][t]; qi += reddi[p][t]; }
+    const bool use = (t & (TNB - 1)) < ii;
+    abr[t] = use ? qr : 0.0;
+    abi[t] = use ? qi : 0.0;
+  }
+  __syncthreads();
+  double wciir = 0.0, wciii = 0.0;  // W(col, ii)
+  if (fin >= 0) {
+    const double vty = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
     double ssum = 0.0;
-    for (int k = 0; k < ii; k++) ssum += abr[k] * abr[TNB + k] + abi[k] * abi[TNB + k];
-    alpha2 = -0.5 * (tr * tr + ti * ti) * (sh[0] - 2.0 * ssum);
-    double *Wir = g.Pr + (size_t)n * (nb + ii), *Wii = g.Pi + (size_t)n * (nb + ii);
 #pragma unroll
-    for (int k = 0; k < TNB; k++) {
+    for (int k = 0; k < TNB; k++) ssum += abr[k] * abr[TNB + k] + abi[k] * abi[TNB + k];
+    alpha2 = -0.5 * (tr * tr + ti * ti) * (vty - 2.0 * ssum);
+    // y -= W(r,k) a_k + V(r,k) b_k over this thread's columns
+    double zr = 0.0, zi = 0.0;
+#pragma unroll
+    for (int u = 0; u < KO; u++) {
+      const int k = KO * q8 + u;
       if (k < ii) {
-        // y -= W(r,k) a_k + V(r,k) b_k
-        yr -= wkr[k] * abr[k] - wki[k] * abi[k] + vkr[k] * abr[TNB + k] - vki[k] * abi[TNB + k];
-        yi -= wkr[k] * abi[k] + wki[k] * abr[k] + vkr[k] * abi[TNB + k] + vki[k] * abr[TNB + k];
+        zr += wkr[u] * abr[k] - wki[u] * abi[k] + vkr[u] * abr[TNB + k] - vki[u] * abi[TNB + k];
+        zi += wkr[u] * abi[k] + wki[u] * abr[k] + vkr[u] * abi[TNB + k] + vki[u] * abr[TNB + k];
       }
     }
-    if (live) {
-      const double vr_ = Vr[(size_t)n * ii + r], vi_ = Vi[(size_t)n * ii + r];
-      wrr = tr * yr - ti * yi + alpha2 * vr_;
-      wri = tr * yi + ti * yr + alpha2 * vi_;
-      Wir[r] = wrr;
-      Wii[r] = wri;
-    }
+    yr = oct_sum(yr) - oct_sum(zr);
+    yi = oct_sum(yi) - oct_sum(zi);
+    wrr = tr * yr - ti * yi + alpha2 * viir;
+    wri = tr * yi + ti * yr + alpha2 * viii;
+    double *Wir = g.Pr + (size_t)n * (nb + ii), *Wii = g.Pi + (size_t)n * (nb + ii);
+    if (live && q8 == 0) { Wir[r] = wrr; Wii[r] = wri; }
     for (int q = blockIdx.x * 256 + t; q < rbase; q += gridDim.x * 256) { Wir[q] = 0.0; Wii[q] = 0.0; }
+    if (col >= 0) {
+      // row col = fin + 1 of the same formula (v(col) = 1), in every thread
+      double cr = 0.0, ci = 0.0;
+      for (int k = 0; k < ii; k++) {
+        cr += wcr[k] * abr[k] - wci[k] * abi[k] + vcr[k] * abr[TNB + k] - vci[k] * abi[TNB + k];
+        ci += wcr[k] * abi[k] + wci[k] * abr[k] + vcr[k] * abi[TNB + k] + vci[k] * abr[TNB + k];
+      }
+      const double sr = ((sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3])) - cr;
+      const double si = ((sh[2][0] + sh[2][1]) + (sh[2][2] + sh[2][3])) - ci;
+      wciir = tr * sr - ti * si + alpha2;
+      wciii = tr * si + ti * sr;
+    }
   }
   if (col < 0) return;
-  const int jj = col - p0;
-  if (t < jj) {
-    vcr[t] = Vr[(size_t)n * t + col];
-    vci[t] = Vi[(size_t)n * t + col];
-    if (t < ii || fin < 0) {
-      wcr[t] = Wr[(size_t)n * t + col];
-      wci[t] = Wi[(size_t)n * t + col];
-    }
-  }
-  if (fin >= 0) {
-    // W(col, ii) recomputed by every block
-    double sr = 0.0, si = 0.0;
-    for (int c = t; c < nch; c += blockDim.x) { sr += g.partr[(size_t)c * n + col]; si += g.parti[(size_t)c * n + col]; }
-    for (int k = t; k < ii; k += blockDim.x) {
-      const double w_r = Wr[(size_t)n * k + col], w_i = Wi[(size_t)n * k + col];
-      const double v_r = Vr[(size_t)n * k + col], v_i = Vi[(size_t)n * k + col];
-      sr -= w_r * abr[k] - w_i * abi[k] + v_r * abr[TNB + k] - v_i * abi[TNB + k];
-      si -= w_r * abi[k] + w_i * abr[k] + v_r * abi[TNB + k] + v_i * abr[TNB + k];
-    }
-    sr = wave_sum(sr);
-    si = wave_sum(si);
-    __syncthreads();
-    if ((t & 63) == 0) { red[t >> 6] = sr; red2[t >> 6] = si; }
-    __syncthreads();
-    if (t == 0) {
-      const double yr = (red[0] + red[1]) + (red[2] + red[3]), yi = (red2[0] + red2[1]) + (red2[2] + red2[3]);
-      wcr[ii] = tr * yr - ti * yi + alpha2;  // v(col) = 1
-      wci[ii] = tr * yi + ti * yr;
-    }
-  }
-  __syncthreads();
-  double *acr = g.ar + (size_t)n * col, *aci = g.ai + (size_t)n * col;
-  double nrm = 0.0;
+  double xr = ar0, xi = ai0;
   {
-    double ukr[TNB], uki[TNB], pkr[TNB], pki[TNB];  // V(r,k), W(r,k), k < jj
+    // -= V(r,k) conj(W(col,k)) + W(r,k) conj(V(col,k)), k < jj
+    double a0r = 0.0, a0i = 0.0;
 #pragma unroll
-    for (int k = 0; k < TNB; k++) {
-      const size_t o = (size_t)n * k + rs;
-      ukr[k] = Vr[o]; uki[k] = Vi[o]; pkr[k] = Wr[o]; pki[k] = Wi[o];
+    for (int u = 0; u < KO; u++) {
+      const int k = KO * q8 + u;
+      const bool use = k < jj && k != ii;
+      const double wcr_ = use ? wcr[min(k, TNB - 1)] : 0.0, wci_ = use ? wci[min(k, TNB - 1)] : 0.0;
+      const double vcr_ = use ? vcr[min(k, TNB - 1)] : 0.0, vci_ = use ? vci[min(k, TNB - 1)] : 0.0;
+      a0r += vkr[u] * wcr_ + vki[u] * wci_ + wkr[u] * vcr_ + wki[u] * vci_;
+      a0i += vki[u] * wcr_ - vkr[u] * wci_ + wki[u] * vcr_ - wkr[u] * vci_;
     }
-    if (live && r >= col) {
-      double xr = acr[r], xi = (r == col) ? 0.0 : aci[r];
-#pragma unroll
-      for (int k = 0; k < TNB; k++) {
-        if (k < jj) {
-          const double w_r = (k == ii) ? wrr : pkr[k], w_i = (k == ii) ? wri : pki[k];
-          // -= V(r,k) conj(W(col,k)) + W(r,k) conj(V(col,k))
-          xr -= ukr[k] * wcr[k] + uki[k] * wci[k] + w_r * vcr[k] + w_i * vci[k];
-          xi -= uki[k] * wcr[k] - ukr[k] * wci[k] + w_i * vcr[k] - w_r * vci[k];
-        }
-      }
-      if (r == col) xi = 0.0;
-      acr[r] = xr;
-      aci[r] = xi;
-      if (r == col) {
-        g.d[col] = xr;
-        if (col >= n - 1) {
-          g.taur[col] = 0.0; g.taui[col] = 0.0;
-          g.scr[col] = 0.0; g.sci[col] = 0.0;
-        }
-      }
-      if (r >= col + 2) nrm = xr * xr + xi * xi;
+    a0r = oct_sum(a0r);
+    a0i = oct_sum(a0i);
+    if (fin >= 0) {  // k = ii: this kernel's W(r, ii) and W(col, ii)
+      const double vr_ = vcr[ii], vi_ = vci[ii];
+      a0r += viir * wciir + viii * wciii + wrr * vr_ + wri * vi_;
+      a0i += viii * wciir - viir * wciii + wri * vr_ - wrr * vi_;
     }
+    xr -= a0r;
+    xi -= a0i;
+  }
+  double nrm = 0.0;
+  if (live && q8 == 0 && r >= col) {
+    if (r == col) xi = 0.0;
+    acr[r] = xr;
+    aci[r] = xi;
+    if (r == col) {
+      g.d[col] = xr;
+      if (col >= n - 1) {
+        g.taur[col] = 0.0; g.taui[col] = 0.0;
+        g.scr[col] = 0.0; g.sci[col] = 0.0;
+      }
+    }
+    if (r >= col + 2) nrm = xr * xr + xi * xi;
   }
   nrm = wave_sum(nrm);
+  if (lane == 0) sh[0][wid] = nrm;  // sh[0] was last read before the second barrier
   __syncthreads();
-  if ((t & 63) == 0) red[t >> 6] = nrm;
-  __syncthreads();
-  if (t == 0) g.norm[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (t == 0) g.norm[blockIdx.x] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
 }
 
 __global__ void k_ztri_store(ZTriWork g, int ncols)
@@ -740,7 +801,7 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
     const int nb = std::min(TNB, n - p0);
     g.p0 = p0;
     g.nb = nb;
-    hipLaunchKernelGGL(k_ztri_col, dim3((n - p0 + 255) / 256), dim3(256), 0, s, g, -1, p0, 0, 0, 0);
+    hipLaunchKernelGGL(k_ztri_col, dim3((n - p0 + ZCOL_ROWS - 1) / ZCOL_ROWS), dim3(256), 0, s, g, -1, p0, 0, 0, 0);
     W2K_STAGE(s, "k_ztri_col");
     int nref = 0;
     for (int k = 0; k < nb; k++) {
@@ -752,7 +813,7 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
       hipLaunchKernelGGL(k_ztri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
       W2K_STAGE(s, "k_ztri_symv");
       const int next = (k + 1 < nb) ? i + 1 : -1;
-      hipLaunchKernelGGL(k_ztri_col, dim3((rows + 255) / 256), dim3(256), 0, s, g, i, next, nt, nt, nblk);
+      hipLaunchKernelGGL(k_ztri_col, dim3((rows + ZCOL_ROWS - 1) / ZCOL_ROWS), dim3(256), 0, s, g, i, next, nt, nt, nblk);
       W2K_STAGE(s, "k_ztri_col");
     }
     if (nref > 0) {
