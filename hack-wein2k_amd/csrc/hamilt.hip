@@ -411,7 +411,10 @@ __global__ __launch_bounds__(256) void k_hamilt_lo(LoArgs g)
 template <int LT, bool CPX>
 static void launch_pw(const PwArgs &a, hipStream_t s)
 {
-  constexpr int C = 4;
+  // columns per wave: the largest count whose registers still allow three
+  // waves per SIMD (measured on the 64-atom cell: C = 3 takes 9.4 ms, C = 4
+  // at two waves per SIMD 13.3 ms)
+  constexpr int C = CPX ? 2 : 3;
   dim3 grid((a.nv + 63) / 64, (a.nv + 4 * C - 1) / (4 * C));
   hipLaunchKernelGGL((k_hamilt_pw<LT, C, CPX>), grid, dim3(256), 0, s, a);
 }
