@@ -31,6 +31,7 @@
 #include <cfloat>
 #include <cmath>
 #include <algorithm>
+#include <functional>
 #include <utility>
 #include <vector>
 
@@ -1507,17 +1508,22 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
   if (hinfo[0] != 0) return hinfo[0];  // POTRF INFO, seclr4.F:200-205
   // ---- 2. reduction: W = U^-1, X = H W, C = W^T X (lower triangle only)
   {
-    for (int j0 = NB; j0 < n; j0 += NB) {
-      const int kb = std::min(NB, n - j0);
-      // T = W(0:j0,0:j0) U(0:j0, j0:j0+kb); W(0:j0, j0:) = -T W_jj
-      double *T = X.p;
-      if ((rc = dgemm(s, 'N', 'N', j0, kb, j0, 1.0, W, n, U + (size_t)n * j0, n, 0.0, T, n, 0, 1, 0, c.w_gk.p,
-                      c.w_gk.n)))
-        return rc;
-      if ((rc = dgemm(s, 'N', 'N', j0, kb, kb, -1.0, T, n, W + (size_t)j0 + (size_t)n * j0, n, 0.0,
-                      W + (size_t)n * j0, n, 0, 0, 1)))
-        return rc;
-    }
+    // W = U^-1 by recursive halving on 64-column boundaries: the two
+    // diagonal halves first (the 64x64 blocks come from the Cholesky), then
+    // W12 = -(W11 U12) W22, so the large products are square
+    std::function<int(int, int)> trtri = [&](int j0, int nb) -> int {
+      if (nb <= NB) return 0;
+      const int n1 = ((nb / 2 + NB - 1) / NB) * NB, n2 = nb - n1;
+      int r;
+      if ((r = trtri(j0, n1)) || (r = trtri(j0 + n1, n2))) return r;
+      double *T = X.p;  // n1 x n2 scratch (leading dimension n)
+      if ((r = dgemm(s, 'N', 'N', n1, n2, n1, 1.0, W + (size_t)j0 + (size_t)n * j0, n,
+                     U + (size_t)j0 + (size_t)n * (j0 + n1), n, 0.0, T, n, 0, 1, 0, c.w_gk.p, c.w_gk.n)))
+        return r;
+      return dgemm(s, 'N', 'N', n1, n2, n2, -1.0, T, n, W + (size_t)(j0 + n1) + (size_t)n * (j0 + n1), n, 0.0,
+                   W + (size_t)j0 + (size_t)n * (j0 + n1), n, 0, 0, 1);
+    };
+    if ((rc = trtri(0, n))) return rc;
     if ((rc = dgemm(s, 'N', 'N', n, n, n, 1.0, Hf, lda, W, n, 0.0, X.p, n, 0, 0, 1))) return rc;
     if ((rc = dgemm(s, 'T', 'N', n, n, n, 1.0, W, n, X.p, n, 0.0, Hf, lda, 1, 2, 0))) return rc;
   }
