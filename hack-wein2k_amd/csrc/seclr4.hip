@@ -5,13 +5,13 @@
 // (diagonal included), S in the strict upper triangle, diag(S) in HSDIAG.
 //
 // Stages (all on the context stream):
-//   1. Cholesky S = U^T U (POTRF 'U', seclr4.F:196): blocked right-looking,
-//      64x64 diagonal blocks factored and inverted by one wave in registers,
-//      the panel TRSM as a GEMM with that inverse, the trailing SYRK as an
-//      MFMA GEMM on the upper triangle.
+//   1. Cholesky S = U^T U (POTRF 'U', seclr4.F:196) together with
+//      W = U^-1: recursive halving on 64-column boundaries, 64x64 leaves
+//      factored and inverted by one wave in registers, the TRSM as a GEMM
+//      with the leading half's inverse, the SYRK as an MFMA GEMM on the upper
+//      triangle, W12 = -(W11 U12) W22.
 //   2. Reduction to standard form C = U^-T H U^-1 (DSCGST, dscgst.f:99-142):
-//      W = U^-1 by blocked triangular inversion, then X = H W and C = W^T X as
-//      MFMA GEMMs (lower triangle of C; the tridiagonalisation reads and
+//      X = H W and C = W^T X as MFMA GEMMs (lower triangle of C; the tridiagonalisation reads and
 //      updates only the lower triangle).
 //   3. Householder tridiagonalisation (DSYTRD2, dsytrd2.f, lower; DLATRD
 //      panels of 32 columns, trailing rank-64 update on MFMA), DLARFG signs.
@@ -30,6 +30,7 @@
 // runs the same stages in seclr4_cplx.hip.
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <algorithm>
 #include <functional>
 #include <utility>
@@ -867,9 +868,16 @@ __global__ void k_stein_prep(BisArgs g, int m, const double *__restrict__ w, dou
 }
 
 // tridiagonal LU with partial pivoting (DLAGTF) and solves (DLAGTS), one
-// thread per eigenvalue; factors and the work vector interleaved
-// [row][eigenvalue] so that a wave's accesses coalesce, and every running
-// value carried in registers so that no pass waits on its own stores
+// thread per eigenvalue.  Factors and the work vector are stored as
+// [row pair][eigenvalue][2]: one 16-byte load brings two rows of a thread's
+// eigenvalue and a wave's loads coalesce, so a group of SG rows needs half
+// the load instructions (a wave holds at most 63 loads in flight, and with
+// one wave per 64 eigenvalues that count sets the pace).  Every running
+// value is carried in registers so that no pass waits on its own stores,
+// and U's diagonal is kept as its reciprocal so the back-substitutions
+// multiply.  The planes hold rows 0 .. n (n + 1 rows rounded up to pairs).
+constexpr int SG = 32;  // rows per group of the solves: the group's loads are issued before its arithmetic
+
 __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__restrict__ xs,
                                               double *__restrict__ z, int ldz, double *__restrict__ u0,
                                               double *__restrict__ u1, double *__restrict__ u2,
@@ -888,6 +896,15 @@ __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__
   const double lam = xs[t];
   const double *d = g.d + ib, *e = g.e + ib;
   const size_t M = (size_t)m;
+  // element (row k, this eigenvalue) and pair index (even row k) of a plane
+  auto at = [&](int k) -> size_t { return ((size_t)(k >> 1) * M + t) * 2 + (k & 1); };
+  auto pr = [&](int k) -> size_t { return (size_t)(k >> 1) * M + t; };
+  const double2 *X2 = reinterpret_cast<const double2 *>(x);
+  const double2 *R2 = reinterpret_cast<const double2 *>(u0);
+  const double2 *A12 = reinterpret_cast<const double2 *>(u1);
+  const double2 *A22 = reinterpret_cast<const double2 *>(u2);
+  const double2 *M2 = reinterpret_cast<const double2 *>(mul);
+  const int2 *P2 = reinterpret_cast<const int2 *>(piv);
   const double onenrm = g.bnrm[b];
   const double tol = fmax(fabs(lam), onenrm) * DBL_EPSILON;
   // factorisation; small pivots of U perturbed to +-tol for the solves
@@ -904,7 +921,7 @@ __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__
       if (k < bs - 1) {
         const double sub = ebt[kq];
         const double dn = dbt[kq] - lam, en = (k + 1 < bs - 1) ? ebt[kq + 1] : 0.0;
-        const size_t o = (size_t)k * M + t;
+        const size_t o = at(k);
         double p0, p1, p2, mm;
         int pv;
         if (fabs(a0) >= fabs(sub)) {
@@ -920,15 +937,15 @@ __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__
           a1 = na1;
         }
         if (fabs(p0) < tol) p0 = (p0 < 0 ? -tol : tol);
-        piv[o] = pv; u0[o] = p0; u1[o] = p1; u2[o] = p2; mul[o] = mm;
+        piv[o] = pv; u0[o] = 1.0 / p0; u1[o] = p1; u2[o] = p2; mul[o] = mm;
       }
     }
   }
   {
-    const size_t o = (size_t)(bs - 1) * M + t;
+    const size_t o = at(bs - 1);
     double p0 = a0;
     if (fabs(p0) < tol) p0 = (p0 < 0 ? -tol : tol);
-    u0[o] = p0; u1[o] = 0.0; u2[o] = 0.0;
+    u0[o] = 1.0 / p0; u1[o] = 0.0; u2[o] = 0.0;
   }
   // deterministic start vector in [-1, 1), then three solves
   unsigned long long seed = 0x9E3779B97F4A7C15ull ^ ((unsigned long long)(t + 1) * 0xD1B54A32D192ED03ull);
@@ -937,78 +954,79 @@ __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__
     return ((double)(seed >> 11) / 9007199254740992.0) * 2.0 - 1.0;
   };
   double sc = 1.0;
-  constexpr int G = 16;  // rows per group: loads of a group are issued before its arithmetic
   for (int it = 0; it < 3; it++) {
-    // forward: L^-1 P x, x scaled by 1/max|x| of the previous solve
-    double xk = (it == 0) ? rnd() : sc * x[t];
+    const bool first = (it == 0);
+    // forward: L^-1 P x, x scaled by 1/max|x| of the previous solve; the
+    // groups start at even rows, and row k reads x(k+1) before writing x(k)
+    double xk = first ? rnd() : sc * x[at(0)];
     int k = 0;
-    for (; k + G <= bs - 1; k += G) {
-      int pg[G];
-      double mg[G], xg[G];
+    for (; k + SG <= bs - 1; k += SG) {
+      int2 pg[SG / 2];
+      double2 mg[SG / 2], xg[SG / 2 + 1];  // xg: x rows k .. k+SG+1
 #pragma unroll
-      for (int q = 0; q < G; q++) {
-        const size_t o = (size_t)(k + q) * M + t;
-        pg[q] = piv[o];
-        mg[q] = mul[o];
-        xg[q] = (it == 0) ? 0.0 : x[o + M];
+      for (int q = 0; q < SG / 2; q++) {
+        pg[q] = P2[pr(k + 2 * q)];
+        mg[q] = M2[pr(k + 2 * q)];
+      }
+      if (!first) {
+#pragma unroll
+        for (int q = 0; q <= SG / 2; q++) xg[q] = X2[pr(k + 2 * q)];
       }
 #pragma unroll
-      for (int q = 0; q < G; q++) {
-        const size_t o = (size_t)(k + q) * M + t;
-        double xn = (it == 0) ? rnd() : sc * xg[q];
-        if (pg[q]) { const double tmp = xk; xk = xn; xn = tmp; }
-        x[o] = xk;
-        xk = xn - mg[q] * xk;
+      for (int q = 0; q < SG; q++) {
+        const double xr = (q & 1) ? xg[(q + 1) / 2].x : xg[q / 2].y;  // row k+q+1
+        double xn = first ? rnd() : sc * xr;
+        const int pv = (q & 1) ? pg[q / 2].y : pg[q / 2].x;
+        const double mm = (q & 1) ? mg[q / 2].y : mg[q / 2].x;
+        if (pv) { const double tmp = xk; xk = xn; xn = tmp; }
+        x[at(k + q)] = xk;
+        xk = xn - mm * xk;
       }
     }
     for (; k < bs - 1; k++) {
-      const size_t o = (size_t)k * M + t;
-      double xn = (it == 0) ? rnd() : sc * x[o + M];
-      if (piv[o]) { const double tmp = xk; xk = xn; xn = tmp; }
-      x[o] = xk;
-      xk = xn - mul[o] * xk;
+      double xn = first ? rnd() : sc * x[at(k + 1)];
+      if (piv[at(k)]) { const double tmp = xk; xk = xn; xn = tmp; }
+      x[at(k)] = xk;
+      xk = xn - mul[at(k)] * xk;
     }
-    x[(size_t)(bs - 1) * M + t] = xk;
-    // backward: U^-1
+    x[at(bs - 1)] = xk;
+    // backward: U^-1, groups of SG rows ending at an odd row
     double mx = 0.0, x1 = 0.0, x2 = 0.0;
     const bool lastit = (it == 2);
-    k = bs - 1;
-    for (; k - G + 1 >= 0; k -= G) {
-      double xg[G], a0g[G], a1g[G], a2g[G];
-#pragma unroll
-      for (int q = 0; q < G; q++) {
-        const size_t o = (size_t)(k - q) * M + t;
-        xg[q] = x[o];
-        a0g[q] = u0[o];
-        a1g[q] = u1[o];
-        a2g[q] = u2[o];
-      }
-#pragma unroll
-      for (int q = 0; q < G; q++) {
-        const int kk = k - q;
-        double v = xg[q];
-        if (kk + 1 < bs) v -= a1g[q] * x1;
-        if (kk + 2 < bs) v -= a2g[q] * x2;
-        v /= a0g[q];
-        if (lastit) zc[ib + kk] = v;
-        else x[(size_t)kk * M + t] = v;
-        mx = fmax(mx, fabs(v));
-        x2 = x1;
-        x1 = v;
-      }
-    }
-    for (; k >= 0; k--) {
-      const size_t o = (size_t)k * M + t;
-      double v = x[o];
-      if (k + 1 < bs) v -= u1[o] * x1;
-      if (k + 2 < bs) v -= u2[o] * x2;
-      v /= u0[o];
-      if (lastit) zc[ib + k] = v;
-      else x[o] = v;
+    auto row = [&](int kk, double xv, double r0, double b1, double b2) {
+      double v = xv;
+      if (kk + 1 < bs) v -= b1 * x1;
+      if (kk + 2 < bs) v -= b2 * x2;
+      v *= r0;
+      if (lastit) zc[ib + kk] = v;
+      else x[at(kk)] = v;
       mx = fmax(mx, fabs(v));
       x2 = x1;
       x1 = v;
+    };
+    k = bs - 1;
+    if (!(k & 1)) {
+      row(k, x[at(k)], u0[at(k)], 0.0, 0.0);
+      k--;
     }
+    for (; k - SG + 1 >= 0; k -= SG) {
+      double2 xg[SG / 2], rg[SG / 2], bg1[SG / 2], bg2[SG / 2];
+#pragma unroll
+      for (int q = 0; q < SG / 2; q++) {
+        const size_t p = pr(k - SG + 1 + 2 * q);
+        xg[q] = X2[p];
+        rg[q] = R2[p];
+        bg1[q] = A12[p];
+        bg2[q] = A22[p];
+      }
+#pragma unroll
+      for (int q = SG / 2 - 1; q >= 0; q--) {
+        const int kk = k - SG + 1 + 2 * q;  // even row of pair q
+        row(kk + 1, xg[q].y, rg[q].y, bg1[q].y, bg2[q].y);
+        row(kk, xg[q].x, rg[q].x, bg1[q].x, bg2[q].x);
+      }
+    }
+    for (; k >= 0; k--) row(k, x[at(k)], u0[at(k)], u1[at(k)], u2[at(k)]);
     sc = 1.0 / mx;
   }
 }
@@ -1168,15 +1186,17 @@ __global__ __launch_bounds__(256) void k_larft_inv(int nb, const double *__restr
   }
 }
 
-// Cholesky G = R^T R of a small Gram matrix (cw <= TNB) and R^-1 (upper),
-// by one wave in registers as k_chol_diag (lane c holds column c; columns
-// past cw and the lanes above TNB carry the identity); *fail set when G is
-// not numerically positive definite (a non-positive or non-finite pivot,
-// taken as 1 so that the arithmetic stays finite)
+// Cholesky G = R^T R of a small Gram matrix (cw <= N <= 64) and R^-1
+// (upper), by one wave in registers as k_chol_diag (lane c holds column c;
+// columns past cw and the lanes above N carry the identity); *fail set when
+// G is not numerically positive definite (a non-positive or non-finite
+// pivot, taken as 1 so that the arithmetic stays finite)
+template <int N>
 __global__ __launch_bounds__(64) void k_chol_rinv(int cw, const double *__restrict__ g, double *__restrict__ rinv,
                                                   int *__restrict__ fail)
 {
-  constexpr int N = TNB, LD = N + 1;
+  static_assert(N <= 64, "one lane per column");
+  constexpr int LD = N + 1;
   __shared__ double sm[64 * LD];  // sm[c LD + r] = R(r, c), then R^-1(r, c)
   __shared__ double urow[2][64];
   __shared__ double dinv[64];
@@ -1301,59 +1321,109 @@ int tridiag_eigen(Ctx &c, Seclr4Bufs &B, double el, double eu, int nume, double 
   const size_t zc = (size_t)n * m;
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((zc + 255) / 256)), dim3(256), 0, s, zc, Zt, 0.0);
   W2K_STAGE(s, "k_fill");
-  if ((rc = c.w_fac.alloc((size_t)5 * n * m)) || (rc = c.w_piv.alloc((size_t)n * m))) return rc;
+  // factor planes of rows 0 .. n in row pairs (16-byte aligned: m pairs per row pair)
+  const size_t zp = (size_t)((n + 2) / 2) * 2 * m;
+  if ((rc = c.w_fac.alloc((size_t)5 * zp)) || (rc = c.w_piv.alloc(zp))) return rc;
   {
     double *f = c.w_fac.p;
-    hipLaunchKernelGGL(k_stein, dim3((m + 63) / 64), dim3(64), 0, s, b, m, B.xs, Zt, n, f, f + zc, f + 2 * zc,
-                       f + 3 * zc, f + 4 * zc, c.w_piv.p);
+    hipLaunchKernelGGL(k_stein, dim3((m + 63) / 64), dim3(64), 0, s, b, m, B.xs, Zt, n, f, f + zp, f + 2 * zp,
+                       f + 3 * zp, f + 4 * zp, c.w_piv.p);
     W2K_STAGE(s, "k_stein");
   }
   std::vector<int> cs(m);
   W2K_HIP(hipMemcpyAsync(cs.data(), B.cstart, sizeof(int) * m, hipMemcpyDeviceToHost, s));
   W2K_HIP(hipStreamSynchronize(s));
-  // clusters [t0, t1): chunks of 32 vectors, each orthogonalised against
+  // clusters [t0, t1): chunks of CW vectors, each orthogonalised against
   // the cluster's earlier chunks (classical Gram-Schmidt, twice, on MFMA)
   // and then within itself by CholQR2 (Z = Q R, R from the Cholesky of
-  // Z^T Z), which equals in-order Gram-Schmidt; a chunk whose Gram matrix
-  // is not numerically definite falls back to modified Gram-Schmidt
-  if ((rc = c.w_cl.alloc((size_t)2 * n * TNB + 2 * TNB * TNB + (size_t)m * TNB))) return rc;
-  double *csave = c.w_cl.p, *ctmp = csave + (size_t)n * TNB, *cg = ctmp + (size_t)n * TNB;
-  double *crinv = cg + TNB * TNB, *cy = crinv + TNB * TNB;
-  int *cfail = B.ticket + 1;
-  auto cgs2 = [&](int t0, int c0, int cw) -> int {
-    const int np = c0 - t0;
-    double *Q = Zt + (size_t)n * t0, *Zc = Zt + (size_t)n * c0;
-    for (int pass = 0; pass < 2 && np > 0; pass++) {
-      int r2;
-      if ((r2 = dgemm(s, 'T', 'N', np, cw, n, 1.0, Q, n, Zc, n, 0.0, cy, np, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
-        return r2;
-      if ((r2 = dgemm(s, 'N', 'N', n, cw, np, -1.0, Q, n, cy, np, 1.0, Zc, n))) return r2;
-    }
-    return 0;
-  };
-  int t0 = 0;
-  while (t0 < m) {
+  // Z^T Z), which equals in-order Gram-Schmidt.  All chunks run without a
+  // host round trip, each with a flag that its Cholesky sets when the Gram
+  // matrix is not numerically definite; the flags are read once, and a
+  // cluster with a flagged chunk is restored from a copy taken before from
+  // that chunk on and redone chunk by chunk, a flagged chunk then falling
+  // back to modified Gram-Schmidt
+  constexpr int CW = 64;
+  std::vector<std::pair<int, int>> cl;
+  size_t nch = 0;
+  for (int t0 = 0; t0 < m;) {
     int t1 = t0 + 1;
     while (t1 < m && cs[t1] == t0) t1++;
     if (t1 - t0 > 1) {
-      for (int c0 = t0; c0 < t1; c0 += TNB) {
-        const int cw = std::min(TNB, t1 - c0);
+      cl.emplace_back(t0, t1);
+      nch += (size_t)(t1 - t0 + CW - 1) / CW;
+    }
+    t0 = t1;
+  }
+  if (!cl.empty()) {
+    if ((rc = c.w_cl.alloc((size_t)2 * n * CW + 2 * CW * CW + (size_t)m * CW + nch + 1))) return rc;
+    double *csave = c.w_cl.p, *ctmp = csave + (size_t)n * CW, *cg = ctmp + (size_t)n * CW;
+    double *crinv = cg + CW * CW, *cy = crinv + CW * CW;
+    int *flags = reinterpret_cast<int *>(cy + (size_t)m * CW);
+    double *zbak = c.w_fac.p;  // the factors are dead; 5 zp >= n m
+    auto cgs2 = [&](int t0, int c0, int cw) -> int {
+      const int np = c0 - t0;
+      double *Q = Zt + (size_t)n * t0, *Zc = Zt + (size_t)n * c0;
+      for (int pass = 0; pass < 2 && np > 0; pass++) {
+        int r2;
+        if ((r2 = dgemm(s, 'T', 'N', np, cw, n, 1.0, Q, n, Zc, n, 0.0, cy, np, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
+          return r2;
+        if ((r2 = dgemm(s, 'N', 'N', n, cw, np, -1.0, Q, n, cy, np, 1.0, Zc, n))) return r2;
+      }
+      return 0;
+    };
+    // two passes, Zc -> ctmp -> Zc
+    auto cholqr2 = [&](double *Zc, int cw, int *flag) -> int {
+      double *src = Zc, *dst = ctmp;
+      for (int pass = 0; pass < 2; pass++) {
+        int r2;
+        if ((r2 = dgemm(s, 'T', 'N', cw, cw, n, 1.0, src, n, src, n, 0.0, cg, cw, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
+          return r2;
+        hipLaunchKernelGGL(k_chol_rinv<CW>, dim3(1), dim3(64), 0, s, cw, cg, crinv, flag);
+        W2K_STAGE(s, "k_chol_rinv");
+        if ((r2 = dgemm(s, 'N', 'N', n, cw, cw, 1.0, src, n, crinv, cw, 0.0, dst, n, 0, 0, 1))) return r2;
+        std::swap(src, dst);
+      }
+      return 0;
+    };
+    W2K_HIP(hipMemcpyAsync(zbak, Zt, sizeof(double) * zc, hipMemcpyDeviceToDevice, s));
+    W2K_HIP(hipMemsetAsync(flags, 0, sizeof(int) * nch, s));
+    size_t fi = 0;
+    for (auto &tc : cl)
+      for (int c0 = tc.first; c0 < tc.second; c0 += CW, fi++) {
+        const int cw = std::min(CW, tc.second - c0);
+        if ((rc = cgs2(tc.first, c0, cw)) || (rc = cholqr2(Zt + (size_t)n * c0, cw, flags + fi))) return rc;
+      }
+    std::vector<int> hf(nch);
+    W2K_HIP(hipMemcpyAsync(hf.data(), flags, sizeof(int) * nch, hipMemcpyDeviceToHost, s));
+    W2K_HIP(hipStreamSynchronize(s));
+    // W2K_TEST_CLUSTER_FALLBACK (tests only): 1 flags every chunk, so that
+    // every cluster is restored and redone; 2 also sends every redone chunk
+    // to modified Gram-Schmidt
+    const char *fe = std::getenv("W2K_TEST_CLUSTER_FALLBACK");
+    const int force = fe ? std::atoi(fe) : 0;
+    if (force >= 1) std::fill(hf.begin(), hf.end(), 1);
+    fi = 0;
+    for (auto &tc : cl) {
+      const int t0 = tc.first, t1 = tc.second;
+      const size_t nc = (size_t)(t1 - t0 + CW - 1) / CW;
+      size_t jf = 0;
+      while (jf < nc && !hf[fi + jf]) jf++;
+      fi += nc;
+      if (jf == nc) continue;
+      const int cf = t0 + (int)jf * CW;
+      W2K_HIP(hipMemcpyAsync(Zt + (size_t)n * cf, zbak + (size_t)n * cf, sizeof(double) * n * (t1 - cf),
+                             hipMemcpyDeviceToDevice, s));
+      int *cfail = flags + nch;
+      for (int c0 = cf; c0 < t1; c0 += CW) {
+        const int cw = std::min(CW, t1 - c0);
         double *Zc = Zt + (size_t)n * c0;
         W2K_HIP(hipMemcpyAsync(csave, Zc, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
         W2K_HIP(hipMemsetAsync(cfail, 0, sizeof(int), s));
-        if ((rc = cgs2(t0, c0, cw))) return rc;
-        for (int pass = 0; pass < 2; pass++) {
-          if ((rc = dgemm(s, 'T', 'N', cw, cw, n, 1.0, Zc, n, Zc, n, 0.0, cg, cw, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
-            return rc;
-          hipLaunchKernelGGL(k_chol_rinv, dim3(1), dim3(64), 0, s, cw, cg, crinv, cfail);
-          W2K_STAGE(s, "k_chol_rinv");
-          if ((rc = dgemm(s, 'N', 'N', n, cw, cw, 1.0, Zc, n, crinv, cw, 0.0, ctmp, n, 0, 0, 1))) return rc;
-          W2K_HIP(hipMemcpyAsync(Zc, ctmp, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
-        }
+        if ((rc = cgs2(t0, c0, cw)) || (rc = cholqr2(Zc, cw, cfail))) return rc;
         int hfail = 0;
         W2K_HIP(hipMemcpyAsync(&hfail, cfail, sizeof(int), hipMemcpyDeviceToHost, s));
         W2K_HIP(hipStreamSynchronize(s));
-        if (hfail) {
+        if (hfail || force >= 2) {
           W2K_HIP(hipMemcpyAsync(Zc, csave, sizeof(double) * n * cw, hipMemcpyDeviceToDevice, s));
           if ((rc = cgs2(t0, c0, cw))) return rc;
           hipLaunchKernelGGL(k_mgs_chunk, dim3(1), dim3(256), 0, s, n, Zt, n, c0, cw);
@@ -1361,7 +1431,6 @@ int tridiag_eigen(Ctx &c, Seclr4Bufs &B, double el, double eu, int nume, double 
         }
       }
     }
-    t0 = t1;
   }
   hipLaunchKernelGGL(k_normsign, dim3(m), dim3(256), 0, s, n, Zt, n);
   W2K_STAGE(s, "k_normsign");
@@ -1475,55 +1544,50 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
   int *info = B.info;
 
   W2K_HIP(hipEventRecord(c.ev[6], s));
-  // ---- 1. Cholesky, with U_kk^-1 of every diagonal block into W (zero
-  // elsewhere; the reduction fills in the blocks above the diagonal)
+  // ---- 1. Cholesky S = U^T U and W = U^-1 together, by recursive halving
+  // on 64-column boundaries: factor the leading half (U11, W11), form
+  // U12 = W11^T S12 and S22 -= U12^T U12 (upper triangle), factor the
+  // trailing half (U22, W22), then W12 = -(W11 U12) W22.  The 64x64 leaves
+  // are factored and inverted by one wave each; the products at the top
+  // levels are square.  W is zero below the diagonal.
   {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nn, W, 0.0);
     W2K_STAGE(s, "k_fill");
     dim3 g2((n + 255) / 256, n);
     hipLaunchKernelGGL(k_prep, g2, dim3(256), 0, s, n, lda, c.d_hs.p, c.d_hsdiag.p, U, Hf);
     W2K_STAGE(s, "k_prep");
-    for (int k0 = 0; k0 < n; k0 += NB) {
-      const int kb = std::min(NB, n - k0);
-      hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(64), 0, s, n, k0, kb, U, W, info);
-      W2K_STAGE(s, "k_chol_diag");
-      const int rest = n - k0 - kb;
-      if (rest > 0) {
-        // U(k0:k0+kb, k0+kb:n) = U_kk^-T S(k0:k0+kb, k0+kb:n)
-        double *S12 = U + (size_t)k0 + (size_t)n * (k0 + kb);
-        if ((rc = dgemm(s, 'T', 'N', kb, rest, kb, 1.0, W + (size_t)k0 + (size_t)n * k0, n, S12, n, 0.0, X.p, kb, 0,
-                        2, 0)))
-          return rc;
-        W2K_HIP(hipMemcpy2DAsync(S12, sizeof(double) * n, X.p, sizeof(double) * kb, sizeof(double) * kb, rest,
-                                 hipMemcpyDeviceToDevice, s));
-        double *C0 = U + (size_t)(k0 + kb) + (size_t)n * (k0 + kb);
-        if ((rc = dgemm(s, 'T', 'N', rest, rest, kb, -1.0, S12, n, S12, n, 1.0, C0, n, 2))) return rc;
+    std::function<int(int, int)> rchol = [&](int j0, int nb) -> int {
+      if (nb <= NB) {
+        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(64), 0, s, n, j0, nb, U, W, info);
+        W2K_STAGE(s, "k_chol_diag");
+        return 0;
       }
-    }
+      const int n1 = ((nb / 2 + NB - 1) / NB) * NB, n2 = nb - n1;
+      int r;
+      if ((r = rchol(j0, n1))) return r;
+      double *S12 = U + (size_t)j0 + (size_t)n * (j0 + n1), *S22 = U + (size_t)(j0 + n1) + (size_t)n * (j0 + n1);
+      const double *W11 = W + (size_t)j0 + (size_t)n * j0;
+      double *T = X.p;  // n1 x n2 scratch (leading dimension n)
+      if ((r = dgemm(s, 'T', 'N', n1, n2, n1, 1.0, W11, n, S12, n, 0.0, T, n, 0, 2, 0, c.w_gk.p, c.w_gk.n)))
+        return r;
+      W2K_HIP(hipMemcpy2DAsync(S12, sizeof(double) * n, T, sizeof(double) * n, sizeof(double) * n1, n2,
+                               hipMemcpyDeviceToDevice, s));
+      if ((r = dgemm(s, 'T', 'N', n2, n2, n1, -1.0, S12, n, S12, n, 1.0, S22, n, 2))) return r;
+      if ((r = rchol(j0 + n1, n2))) return r;
+      if ((r = dgemm(s, 'N', 'N', n1, n2, n1, 1.0, W11, n, S12, n, 0.0, T, n, 0, 1, 0, c.w_gk.p, c.w_gk.n)))
+        return r;
+      return dgemm(s, 'N', 'N', n1, n2, n2, -1.0, T, n, W + (size_t)(j0 + n1) + (size_t)n * (j0 + n1), n, 0.0,
+                   W + (size_t)j0 + (size_t)n * (j0 + n1), n, 0, 0, 1);
+    };
+    if ((rc = rchol(0, n))) return rc;
   }
   W2K_HIP(hipEventRecord(c.ev[7], s));
   int hinfo[4];
   W2K_HIP(hipMemcpyAsync(hinfo, info, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
   W2K_HIP(hipStreamSynchronize(s));
   if (hinfo[0] != 0) return hinfo[0];  // POTRF INFO, seclr4.F:200-205
-  // ---- 2. reduction: W = U^-1, X = H W, C = W^T X (lower triangle only)
+  // ---- 2. reduction: X = H W, C = W^T X (lower triangle only)
   {
-    // W = U^-1 by recursive halving on 64-column boundaries: the two
-    // diagonal halves first (the 64x64 blocks come from the Cholesky), then
-    // W12 = -(W11 U12) W22, so the large products are square
-    std::function<int(int, int)> trtri = [&](int j0, int nb) -> int {
-      if (nb <= NB) return 0;
-      const int n1 = ((nb / 2 + NB - 1) / NB) * NB, n2 = nb - n1;
-      int r;
-      if ((r = trtri(j0, n1)) || (r = trtri(j0 + n1, n2))) return r;
-      double *T = X.p;  // n1 x n2 scratch (leading dimension n)
-      if ((r = dgemm(s, 'N', 'N', n1, n2, n1, 1.0, W + (size_t)j0 + (size_t)n * j0, n,
-                     U + (size_t)j0 + (size_t)n * (j0 + n1), n, 0.0, T, n, 0, 1, 0, c.w_gk.p, c.w_gk.n)))
-        return r;
-      return dgemm(s, 'N', 'N', n1, n2, n2, -1.0, T, n, W + (size_t)(j0 + n1) + (size_t)n * (j0 + n1), n, 0.0,
-                   W + (size_t)j0 + (size_t)n * (j0 + n1), n, 0, 0, 1);
-    };
-    if ((rc = trtri(0, n))) return rc;
     if ((rc = dgemm(s, 'N', 'N', n, n, n, 1.0, Hf, lda, W, n, 0.0, X.p, n, 0, 0, 1))) return rc;
     if ((rc = dgemm(s, 'T', 'N', n, n, n, 1.0, W, n, X.p, n, 0.0, Hf, lda, 1, 2, 0))) return rc;
   }

@@ -220,6 +220,33 @@ def test_seclr4_degenerate_clusters(n):
         assert np.max(np.abs(R)) < 1e-9
 
 
+@pytest.mark.parametrize("force", [0, 1, 2])
+def test_seclr4_cluster_spanning_chunks(force, monkeypatch):
+    """One cluster of 150 eigenvalues (three orthogonalisation chunks of 64,
+    four of them exactly degenerate) among well separated ones.  force 1
+    flags every chunk after the batched pass, so every cluster is restored
+    from its copy and redone chunk by chunk; force 2 also sends every redone
+    chunk to modified Gram-Schmidt.  Every path must give S-orthonormal
+    vectors with small residuals."""
+    n = 200
+    rng = np.random.default_rng(5)
+    clus = 0.3 + np.sort(rng.uniform(0, 1e-4, 150))
+    clus[10:14] = clus[10]
+    rest = np.sort(rng.uniform(-2, 2, n - 150))
+    lam = np.sort(np.concatenate([clus, rest]))
+    if force:
+        monkeypatch.setenv("W2K_TEST_CLUSTER_FALLBACK", str(force))
+    for cpx in (False, True):
+        H, S = _pencil_with_spectrum(lam, n, cpx)
+        ne, nb, ev, z = _run_pencil(H, S, -1e9, 1e9, n)
+        assert ne == n and nb == 0
+        assert np.max(np.abs(ev - lam)) < 1e-10
+        G = z.conj().T @ S @ z
+        assert np.max(np.abs(G - np.eye(n))) < 1e-10, f"cpx {cpx}"
+        R = H @ z - (S @ z) * ev
+        assert np.max(np.abs(R)) < 1e-9, f"cpx {cpx}"
+
+
 def test_seclr4_not_positive_definite_reports_info():
     n = 40
     rng = np.random.default_rng(3)
