@@ -288,13 +288,15 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_big(GemmArgs g)
   }
 }
 
-// C = alpha * sum_z ws[z] + beta * C, fixed summation order
+// C = alpha * sum_z ws[z] + beta * C, fixed summation order; with a
+// triangular output (tri 1 lower, 2 upper) only that triangle is written
 __global__ void k_splitk_reduce(int m, int n, int nz, const double *__restrict__ ws, double alpha, double beta,
-                                double *__restrict__ C, int ldc)
+                                double *__restrict__ C, int ldc, int tri)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= m || j >= n) return;
+  if ((tri == 1 && i < j) || (tri == 2 && i > j)) return;
   double v = 0.0;
   for (int z = 0; z < nz; z++) v += ws[((size_t)z * n + j) * m + i];
   double *cp = C + (size_t)i + (size_t)ldc * j;
@@ -386,11 +388,21 @@ int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
   if (m <= 0 || n <= 0) return 0;
   GemmArgs g = gemm_args(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, atri, btri);
   // few output tiles and a long K: split K over z-slices so that the whole
-  // chip works, then reduce the slices in a fixed order
+  // chip works, then reduce the slices in a fixed order.  Products with a
+  // triangular operand or output and up to 4096 tiles are split into slices
+  // of about 1024 as well: their K range grows across the tiles, and
+  // unsplit, the tiles with the full K set the duration.  The rule depends
+  // on the shape and the workspace only, so a product is always summed the
+  // same way.
   dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
   const long tiles = (long)grid.x * grid.y;
-  if (ws && tri == 0 && tiles < 256 && k >= 4 * 256) {
-    int nz = (int)std::min<long>((k + 255) / 256, std::max<long>(1, 1024 / tiles));
+  const bool structured = tri || atri || btri;
+  int nz = 0;
+  if (ws && tri == 0 && tiles < 256 && k >= 4 * 256)
+    nz = (int)std::min<long>((k + 255) / 256, std::max<long>(1, 1024 / tiles));
+  else if (ws && structured && tiles >= 256 && tiles <= 4096 && k >= 1024)
+    nz = std::min((k + 1023) / 1024, 4);
+  if (nz >= 2) {
     nz = (int)std::min<size_t>((size_t)nz, wsn / ((size_t)m * n));
     if (nz >= 2) {
       g.kslice = ((k + nz - 1) / nz + BK - 1) / BK * BK;
@@ -400,7 +412,7 @@ int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
       hipLaunchKernelGGL(k_dgemm<true>, grid, dim3(256), 0, s, g);
       W2K_STAGE(s, "k_dgemm<split>");
       hipLaunchKernelGGL(k_splitk_reduce, dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, nz, ws, alpha, beta, C,
-                         ldc);
+                         ldc, tri);
       W2K_STAGE(s, "k_splitk_reduce");
       return 0;
     }

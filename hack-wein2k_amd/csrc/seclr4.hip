@@ -1567,17 +1567,18 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
       if ((r = rchol(j0, n1))) return r;
       double *S12 = U + (size_t)j0 + (size_t)n * (j0 + n1), *S22 = U + (size_t)(j0 + n1) + (size_t)n * (j0 + n1);
       const double *W11 = W + (size_t)j0 + (size_t)n * j0;
-      double *T = X.p;  // n1 x n2 scratch (leading dimension n)
-      if ((r = dgemm(s, 'T', 'N', n1, n2, n1, 1.0, W11, n, S12, n, 0.0, T, n, 0, 2, 0, c.w_gk.p, c.w_gk.n)))
-        return r;
-      W2K_HIP(hipMemcpy2DAsync(S12, sizeof(double) * n, T, sizeof(double) * n, sizeof(double) * n1, n2,
+      // X is free until the reduction: T (n1 x n2, leading dimension n1) at
+      // its start, the rest the split-K workspace of the products
+      double *T = X.p, *ws = X.p + (size_t)n1 * n2;
+      const size_t wsn = nn - (size_t)n1 * n2;
+      if ((r = dgemm(s, 'T', 'N', n1, n2, n1, 1.0, W11, n, S12, n, 0.0, T, n1, 0, 2, 0, ws, wsn))) return r;
+      W2K_HIP(hipMemcpy2DAsync(S12, sizeof(double) * n, T, sizeof(double) * n1, sizeof(double) * n1, n2,
                                hipMemcpyDeviceToDevice, s));
-      if ((r = dgemm(s, 'T', 'N', n2, n2, n1, -1.0, S12, n, S12, n, 1.0, S22, n, 2))) return r;
+      if ((r = dgemm(s, 'T', 'N', n2, n2, n1, -1.0, S12, n, S12, n, 1.0, S22, n, 2, 0, 0, X.p, nn))) return r;
       if ((r = rchol(j0 + n1, n2))) return r;
-      if ((r = dgemm(s, 'N', 'N', n1, n2, n1, 1.0, W11, n, S12, n, 0.0, T, n, 0, 1, 0, c.w_gk.p, c.w_gk.n)))
-        return r;
-      return dgemm(s, 'N', 'N', n1, n2, n2, -1.0, T, n, W + (size_t)(j0 + n1) + (size_t)n * (j0 + n1), n, 0.0,
-                   W + (size_t)j0 + (size_t)n * (j0 + n1), n, 0, 0, 1);
+      if ((r = dgemm(s, 'N', 'N', n1, n2, n1, 1.0, W11, n, S12, n, 0.0, T, n1, 0, 1, 0, ws, wsn))) return r;
+      return dgemm(s, 'N', 'N', n1, n2, n2, -1.0, T, n1, W + (size_t)(j0 + n1) + (size_t)n * (j0 + n1), n, 0.0,
+                   W + (size_t)j0 + (size_t)n * (j0 + n1), n, 0, 0, 1, ws, wsn);
     };
     if ((rc = rchol(0, n))) return rc;
   }
@@ -1680,26 +1681,59 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
   W2K_HIP(hipEventRecord(c.ev[10], s));
   if (m > 0) {
     // ---- 6. Z = Q Zt (reflectors of step 3, last panel first), sort, Z = W Z
-    if ((rc = c.w_bt.alloc((size_t)(n + 2 * m + 2 * BTNB) * BTNB))) return rc;
-    // w_bt: V block (n x BTNB), V^T V and T (BTNB x BTNB each), Y1 and Y2
-    // (BTNB x m each); blocks of BTNB reflectors, last block first
-    double *vb = c.w_bt.p, *gm = vb + (size_t)n * BTNB, *tm = gm + BTNB * BTNB, *y1 = tm + BTNB * BTNB;
-    double *y2 = y1 + (size_t)BTNB * m;
+    // Blocks of BTNB reflectors, last block first.  The reflector panel V,
+    // its Gram matrix and T (I - V T V^T) depend on the reflectors only, so
+    // they are formed on the second stream up to two blocks ahead, in two
+    // buffers, while this stream applies the blocks to Z:
+    //   Y1 = V^T Z,  Y2 = T Y1,  Z -= V Y2.
+    // bt_ready[u]: block in buffer u formed; bt_used[u]: its products with Z
+    // done, so the buffer may take the block after next.
+    // w_bt: V (n x BTNB) x 2, T (BTNB x BTNB) x 2, the Gram matrix, Y1 and
+    // Y2 (BTNB x m each), and the split-K workspace of the Gram products
+    constexpr size_t GWS = (size_t)32 * BTNB * BTNB;
+    if ((rc = c.w_bt.alloc((size_t)(2 * n + 2 * m + 3 * BTNB) * BTNB + GWS))) return rc;
+    double *vbuf[2], *tbuf[2];
+    vbuf[0] = c.w_bt.p;
+    vbuf[1] = vbuf[0] + (size_t)n * BTNB;
+    tbuf[0] = vbuf[1] + (size_t)n * BTNB;
+    tbuf[1] = tbuf[0] + BTNB * BTNB;
+    double *gm = tbuf[1] + BTNB * BTNB, *y1 = gm + BTNB * BTNB, *y2 = y1 + (size_t)BTNB * m;
+    double *gws = y2 + (size_t)BTNB * m;
     const int nref = n - 1;  // reflectors 0..n-2
-    const int last = nref > 0 ? ((nref - 1) / BTNB) * BTNB : -1;
-    for (int p0 = last; p0 >= 0; p0 -= BTNB) {
-      const int nb = std::min(BTNB, nref - p0);
-      if (nb <= 0) continue;
-      hipLaunchKernelGGL(k_vpanel, dim3((n + 255) / 256, nb), dim3(256), 0, s, n, lda, p0, nb, A, vb);
-      W2K_STAGE(s, "k_vpanel");
-      if ((rc = dgemm(s, 'T', 'N', nb, nb, n, 1.0, vb, n, vb, n, 0.0, gm, nb, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
+    const int nblk = nref > 0 ? (nref + BTNB - 1) / BTNB : 0;
+    hipStream_t sa = c.aux;
+    auto p0_of = [&](int q) { return (nblk - 1 - q) * BTNB; };
+    auto prep = [&](int q) -> int {
+      const int u = q & 1, p0 = p0_of(q), nb = std::min(BTNB, nref - p0);
+      if (q >= 2) W2K_HIP(hipStreamWaitEvent(sa, c.bt_used[u], 0));
+      hipLaunchKernelGGL(k_vpanel, dim3((n + 255) / 256, nb), dim3(256), 0, sa, n, lda, p0, nb, A, vbuf[u]);
+      W2K_STAGE(sa, "k_vpanel");
+      int r2;
+      if ((r2 = dgemm(sa, 'T', 'N', nb, nb, n, 1.0, vbuf[u], n, vbuf[u], n, 0.0, gm, nb, 0, 0, 0, gws, GWS)))
+        return r2;
+      hipLaunchKernelGGL(k_larft_inv, dim3(1), dim3(256), 0, sa, nb, gm, tau + p0, tbuf[u]);
+      W2K_STAGE(sa, "k_larft_inv");
+      W2K_HIP(hipEventRecord(c.bt_ready[u], sa));
+      return 0;
+    };
+    if (nblk > 0) {
+      // the second stream starts once the reflectors and Zt are in place
+      W2K_HIP(hipEventRecord(c.bt_start, s));
+      W2K_HIP(hipStreamWaitEvent(sa, c.bt_start, 0));
+      for (int q = 0; q < std::min(nblk, 2); q++)
+        if ((rc = prep(q))) return rc;
+    }
+    for (int q = 0; q < nblk; q++) {
+      const int u = q & 1, p0 = p0_of(q), nb = std::min(BTNB, nref - p0);
+      W2K_HIP(hipStreamWaitEvent(s, c.bt_ready[u], 0));
+      if ((rc = dgemm(s, 'T', 'N', nb, m, n, 1.0, vbuf[u], n, Zt, n, 0.0, y1, nb, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
         return rc;
-      hipLaunchKernelGGL(k_larft_inv, dim3(1), dim3(256), 0, s, nb, gm, tau + p0, tm);
-      W2K_STAGE(s, "k_larft_inv");
-      if ((rc = dgemm(s, 'T', 'N', nb, m, n, 1.0, vb, n, Zt, n, 0.0, y1, nb, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
-        return rc;
-      if ((rc = dgemm(s, 'N', 'N', nb, m, nb, 1.0, tm, nb, y1, nb, 0.0, y2, nb, 0, 1, 0))) return rc;
-      if ((rc = dgemm(s, 'N', 'N', n, m, nb, -1.0, vb, n, y2, nb, 1.0, Zt, n))) return rc;
+      if ((rc = dgemm(s, 'N', 'N', nb, m, nb, 1.0, tbuf[u], nb, y1, nb, 0.0, y2, nb, 0, 1, 0))) return rc;
+      if ((rc = dgemm(s, 'N', 'N', n, m, nb, -1.0, vbuf[u], n, y2, nb, 1.0, Zt, n))) return rc;
+      if (q + 2 < nblk) {
+        W2K_HIP(hipEventRecord(c.bt_used[u], s));
+        if ((rc = prep(q + 2))) return rc;
+      }
     }
     if ((rc = sort_eigen(c, B, m))) return rc;
     double *Zs = U;  // U is not needed after the inversion

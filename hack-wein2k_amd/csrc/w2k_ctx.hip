@@ -412,7 +412,14 @@ static int ctx_open(w2k_ctx **out, int device)
     delete x;
     return W2K_EDEVICE;
   }
+  if (hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipStreamDestroy(c.stream);
+    delete x;
+    return W2K_EDEVICE;
+  }
   for (int k = 0; k < W2K_NEVENTS; k++) (void)hipEventCreate(&c.ev[k]);
+  hipEvent_t *bt[5] = {&c.bt_start, &c.bt_ready[0], &c.bt_ready[1], &c.bt_used[0], &c.bt_used[1]};
+  for (hipEvent_t *e : bt) (void)hipEventCreateWithFlags(e, hipEventDisableTiming);
   *out = x;
   return 0;
 }
@@ -454,8 +461,12 @@ void w2k_destroy(w2k_ctx *x)
   drop_workers(x->c);
   (void)hipSetDevice(x->c.device);
   if (x->c.stream) (void)hipStreamSynchronize(x->c.stream);
+  if (x->c.aux) (void)hipStreamSynchronize(x->c.aux);
   if (x->c.tri_graph.exec) (void)hipGraphExecDestroy(x->c.tri_graph.exec);
   for (int k = 0; k < W2K_NEVENTS; k++) (void)hipEventDestroy(x->c.ev[k]);
+  for (hipEvent_t e : {x->c.bt_start, x->c.bt_ready[0], x->c.bt_ready[1], x->c.bt_used[0], x->c.bt_used[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (x->c.aux) (void)hipStreamDestroy(x->c.aux);
   for (hipEvent_t e : x->c.kev) (void)hipEventDestroy(e);
   if (x->c.stream) (void)hipStreamDestroy(x->c.stream);
   x->c.stream = nullptr;
@@ -837,10 +848,11 @@ extern "C" int w2k_test_dgemm_tiles(char ta, char tb, int m, int n, int k, doubl
 }
 
 extern "C" int w2k_test_dgemm_ws(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
-                                 const double *B, int ldb, double beta, double *C, int ldc, long wsn)
+                                 const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri,
+                                 int btri, long wsn)
 {
   if (wsn < 0) return W2K_EINVAL;
-  return test_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0, 0, 0, (size_t)wsn);
+  return test_dgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tri, atri, btri, (size_t)wsn);
 }
 
 extern "C" int w2k_test_split_gemm(int m, int n, int k, const double *A, int lda, const double *B, int ldb, double *C,

@@ -155,6 +155,11 @@ struct Ctx {
     int use = 1;       // 0 once a replay was slower than direct launches (or capture failed)
     double direct_ms = 0.0, graph_ms = 0.0;
   } tri_graph;
+  // second stream of the back-transform: the reflector panel, Gram matrix
+  // and T factor of the next blocks are formed there, ahead of the products
+  // with Z on `stream` (default priority, like `stream`)
+  hipStream_t aux = nullptr;
+  hipEvent_t bt_start = nullptr, bt_ready[2] = {nullptr, nullptr}, bt_used[2] = {nullptr, nullptr};
   // timing
   hipEvent_t ev[W2K_NEVENTS];
   double last_ms[W2K_NTIMINGS] = {0};

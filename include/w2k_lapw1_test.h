@@ -23,10 +23,13 @@ int w2k_test_dgemm_tiles(char ta, char tb, int m, int n, int k, double alpha, co
                          const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri,
                          int variant);
 
-/* The same dense product with a device workspace of wsn doubles, which lets
- * a product with few output tiles and a long K split K across the chip. */
+/* The same product with a device workspace of wsn doubles, which lets a
+ * product with few output tiles and a long K, or a product with a
+ * triangular operand or output and a long K, split K across the chip
+ * (tri / atri / btri as in w2k_test_dgemm). */
 int w2k_test_dgemm_ws(char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
-                      const double *B, int ldb, double beta, double *C, int ldc, long wsn);
+                      const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri,
+                      long wsn);
 
 /* Split-precision C (+)= A B^T (A: m x k, B: n x k, column-major FP64) with
  * `slices` INT8 digits per element (3..7), the kernel behind

@@ -51,16 +51,48 @@ def test_dgemm_split_k_matches_numpy(ta, tb, m, n, k):
     A = rng.standard_normal((m, k) if ta == "N" else (k, m))
     B = rng.standard_normal((k, n) if tb == "N" else (n, k))
     C = rng.standard_normal((m, n))
+    out = _dgemm_ws(ta, tb, m, n, k, 0.5, A, B, 2.0, C, 0, 0, 0, 1 << 22)
+    ref = 0.5 * (A if ta == "N" else A.T) @ (B if tb == "N" else B.T) + 2.0 * C
+    assert np.max(np.abs(out - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+def _dgemm_ws(ta, tb, m, n, k, alpha, A, B, beta, C, tri, atri, btri, wsn):
     lib = L.load_library()
     fn = lib.w2k_test_dgemm_ws
     dp = ctypes.POINTER(ctypes.c_double)
     fn.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                   dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int, ctypes.c_long]
+                   dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long]
     A = np.asfortranarray(A); B = np.asfortranarray(B); out = np.asfortranarray(C.copy())
-    rc = fn(ta.encode(), tb.encode(), m, n, k, 0.5, A.ctypes.data_as(dp), A.shape[0], B.ctypes.data_as(dp),
-            B.shape[0], 2.0, out.ctypes.data_as(dp), out.shape[0], 1 << 22)
+    rc = fn(ta.encode(), tb.encode(), m, n, k, alpha, A.ctypes.data_as(dp), A.shape[0], B.ctypes.data_as(dp),
+            B.shape[0], beta, out.ctypes.data_as(dp), out.shape[0], tri, atri, btri, wsn)
     assert rc == 0
-    ref = 0.5 * (A if ta == "N" else A.T) @ (B if tb == "N" else B.T) + 2.0 * C
+    return out
+
+
+@pytest.mark.parametrize("ta,tb,m,n,k,tri,atri,btri", [
+    ("T", "N", 1100, 1100, 2100, 2, 0, 0), ("T", "N", 1100, 1100, 2100, 1, 0, 0),
+    ("T", "N", 1300, 1250, 1300, 0, 2, 0), ("N", "N", 1300, 1250, 1300, 0, 1, 0),
+    ("N", "N", 1250, 1300, 1300, 0, 0, 1), ("T", "N", 1100, 1100, 4500, 0, 0, 0)])
+def test_dgemm_structured_split_k(ta, tb, m, n, k, tri, atri, btri):
+    """Products with a triangular operand or output, 256..4096 output tiles
+    and K >= 1024 are split into K slices of about 1024 (partials reduced in
+    a fixed order, only the output triangle written); the last case has no
+    structure and stays unsplit.  Elements outside an operand's triangle
+    hold values that must not be read, and those outside the output
+    triangle must keep their values."""
+    rng = np.random.default_rng(m + 7 * k + 3 * tri + 5 * atri + 11 * btri)
+    A = rng.standard_normal((m, k) if ta == "N" else (k, m))
+    B = rng.standard_normal((k, n) if tb == "N" else (n, k))
+    C = rng.standard_normal((m, n))
+    opA = A if ta == "N" else A.T
+    Am = np.triu(opA) if atri == 1 else (np.tril(opA) if atri == 2 else opA)
+    Bm = np.triu(B) if btri == 1 else (np.tril(B) if btri == 2 else B)
+    out = _dgemm_ws(ta, tb, m, n, k, -0.5, A, B, 1.5, C, tri, atri, btri, 3 * m * n)
+    ref = -0.5 * Am @ Bm + 1.5 * C
+    if tri:
+        keep = np.tril(np.ones((m, n), bool), -1) if tri == 2 else np.triu(np.ones((m, n), bool), 1)
+        ref[keep] = C[keep]
     assert np.max(np.abs(out - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
