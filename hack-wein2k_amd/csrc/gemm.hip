@@ -398,7 +398,7 @@ int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
   const long tiles = (long)grid.x * grid.y;
   const bool structured = tri || atri || btri;
   int nz = 0;
-  if (ws && tri == 0 && tiles < 256 && k >= 4 * 256)
+  if (ws && tri == 0 && tiles < 256 && k >= 2 * 256)
     nz = (int)std::min<long>((k + 255) / 256, std::max<long>(1, 1024 / tiles));
   else if (ws && structured && tiles >= 256 && tiles <= 4096 && k >= 1024)
     nz = std::min((k + 1023) / 1024, 4);

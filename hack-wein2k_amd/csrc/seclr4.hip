@@ -1709,7 +1709,11 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
       hipLaunchKernelGGL(k_vpanel, dim3((n + 255) / 256, nb), dim3(256), 0, sa, n, lda, p0, nb, A, vbuf[u]);
       W2K_STAGE(sa, "k_vpanel");
       int r2;
-      if ((r2 = dgemm(sa, 'T', 'N', nb, nb, n, 1.0, vbuf[u], n, vbuf[u], n, 0.0, gm, nb, 0, 0, 0, gws, GWS)))
+      // rows 0 .. p0 of the block's reflectors are zero: the products start at
+      // row p0+1 rounded down to 16 rows, which keeps the operands' 128-byte alignment
+      const int o = (p0 + 1) & ~15, nr = n - o;
+      if ((r2 = dgemm(sa, 'T', 'N', nb, nb, nr, 1.0, vbuf[u] + o, n, vbuf[u] + o, n, 0.0, gm, nb, 0, 0, 0, gws,
+                      GWS)))
         return r2;
       hipLaunchKernelGGL(k_larft_inv, dim3(1), dim3(256), 0, sa, nb, gm, tau + p0, tbuf[u]);
       W2K_STAGE(sa, "k_larft_inv");
@@ -1726,10 +1730,12 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
     for (int q = 0; q < nblk; q++) {
       const int u = q & 1, p0 = p0_of(q), nb = std::min(BTNB, nref - p0);
       W2K_HIP(hipStreamWaitEvent(s, c.bt_ready[u], 0));
-      if ((rc = dgemm(s, 'T', 'N', nb, m, n, 1.0, vbuf[u], n, Zt, n, 0.0, y1, nb, 0, 0, 0, c.w_gk.p, c.w_gk.n)))
+      const int o = (p0 + 1) & ~15, nr = n - o;  // the block's reflectors are zero above row p0+1
+      if ((rc = dgemm(s, 'T', 'N', nb, m, nr, 1.0, vbuf[u] + o, n, Zt + o, n, 0.0, y1, nb, 0, 0, 0, c.w_gk.p,
+                      c.w_gk.n)))
         return rc;
       if ((rc = dgemm(s, 'N', 'N', nb, m, nb, 1.0, tbuf[u], nb, y1, nb, 0.0, y2, nb, 0, 1, 0))) return rc;
-      if ((rc = dgemm(s, 'N', 'N', n, m, nb, -1.0, vbuf[u], n, y2, nb, 1.0, Zt, n))) return rc;
+      if ((rc = dgemm(s, 'N', 'N', nr, m, nb, -1.0, vbuf[u] + o, n, y2, nb, 1.0, Zt + o, n))) return rc;
       if (q + 2 < nblk) {
         W2K_HIP(hipEventRecord(c.bt_used[u], s));
         if ((rc = prep(q + 2))) return rc;

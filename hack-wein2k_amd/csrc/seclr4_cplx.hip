@@ -857,14 +857,20 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
       if (nb <= 0) continue;
       hipLaunchKernelGGL(k_zvpanel, dim3((n + 255) / 256, nb), dim3(256), 0, s, n, p0, nb, Hr, Hi, vbr, vbi);
       W2K_STAGE(s, "k_zvpanel");
-      if ((rc = zgemm(s, 'C', 'N', nb, nb, n, 1.0, vbr, vbi, n, vbr, vbi, n, 0.0, gmr, gmi, nb, 0, 0, 0, ws, wsn)))
+      // rows 0 .. p0 of the block's reflectors are zero: the products start at
+      // row p0+1 rounded down to 16 rows, which keeps the operands' 128-byte alignment
+      const int o = (p0 + 1) & ~15, nr = n - o;
+      if ((rc = zgemm(s, 'C', 'N', nb, nb, nr, 1.0, vbr + o, vbi + o, n, vbr + o, vbi + o, n, 0.0, gmr, gmi, nb, 0,
+                      0, 0, ws, wsn)))
         return rc;
       hipLaunchKernelGGL(k_zlarft, dim3(1), dim3(64), 0, s, nb, gmr, gmi, g.taur + p0, g.taui + p0, tmr, tmi);
       W2K_STAGE(s, "k_zlarft");
-      if ((rc = zgemm(s, 'C', 'N', nb, m, n, 1.0, vbr, vbi, n, Zt, Zi, n, 0.0, y1r, y1i, nb, 0, 0, 0, ws, wsn)))
+      if ((rc = zgemm(s, 'C', 'N', nb, m, nr, 1.0, vbr + o, vbi + o, n, Zt + o, Zi + o, n, 0.0, y1r, y1i, nb, 0, 0,
+                      0, ws, wsn)))
         return rc;
       if ((rc = zgemm(s, 'N', 'N', nb, m, nb, 1.0, tmr, tmi, nb, y1r, y1i, nb, 0.0, y2r, y2i, nb, 0, 1, 0))) return rc;
-      if ((rc = zgemm(s, 'N', 'N', n, m, nb, -1.0, vbr, vbi, n, y2r, y2i, nb, 1.0, Zt, Zi, n))) return rc;
+      if ((rc = zgemm(s, 'N', 'N', nr, m, nb, -1.0, vbr + o, vbi + o, n, y2r, y2i, nb, 1.0, Zt + o, Zi + o, n)))
+        return rc;
     }
     if ((rc = sort_eigen(c, B, m))) return rc;
     hipLaunchKernelGGL(k_zgather_cols, dim3((n + 255) / 256, m), dim3(256), 0, s, n, m, B.perm, Zt, Zi, Ur, Ui);

@@ -44,7 +44,8 @@ def test_dgemm_matches_numpy(ta, tb, m, n, k):
 
 
 @pytest.mark.parametrize("ta,tb", [("T", "N"), ("N", "N"), ("T", "T")])
-@pytest.mark.parametrize("m,n,k", [(32, 32, 5951), (32, 1184, 5951), (7, 33, 2049), (130, 20, 4097)])
+@pytest.mark.parametrize("m,n,k", [(32, 32, 5951), (32, 1184, 5951), (7, 33, 2049), (130, 20, 4097), (32, 195, 700),
+                                   (7, 33, 512)])
 def test_dgemm_split_k_matches_numpy(ta, tb, m, n, k):
     """Few output tiles and a long K: the split-K path (partials reduced in a fixed order)."""
     rng = np.random.default_rng(m + n + k)
