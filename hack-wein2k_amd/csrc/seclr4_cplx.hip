@@ -654,12 +654,15 @@ __global__ void k_zvpanel(int n, int p0, int nb, const double *__restrict__ ar, 
   vi[(size_t)r + (size_t)n * k] = xi;
 }
 
-// T of I - V T V^H from G = V^H V (ZLARFT 'F','C'): T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i)
+// T of I - V T V^H from G = V^H V (ZLARFT 'F','C'): T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i),
+// nb <= BN <= 64 (one thread per row)
+template <int BN>
 __global__ __launch_bounds__(64) void k_zlarft(int nb, const double *__restrict__ gr, const double *__restrict__ gi,
                                                const double *__restrict__ taur, const double *__restrict__ taui,
                                                double *__restrict__ tmr, double *__restrict__ tmi)
 {
-  __shared__ double gsr[TNB][TNB + 1], gsi[TNB][TNB + 1], tsr[TNB][TNB + 1], tsi[TNB][TNB + 1];
+  static_assert(BN <= 64, "one thread per row");
+  __shared__ double gsr[BN][BN + 1], gsi[BN][BN + 1], tsr[BN][BN + 1], tsi[BN][BN + 1];
   const int t = threadIdx.x;
   for (int idx = t; idx < nb * nb; idx += blockDim.x) {
     const int r = idx % nb, c = idx / nb;
@@ -845,15 +848,18 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
     const size_t zc = (size_t)n * m;
     hipLaunchKernelGGL(k_zfill, dim3((unsigned)((zc + 255) / 256)), dim3(256), 0, s, zc, Zi, 0.0);
     W2K_STAGE(s, "k_zfill");
-    if ((rc = c.w_bt.alloc((size_t)2 * (n + 2 * m + 2 * TNB) * TNB))) return rc;
-    double *vbr = c.w_bt.p, *vbi = vbr + (size_t)n * TNB;
-    double *gmr = vbi + (size_t)n * TNB, *gmi = gmr + TNB * TNB, *tmr = gmi + TNB * TNB, *tmi = tmr + TNB * TNB;
-    double *y1r = tmi + TNB * TNB, *y1i = y1r + (size_t)TNB * m, *y2r = y1i + (size_t)TNB * m;
-    double *y2i = y2r + (size_t)TNB * m;
+    // blocks of ZBT reflectors (the launches of a block, not its arithmetic,
+    // set the pace at these orders: 64-reflector blocks halve them)
+    constexpr int ZBT = 64;
+    if ((rc = c.w_bt.alloc((size_t)2 * (n + 2 * m + 2 * ZBT) * ZBT))) return rc;
+    double *vbr = c.w_bt.p, *vbi = vbr + (size_t)n * ZBT;
+    double *gmr = vbi + (size_t)n * ZBT, *gmi = gmr + ZBT * ZBT, *tmr = gmi + ZBT * ZBT, *tmi = tmr + ZBT * ZBT;
+    double *y1r = tmi + ZBT * ZBT, *y1i = y1r + (size_t)ZBT * m, *y2r = y1i + (size_t)ZBT * m;
+    double *y2i = y2r + (size_t)ZBT * m;
     const int nref = n - 1;
-    const int last = nref > 0 ? ((nref - 1) / TNB) * TNB : -1;
-    for (int p0 = last; p0 >= 0; p0 -= TNB) {
-      const int nb = std::min(TNB, nref - p0);
+    const int last = nref > 0 ? ((nref - 1) / ZBT) * ZBT : -1;
+    for (int p0 = last; p0 >= 0; p0 -= ZBT) {
+      const int nb = std::min(ZBT, nref - p0);
       if (nb <= 0) continue;
       hipLaunchKernelGGL(k_zvpanel, dim3((n + 255) / 256, nb), dim3(256), 0, s, n, p0, nb, Hr, Hi, vbr, vbi);
       W2K_STAGE(s, "k_zvpanel");
@@ -863,7 +869,7 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
       if ((rc = zgemm(s, 'C', 'N', nb, nb, nr, 1.0, vbr + o, vbi + o, n, vbr + o, vbi + o, n, 0.0, gmr, gmi, nb, 0,
                       0, 0, ws, wsn)))
         return rc;
-      hipLaunchKernelGGL(k_zlarft, dim3(1), dim3(64), 0, s, nb, gmr, gmi, g.taur + p0, g.taui + p0, tmr, tmi);
+      hipLaunchKernelGGL(k_zlarft<ZBT>, dim3(1), dim3(64), 0, s, nb, gmr, gmi, g.taur + p0, g.taui + p0, tmr, tmi);
       W2K_STAGE(s, "k_zlarft");
       if ((rc = zgemm(s, 'C', 'N', nb, m, nr, 1.0, vbr + o, vbi + o, n, Zt + o, Zi + o, n, 0.0, y1r, y1i, nb, 0, 0,
                       0, ws, wsn)))
