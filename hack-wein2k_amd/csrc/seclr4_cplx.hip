@@ -851,32 +851,71 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
     // blocks of ZBT reflectors (the launches of a block, not its arithmetic,
     // set the pace at these orders: 64-reflector blocks halve them)
     constexpr int ZBT = 64;
-    if ((rc = c.w_bt.alloc((size_t)2 * (n + 2 * m + 2 * ZBT) * ZBT))) return rc;
-    double *vbr = c.w_bt.p, *vbi = vbr + (size_t)n * ZBT;
-    double *gmr = vbi + (size_t)n * ZBT, *gmi = gmr + ZBT * ZBT, *tmr = gmi + ZBT * ZBT, *tmi = tmr + ZBT * ZBT;
-    double *y1r = tmi + ZBT * ZBT, *y1i = y1r + (size_t)ZBT * m, *y2r = y1i + (size_t)ZBT * m;
-    double *y2i = y2r + (size_t)ZBT * m;
+    // The reflector panel V, its Gram matrix and T depend on the reflectors
+    // only: they are formed on the second stream up to two blocks ahead, in
+    // two buffers, while this stream applies the blocks to Z (Y1 = V^H Z,
+    // Y2 = T Y1, Z -= V Y2).  bt_ready[u]: block in buffer u formed;
+    // bt_used[u]: its products with Z done.
+    // w_bt: V (re, im: n x ZBT each) x 2, T (re, im) x 2, the Gram matrix,
+    // Y1 and Y2 (re, im: ZBT x m each), the Gram products' split-K workspace
+    constexpr size_t GWS = (size_t)64 * ZBT * ZBT;
+    if ((rc = c.w_bt.alloc((size_t)2 * (2 * n + 3 * ZBT + 2 * m) * ZBT + GWS))) return rc;
+    double *vr[2], *vi[2], *tr[2], *ti[2];
+    double *q = c.w_bt.p;
+    for (int u = 0; u < 2; u++) {
+      vr[u] = q; vi[u] = q + (size_t)n * ZBT; q += (size_t)2 * n * ZBT;
+    }
+    for (int u = 0; u < 2; u++) {
+      tr[u] = q; ti[u] = q + ZBT * ZBT; q += 2 * ZBT * ZBT;
+    }
+    double *gmr = q, *gmi = q + ZBT * ZBT;
+    q += 2 * ZBT * ZBT;
+    double *y1r = q, *y1i = y1r + (size_t)ZBT * m, *y2r = y1i + (size_t)ZBT * m, *y2i = y2r + (size_t)ZBT * m;
+    double *gws = y2i + (size_t)ZBT * m;
     const int nref = n - 1;
-    const int last = nref > 0 ? ((nref - 1) / ZBT) * ZBT : -1;
-    for (int p0 = last; p0 >= 0; p0 -= ZBT) {
-      const int nb = std::min(ZBT, nref - p0);
-      if (nb <= 0) continue;
-      hipLaunchKernelGGL(k_zvpanel, dim3((n + 255) / 256, nb), dim3(256), 0, s, n, p0, nb, Hr, Hi, vbr, vbi);
-      W2K_STAGE(s, "k_zvpanel");
+    const int nblk = nref > 0 ? (nref + ZBT - 1) / ZBT : 0;
+    hipStream_t sa = c.aux;
+    auto p0_of = [&](int k) { return (nblk - 1 - k) * ZBT; };
+    auto prep = [&](int k) -> int {
+      const int u = k & 1, p0 = p0_of(k), nb = std::min(ZBT, nref - p0);
+      if (k >= 2) W2K_HIP(hipStreamWaitEvent(sa, c.bt_used[u], 0));
+      hipLaunchKernelGGL(k_zvpanel, dim3((n + 255) / 256, nb), dim3(256), 0, sa, n, p0, nb, Hr, Hi, vr[u], vi[u]);
+      W2K_STAGE(sa, "k_zvpanel");
       // rows 0 .. p0 of the block's reflectors are zero: the products start at
       // row p0+1 rounded down to 16 rows, which keeps the operands' 128-byte alignment
       const int o = (p0 + 1) & ~15, nr = n - o;
-      if ((rc = zgemm(s, 'C', 'N', nb, nb, nr, 1.0, vbr + o, vbi + o, n, vbr + o, vbi + o, n, 0.0, gmr, gmi, nb, 0,
+      int r2;
+      if ((r2 = zgemm(sa, 'C', 'N', nb, nb, nr, 1.0, vr[u] + o, vi[u] + o, n, vr[u] + o, vi[u] + o, n, 0.0, gmr, gmi,
+                      nb, 0, 0, 0, gws, GWS)))
+        return r2;
+      hipLaunchKernelGGL(k_zlarft<ZBT>, dim3(1), dim3(64), 0, sa, nb, gmr, gmi, g.taur + p0, g.taui + p0, tr[u],
+                         ti[u]);
+      W2K_STAGE(sa, "k_zlarft");
+      W2K_HIP(hipEventRecord(c.bt_ready[u], sa));
+      return 0;
+    };
+    if (nblk > 0) {
+      // the second stream starts once the reflectors and Zt are in place
+      W2K_HIP(hipEventRecord(c.bt_start, s));
+      W2K_HIP(hipStreamWaitEvent(sa, c.bt_start, 0));
+      for (int k = 0; k < std::min(nblk, 2); k++)
+        if ((rc = prep(k))) return rc;
+    }
+    for (int k = 0; k < nblk; k++) {
+      const int u = k & 1, p0 = p0_of(k), nb = std::min(ZBT, nref - p0);
+      const int o = (p0 + 1) & ~15, nr = n - o;  // the block's reflectors are zero above row p0+1
+      W2K_HIP(hipStreamWaitEvent(s, c.bt_ready[u], 0));
+      if ((rc = zgemm(s, 'C', 'N', nb, m, nr, 1.0, vr[u] + o, vi[u] + o, n, Zt + o, Zi + o, n, 0.0, y1r, y1i, nb, 0,
                       0, 0, ws, wsn)))
         return rc;
-      hipLaunchKernelGGL(k_zlarft<ZBT>, dim3(1), dim3(64), 0, s, nb, gmr, gmi, g.taur + p0, g.taui + p0, tmr, tmi);
-      W2K_STAGE(s, "k_zlarft");
-      if ((rc = zgemm(s, 'C', 'N', nb, m, nr, 1.0, vbr + o, vbi + o, n, Zt + o, Zi + o, n, 0.0, y1r, y1i, nb, 0, 0,
-                      0, ws, wsn)))
+      if ((rc = zgemm(s, 'N', 'N', nb, m, nb, 1.0, tr[u], ti[u], nb, y1r, y1i, nb, 0.0, y2r, y2i, nb, 0, 1, 0)))
         return rc;
-      if ((rc = zgemm(s, 'N', 'N', nb, m, nb, 1.0, tmr, tmi, nb, y1r, y1i, nb, 0.0, y2r, y2i, nb, 0, 1, 0))) return rc;
-      if ((rc = zgemm(s, 'N', 'N', nr, m, nb, -1.0, vbr + o, vbi + o, n, y2r, y2i, nb, 1.0, Zt + o, Zi + o, n)))
+      if ((rc = zgemm(s, 'N', 'N', nr, m, nb, -1.0, vr[u] + o, vi[u] + o, n, y2r, y2i, nb, 1.0, Zt + o, Zi + o, n)))
         return rc;
+      if (k + 2 < nblk) {
+        W2K_HIP(hipEventRecord(c.bt_used[u], s));
+        if ((rc = prep(k + 2))) return rc;
+      }
     }
     if ((rc = sort_eigen(c, B, m))) return rc;
     hipLaunchKernelGGL(k_zgather_cols, dim3((n + 255) / 256, m), dim3(256), 0, s, n, m, B.perm, Zt, Zi, Ur, Ui);
