@@ -176,6 +176,12 @@ __global__ void k_zfill(size_t cnt, double *p, double v)
   if (i < cnt) p[i] = v;
 }
 
+__global__ void k_negate(size_t cnt, double *p)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) p[i] = -p[i];
+}
+
 // Hermitian completion: C(j,i) = conj C(i,j) for i > j, real diagonal
 __global__ void k_zmirror_lower(int n, double *Cr, double *Ci)
 {
@@ -735,15 +741,19 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
     dim3 g2((n + 255) / 256, n);
     hipLaunchKernelGGL(k_zprep, g2, dim3(256), 0, s, n, c.d_hs.p, c.d_hsdiag.p, Ur, Ui, Hr, Hi);
     W2K_STAGE(s, "k_zprep");
+    // W: the diagonal-block inverses U_kk^-1 (zero elsewhere), used by the
+    // row panels here and, conjugated, by the reduction
+    hipLaunchKernelGGL(k_zfill, dim3((unsigned)((2 * nn + 255) / 256)), dim3(256), 0, s, 2 * nn, Wr, 0.0);
+    W2K_STAGE(s, "k_zfill");
     for (int k0 = 0; k0 < n; k0 += NB) {
       const int kb = std::min(NB, n - k0);
       hipLaunchKernelGGL(k_zpotf2, dim3(1), dim3(256), 0, s, n, k0, kb, Ur, Ui, B.info);
       W2K_STAGE(s, "k_zpotf2");
+      hipLaunchKernelGGL(k_ztrtri_diag, dim3(1), dim3(256), 0, s, n, k0, kb, Ur, Ui, 0, Wr, Wi);
+      W2K_STAGE(s, "k_ztrtri_diag");
       const int rest = n - k0 - kb;
       if (rest > 0) {
-        // U12 = U11^-H S12 with U11^-1 in W (free until the reduction)
-        hipLaunchKernelGGL(k_ztrtri_diag, dim3(1), dim3(256), 0, s, n, k0, kb, Ur, Ui, 0, Wr, Wi);
-        W2K_STAGE(s, "k_ztrtri_diag");
+        // U12 = U11^-H S12 with U11^-1 from W
         const size_t o11 = (size_t)k0 + (size_t)n * k0, o12 = (size_t)k0 + (size_t)n * (k0 + kb);
         if ((rc = zgemm(s, 'C', 'N', kb, rest, kb, 1.0, Wr + o11, Wi + o11, n, Ur + o12, Ui + o12, n, 0.0, Xr, Xi, kb,
                         0, 2, 0)))
@@ -766,12 +776,13 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
   if (hinfo[0] != 0) return hinfo[0];  // ZPOTRF INFO, seclr4.F:200-205
   // ---- 2. reduction with V = conj(U): W = V^-1, X = H W, C = W^H X
   {
-    hipLaunchKernelGGL(k_zfill, dim3((unsigned)((2 * nn + 255) / 256)), dim3(256), 0, s, 2 * nn, Wr, 0.0);
-    W2K_STAGE(s, "k_zfill");
+    // the diagonal blocks of V^-1 = conj(U^-1): conjugate the Cholesky's
+    // U_kk^-1 (the kernel's arithmetic mirrors exactly under conjugation, so
+    // this equals inverting conj(U_kk))
+    hipLaunchKernelGGL(k_negate, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nn, Wi);
+    W2K_STAGE(s, "k_negate");
     for (int j0 = 0; j0 < n; j0 += NB) {
       const int kb = std::min(NB, n - j0);
-      hipLaunchKernelGGL(k_ztrtri_diag, dim3(1), dim3(256), 0, s, n, j0, kb, Ur, Ui, 1, Wr, Wi);
-      W2K_STAGE(s, "k_ztrtri_diag");
       if (j0 > 0) {
         // T = W(0:j0,0:j0) conj(U)(0:j0, j0:j0+kb); W(0:j0, j0:) = -T W_jj
         const size_t oc = (size_t)n * j0, ojj = (size_t)j0 + (size_t)n * j0;
