@@ -264,6 +264,26 @@ def test_seclr4_not_positive_definite_reports_info():
     ctx.close()
 
 
+@pytest.mark.parametrize("bad", [5, 70])
+def test_seclr4_complex_not_positive_definite_reports_info(bad):
+    """ZPOTRF INFO from the complex block Cholesky: a non-positive pivot in
+    the first or the second 64-column block reports its 1-based column."""
+    n = 100
+    rng = np.random.default_rng(4 + bad)
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    S = X @ X.conj().T / n + np.eye(n)
+    H = np.eye(n, dtype=complex)
+    hs, hd = _pack(H, S)
+    hd[bad] = -1.0
+    ctx = L.Lapw1(Fixture("lo_cplx").run())
+    ctx.set_basis(_basis_of_size(n))
+    ctx.put_hs(hs, hd)
+    with pytest.raises(L.Lapw1Error) as ei:
+        ctx.seclr4(-1e9, 1e9, n)
+    assert ei.value.code == bad + 1
+    ctx.close()
+
+
 def test_c2_rocksalt_chain_matches_reference():
     """64-atom rocksalt cell at RKMAX 7 (N = 5951): the full device chain
     against the reference eigenvalues, NBELW and the first/last vectors."""
