@@ -882,7 +882,7 @@ __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__
                                               double *__restrict__ z, int ldz, double *__restrict__ u0,
                                               double *__restrict__ u1, double *__restrict__ u2,
                                               double *__restrict__ mul, double *__restrict__ x,
-                                              int *__restrict__ piv)
+                                              int *__restrict__ piv, double dtscale)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= m) return;
@@ -941,24 +941,37 @@ __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__
       }
     }
   }
+  double ulast;  // |U(bs,bs)| after the perturbation (DSTEIN's scale of the start vector)
   {
     const size_t o = at(bs - 1);
     double p0 = a0;
     if (fabs(p0) < tol) p0 = (p0 < 0 ? -tol : tol);
     u0[o] = 1.0 / p0; u1[o] = 0.0; u2[o] = 0.0;
+    ulast = fabs(p0);
   }
-  // deterministic start vector in [-1, 1), then three solves
+  // deterministic start vector in [-1, 1), then solves until DSTEIN's test
+  // (dstein.f): the input scaled to 1-norm bs |T|_1 max(eps, |U(bs,bs)|), a
+  // solve passes when the largest entry of its solution reaches
+  // sqrt(0.1 / bs), and the vector is accepted after EXTRA + 1 = 3 passing
+  // solves; at most MAXITS = 5 solves, after which the vector is counted as
+  // not converged (INFO > 0, dsyevx2.f:352-354).  A first solve that passes
+  // (the usual case) gives exactly three solves.
+  constexpr int MAXITS = 5, EXTRA = 2;
+  const double dtpcrt = sqrt(0.1 / (double)bs) * dtscale;
   unsigned long long seed = 0x9E3779B97F4A7C15ull ^ ((unsigned long long)(t + 1) * 0xD1B54A32D192ED03ull);
   auto rnd = [&]() {
     seed = seed * 6364136223846793005ull + 1442695040888963407ull;
     return ((double)(seed >> 11) / 9007199254740992.0) * 2.0 - 1.0;
   };
   double sc = 1.0;
-  for (int it = 0; it < 3; it++) {
+  int nrmchk = 0;
+  bool conv = false;
+  for (int it = 0; it < MAXITS; it++) {
     const bool first = (it == 0);
     // forward: L^-1 P x, x scaled by 1/max|x| of the previous solve; the
     // groups start at even rows, and row k reads x(k+1) before writing x(k)
     double xk = first ? rnd() : sc * x[at(0)];
+    double l1 = fabs(xk);  // 1-norm of this solve's input
     int k = 0;
     for (; k + SG <= bs - 1; k += SG) {
       int2 pg[SG / 2];
@@ -976,6 +989,7 @@ __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__
       for (int q = 0; q < SG; q++) {
         const double xr = (q & 1) ? xg[(q + 1) / 2].x : xg[q / 2].y;  // row k+q+1
         double xn = first ? rnd() : sc * xr;
+        l1 += fabs(xn);
         const int pv = (q & 1) ? pg[q / 2].y : pg[q / 2].x;
         const double mm = (q & 1) ? mg[q / 2].y : mg[q / 2].x;
         if (pv) { const double tmp = xk; xk = xn; xn = tmp; }
@@ -985,21 +999,25 @@ __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__
     }
     for (; k < bs - 1; k++) {
       double xn = first ? rnd() : sc * x[at(k + 1)];
+      l1 += fabs(xn);
       if (piv[at(k)]) { const double tmp = xk; xk = xn; xn = tmp; }
       x[at(k)] = xk;
       xk = xn - mul[at(k)] * xk;
     }
     x[at(bs - 1)] = xk;
-    // backward: U^-1, groups of SG rows ending at an odd row
+    // backward: U^-1, groups of SG rows ending at an odd row.  A solve that
+    // may be the last (two passing solves behind it, or the last allowed)
+    // writes Z; one that may be followed by another writes x.
     double mx = 0.0, x1 = 0.0, x2 = 0.0;
-    const bool lastit = (it == 2);
+    const bool wz = (nrmchk == EXTRA) || (it == MAXITS - 1);
+    const bool wx = !wz;
     auto row = [&](int kk, double xv, double r0, double b1, double b2) {
       double v = xv;
       if (kk + 1 < bs) v -= b1 * x1;
       if (kk + 2 < bs) v -= b2 * x2;
       v *= r0;
-      if (lastit) zc[ib + kk] = v;
-      else x[at(kk)] = v;
+      if (wz) zc[ib + kk] = v;
+      if (wx) x[at(kk)] = v;
       mx = fmax(mx, fabs(v));
       x2 = x1;
       x1 = v;
@@ -1027,8 +1045,20 @@ __global__ __launch_bounds__(64) void k_stein(BisArgs g, int m, const double *__
       }
     }
     for (; k >= 0; k--) row(k, x[at(k)], u0[at(k)], u1[at(k)], u2[at(k)]);
+    // DSTEIN's growth test on the solution of the scaled input
+    const double scl = (double)bs * onenrm * fmax(DBL_EPSILON, ulast) / l1;
+    if (mx * scl >= dtpcrt && ++nrmchk >= EXTRA + 1) {
+      conv = true;
+      break;
+    }
+    if (wz && it < MAXITS - 1) {
+      // a candidate last solve failed: it wrote only Z, and the next solve
+      // reads its input back from there (the same values)
+      for (int kk = 0; kk < bs; kk++) x[at(kk)] = zc[ib + kk];
+    }
     sc = 1.0 / mx;
   }
+  if (!conv) atomicAdd(g.cnt + 3, 1);
 }
 
 // orthonormalise columns c0..c0+cw-1 of z in order (MGS), one work-group
@@ -1325,14 +1355,25 @@ int tridiag_eigen(Ctx &c, Seclr4Bufs &B, double el, double eu, int nume, double 
   const size_t zp = (size_t)((n + 2) / 2) * 2 * m;
   if ((rc = c.w_fac.alloc((size_t)5 * zp)) || (rc = c.w_piv.alloc(zp))) return rc;
   {
+    // W2K_TEST_STEIN_FAIL=1 (tests only) raises DSTEIN's growth threshold
+    // out of reach, so that every vector takes the non-convergence path
+    const bool force_fail = std::getenv("W2K_TEST_STEIN_FAIL") && std::atoi(std::getenv("W2K_TEST_STEIN_FAIL"));
     double *f = c.w_fac.p;
     hipLaunchKernelGGL(k_stein, dim3((m + 63) / 64), dim3(64), 0, s, b, m, B.xs, Zt, n, f, f + zp, f + 2 * zp,
-                       f + 3 * zp, f + 4 * zp, c.w_piv.p);
+                       f + 3 * zp, f + 4 * zp, c.w_piv.p, force_fail ? 1.0e300 : 1.0);
     W2K_STAGE(s, "k_stein");
   }
   std::vector<int> cs(m);
+  int nfail = 0;
   W2K_HIP(hipMemcpyAsync(cs.data(), B.cstart, sizeof(int) * m, hipMemcpyDeviceToHost, s));
+  W2K_HIP(hipMemcpyAsync(&nfail, B.cnt + 3, sizeof(int), hipMemcpyDeviceToHost, s));
   W2K_HIP(hipStreamSynchronize(s));
+  if (nfail > 0) {
+    // DSTEIN's INFO: the number of vectors that did not converge in MAXITS
+    // solves (dsyevx2.f:352-354); SECLR4 stops on it (seclr4.F:304-309)
+    set_last_error("SYEVX: %d eigenvectors failed to converge in inverse iteration", nfail);
+    return nfail;
+  }
   // clusters [t0, t1): chunks of CW vectors, each orthogonalised against
   // the cluster's earlier chunks (classical Gram-Schmidt, twice, on MFMA)
   // and then within itself by CholQR2 (Z = Q R, R from the Cholesky of

@@ -1,6 +1,7 @@
 // Context, per-run tables and the extern "C" entry points of libw2klapw1.so.
 #include <algorithm>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -22,6 +23,15 @@ int w2k_hip_fail(hipError_t e, const char *what, const char *file, int line)
            base ? base + 1 : file, line);
   if (getenv("W2K_VERBOSE")) fprintf(stderr, "w2k: %s\n", g_last_error);
   return e == hipErrorOutOfMemory ? W2K_ENOMEM : W2K_EDEVICE;
+}
+
+void set_last_error(const char *fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof g_last_error, fmt, ap);
+  va_end(ap);
+  if (getenv("W2K_VERBOSE")) fprintf(stderr, "w2k: %s\n", g_last_error);
 }
 
 // W2K_SYNC_CHECK=1: synchronise after every kernel so a device fault is
@@ -114,8 +124,15 @@ int run_upload(Ctx &c, const w2k_run_desc *r)
   RunDev &R = c.run();
   hipStream_t s = c.stream;
   if (!r || r->nat < 1 || r->ndf < r->nat || r->ltop < 3 || r->ltop > W2K_LMAX || r->lmaxd != W2K_LMAX ||
-      r->nnsl < 1 || r->nnsl > W2K_NSLMAX)
+      r->nnsl < 1 || r->nnsl > W2K_NSLMAX || r->nnsl > r->ltop)
     return W2K_EINVAL;
+  // the HNS and LO kernels read AL/BL at l < NNSL and at every LO's l: both
+  // must lie inside the (n, ltop, nat) table
+  for (int a = 0; a < r->nat; a++)
+    for (int l = 0; l <= W2K_LOMAX; l++)
+      if (r->ilo[l + (W2K_LOMAX + 1) * a] < 0 || r->ilo[l + (W2K_LOMAX + 1) * a] > W2K_NLOAT ||
+          (r->ilo[l + (W2K_LOMAX + 1) * a] > 0 && l >= r->ltop))
+        return W2K_EINVAL;
   R.is_complex = r->is_complex ? 1 : 0;
   R.nat = r->nat;
   R.ndf = r->ndf;
@@ -357,6 +374,25 @@ void take_ms(Ctx &c, int slot, int e0, int e1)
   c.last_ms[slot] = ms;
 }
 
+// every per-k-point buffer and cached key of a context (not the per-run
+// tables, streams or events); the streams must be idle
+void drop_kpoint_state(Ctx &c)
+{
+  DBuf<double> *bufs[] = {&c.d_kvec, &c.d_hs, &c.d_hsdiag, &c.d_al, &c.d_bl, &c.d_cs, &c.d_theta, &c.d_colcoef,
+                          &c.d_hnsL, &c.d_hnsR, &c.h_small, &c.h_re, &c.h_im, &c.d_eig, &c.d_z, &c.w_a,
+                          &c.w_b, &c.w_c, &c.w_v, &c.w_x, &c.w_small, &c.w_fac, &c.w_bt, &c.w_cl, &c.w_gk};
+  for (DBuf<double> *b : bufs) b->release();
+  c.d_kzz.release();
+  c.h_int.release();
+  c.w_int.release();
+  c.w_piv.release();
+  c.w_dig.release();
+  if (c.tri_graph.exec) (void)hipGraphExecDestroy(c.tri_graph.exec);
+  c.tri_graph = Ctx::TriGraph();
+  c.have_basis = c.have_hamilt = c.have_hns_ops = 0;
+  c.ne = c.nbelw = 0;
+}
+
 static int timed_begin(Ctx &c, int slot) { W2K_HIP(hipEventRecord(c.ev[2 * slot], c.stream)); return 0; }
 static int timed_end(Ctx &c, int slot)
 {
@@ -481,6 +517,16 @@ int w2k_set_basis(w2k_ctx *x, int nv, int nlo, const int *kzz, const double *xk,
   if (nlo != 0 && nlo != c.run().nlo_cols) return W2K_EINVAL;
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
   const int n = nv + nlo;
+  if (n > c.n_cap) {
+    // a basis larger than any this context has held: drain both streams and
+    // release every per-k buffer and cached key, so that the k-point runs
+    // exactly as on a fresh context (smaller bases reuse the buffers, as the
+    // batched path's streams do)
+    W2K_HIP(hipStreamSynchronize(c.stream));
+    W2K_HIP(hipStreamSynchronize(c.aux));
+    drop_kpoint_state(c);
+    c.n_cap = n;
+  }
   c.nv = nv;
   c.nlo = nlo;
   c.n = n;

@@ -28,6 +28,8 @@ struct Status {
   } while (0)
 
 int w2k_hip_fail(hipError_t e, const char *what, const char *file, int line);
+// message for w2k_last_error() of a call that returns a positive INFO
+void set_last_error(const char *fmt, ...);
 // W2K_SYNC_CHECK=1 in the environment: synchronise after every kernel
 int sync_checks_enabled();
 
@@ -112,6 +114,7 @@ struct Ctx {
   std::vector<w2k_ctx *> workers;
   // basis
   int nv = 0, nlo = 0, n = 0;
+  int n_cap = 0;  // largest basis since the per-k buffers were last dropped
   int have_basis = 0, have_hamilt = 0;
   DBuf<int> d_kzz;          // [n][3]
   std::vector<int> h_kzz;   // host copy of KZZ (index bounds for the adjoint)
@@ -181,6 +184,8 @@ int adjoint_gamma(Ctx &c, const double *d_f, int rows, double *Gr, double *Gi);
 // band-energy adjoint with respect to the VNS radial integrals (after a
 // CALKPT that ran HNS); g[0..5] = dE/dVNS1..3, dE/dVNS1LO..3LO (host, desc layout)
 int adjoint_vns(Ctx &c, const double *f_host, double *const g[6]);
+// release every per-k-point buffer and cached key (streams idle)
+void drop_kpoint_state(Ctx &c);
 // elapsed milliseconds between events e0 and e1 into last_ms[slot]
 void take_ms(Ctx &c, int slot, int e0, int e1);
 

@@ -19,9 +19,29 @@ from oracle_lib import load_package  # noqa: E402
 
 PKG = load_package()
 
+# W2K_TEST_EMU=1: the `gpu` tests run on the CPU build of the library's HIP
+# sources (tests/cpuemu; make -C tests/cpuemu) instead of the device library,
+# so that the whole device test sequence can be replayed on the CPU with every
+# device buffer against a guard page.  Tests marked `big` (cells too large for
+# the emulation, or programs linked to the device library) are skipped then.
+EMU = os.environ.get("W2K_TEST_EMU") == "1"
+EMU_LIB = os.path.join(REPO, "tests", "cpuemu", "_build", "libw2klapw1_emu.so")
+if EMU:
+    PKG.lapw1._LIB = PKG.lapw1.load_library(path=EMU_LIB)
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); run with -m gpu")
+    config.addinivalue_line("markers", "big: too large for the CPU emulation of the device library")
+
+
+def pytest_collection_modifyitems(config, items):
+    if not EMU:
+        return
+    skip = pytest.mark.skip(reason="too large for the CPU emulation (W2K_TEST_EMU=1)")
+    for it in items:
+        if "big" in it.keywords:
+            it.add_marker(skip)
 
 
 class Fixture:

@@ -50,12 +50,20 @@ RUN_KEYS_ARR = ["MULT", "POS", "RMT", "V", "P", "PE", "DP", "DPE", "PEI", "E", "
                 "PEILO", "PI12LO", "PE12LO", "ROTLOC", "ROTIJ", "LQIND"]
 
 
+def _unlimited_stack():
+    import resource
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
 def run_reference(workdir: str, complex_: bool, threads: int = 8, dump: bool = True, mode: str = "") -> float:
     exe = os.path.join(REF_DIR, "lapw1_complex" if complex_ else "lapw1_real")
     env = dict(os.environ, OPENBLAS_NUM_THREADS=str(threads))
     t0 = time.time()
     args = [exe] + ([mode] if mode else ([] if dump else ["nodump"]))
-    r = subprocess.run(args, cwd=workdir, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    # the reference keeps automatic arrays on the stack that grow with the
+    # number of atoms (the 100-atom cell overflows the default 8 MiB)
+    r = subprocess.run(args, cwd=workdir, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       preexec_fn=_unlimited_stack)
     dt = time.time() - t0
     if r.returncode != 0 or not os.path.exists(os.path.join(workdir, "ref_times.txt")):
         sys.stdout.write(r.stdout.decode(errors="replace")[-3000:])
@@ -187,6 +195,37 @@ def make_mgo(tmp: str):
     save_case("mgo", d, 1, False, sec, extra)
 
 
+def make_c3(name: str, workdir: str, seconds: float, frac: float = 0.2):
+    """Fixture of the 100-atom complex cell from one wide-window run: the
+    window is narrowed to the lowest `frac` of the spectrum after the fact."""
+    run = read_dump(os.path.join(workdir, "run.bin"))
+    payload = {f"run/{k}": v for k, v in crop_run(run).items()}
+    kd = read_dump(os.path.join(workdir, "k0001.bin"))
+    n = int(kd["HSROWS"])
+    ev = np.atleast_1d(kd["EIGVAL"])[: int(kd["NE"])]
+    target = int(round(frac * n))
+    assert len(ev) > target, (len(ev), target)
+    kd["EU"] = np.asarray(0.5 * (ev[target - 1] + ev[target]))
+    kd["NE"] = np.asarray(target)
+    rng = np.random.Generator(np.random.PCG64(W.SEED_BASE + 999))
+    out = kdata(kd, False, rng)
+    out["Z_LAST"] = kd["Z"][:, target - 1]
+    out["Z_MID"] = kd["Z"][:, target // 2]
+    out["Z_MID_INDEX"] = np.asarray(target // 2)
+    for k, v in out.items():
+        payload[f"k0/{k}"] = v
+    payload["meta/nk"] = np.asarray(1)
+    payload["meta/complex"] = np.asarray(1)
+    payload["meta/ref_seconds"] = np.asarray(seconds)
+    times = dict(l.split() for l in open(os.path.join(workdir, "ref_times.txt")))
+    for k, v in times.items():
+        payload[f"meta/{k}"] = np.asarray(float(v))
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **payload)
+    print(f"{name}: N={n}, NE={target}, EU={float(kd['EU']):.6f}, {os.path.getsize(path)/1024:.0f} KiB, "
+          f"reference {seconds:.1f} s")
+
+
 def main(argv):
     names = argv or ["mgo", "c1_fcc", "lo_mult2", "lo_cplx", "rs8"]
     tmp = tempfile.mkdtemp(prefix="w2kfix_")
@@ -212,6 +251,16 @@ def main(argv):
                 W.write_case(case, d)
                 sec = run_reference(d, True, mode="basis")
                 save_case(name, d, 0, True, sec)
+                continue
+            if name == "c3_lowsym100":
+                # one pass with a wide window capped at NUME (4000, the lowest
+                # eigenvalues: the tridiagonal matrix does not split), then
+                # the window (EL, EU'] of the lowest ~20% of the spectrum is
+                # cut from it: same matrices, same eigenvalues (bisection to
+                # ABSTOL = 0 does not depend on the window)
+                W.write_case(case, d)
+                sec = run_reference(d, True)
+                make_c3(name, d, sec)
                 continue
             if name == "c2_rs64":
                 # window chosen so that the lowest ~20% of the spectrum is

@@ -480,6 +480,49 @@ def case_c4_ortho8(nk: int = 64) -> Case:
                 ops_identity(), rkmax=7.0, kpoints=ks, window=(-9.0, 12.0), seed=SEED_BASE + 3, complex=True)
 
 
+def _random_positions(rng: np.random.Generator, alat, radii, margin: float) -> List[Tuple[float, float, float]]:
+    """Seeded fractional positions, one per radius, with every periodic
+    distance at least the sum of the two radii plus `margin` (random
+    sequential addition; the cells here pack to about a quarter)."""
+    pos: List[Tuple[float, float, float]] = []
+    A = np.asarray(alat, float)
+    for r in radii:
+        for _ in range(100000):
+            f = rng.uniform(0.0, 1.0, size=3)
+            if pos:
+                d = np.asarray(pos) - f
+                d -= np.round(d)
+                dist = np.linalg.norm(d * A, axis=1)
+                if np.any(dist < np.asarray(radii[:len(pos)]) + r + margin):
+                    continue
+            pos.append(tuple(float(x) for x in f))
+            break
+        else:
+            raise RuntimeError("no room for another sphere")
+    return pos
+
+
+def case_c3_lowsym100(rkmax: float = 6.5, name: str = "c3_lowsym100") -> Case:
+    """Split-precision cell: 100 atoms of 3 species (34 x Z=22 RMT 2.0,
+    33 x Z=12 RMT 1.8, 33 x Z=8 RMT 1.6) at seeded random positions in an
+    orthorhombic P cell with no symmetry but the identity (so the complex
+    build), every atom its own type, APW+lo on l = 0, 1 (4 local orbitals per
+    atom), RKmax 6.5, lmax 10, lnsmax 4, one general k-point; N about 12400."""
+    rng = np.random.Generator(np.random.PCG64(SEED_BASE + 2))
+    alat = (21.2, 22.0, 22.8)
+    species = [("Ti", 22.0, 2.0)] * 34 + [("Mg", 12.0, 1.8)] * 33 + [("O", 8.0, 1.6)] * 33
+    order = rng.permutation(len(species))
+    species = [species[i] for i in order]
+    pos = _random_positions(rng, alat, [s[2] for s in species], 0.05)
+    vlms = [[(1, 0), (2, 0), (2, 2), (3, 1), (4, 0)], [(1, 1), (2, -1), (3, 0), (4, 2)],
+            [(2, 1), (3, -2), (4, 4)]]
+    atoms = [AtomType(nm, z, r, [p], apw_lo=[0, 1], vlm=vlms[i % 3])
+             for i, ((nm, z, r), p) in enumerate(zip(species, pos))]
+    return Case(name, "100-atom cell, 3 species, seeded positions, no symmetry", "P", alat, atoms,
+                ops_identity(), rkmax=rkmax, kpoints=[(1, 3, 2, 8)], window=(-9.0, 12.0), seed=SEED_BASE + 2,
+                complex=True)
+
+
 def _with_window(c: Case, w) -> Case:
     c.window = w
     return c
@@ -492,4 +535,5 @@ CASES = {
     "rs8": lambda: case_rocksalt(1, 5.0, "rs8", SEED_BASE + 21),
     "c2_rs64": lambda: _with_window(case_rocksalt(2, 7.0, "c2_rs64", SEED_BASE + 1), (-9.0, 12.0)),
     "c4_ortho8": case_c4_ortho8,
+    "c3_lowsym100": case_c3_lowsym100,
 }

@@ -4,15 +4,18 @@ FP64 MFMA GEMM against numpy).  Bar: H and S elementwise to 1e-12 relative
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 import pytest
 
-from conftest import Fixture, relerr, CASES, PKG
+from conftest import GOLDEN, Fixture, relerr, CASES, PKG
 
 pytestmark = pytest.mark.gpu
 L = PKG.lapw1
 TOL = 1e-12
+# cells stored as sampled entries and sums (the matrices are hundreds of MB)
+LARGE = [c for c in ("c2_rs64", "c3_lowsym100") if os.path.exists(os.path.join(GOLDEN, f"{c}.npz"))]
 
 
 def _dgemm(ta, tb, m, n, k, alpha, A, B, beta, C, tri=0, atri=0, btri=0):
@@ -249,6 +252,50 @@ def test_hns_matches_reference(name):
     assert relerr(np.tril(hs), np.tril(ref)) < TOL, "H after HNS"
     assert relerr(np.triu(hs, 1), np.triu(ref, 1)) < TOL, "S untouched"
     assert relerr(hd, kd["HSDIAG_HNS"]) < TOL
+    ctx.close()
+
+
+def _check_sampled(hs, hd, kd, stage):
+    """HS/HSDIAG of one stage against the sampled reference entries (per
+    triangle, relative to that triangle's largest entry), the reference's
+    column and row sums and its HSDIAG.  The sums hold N entries, so the
+    elementwise bar gives |d sum| <= N TOL max|HS|."""
+    n = hs.shape[0]
+    ii, jj = kd[f"SAMPLE_I_{stage}"], kd[f"SAMPLE_J_{stage}"]
+    up = ii <= jj
+    smax = np.max(np.abs(np.triu(hs))) if stage == "HAMILT" else np.max(np.abs(np.triu(hs, 1)))
+    hmax = np.max(np.abs(np.tril(hs, -1))) if stage == "HAMILT" else np.max(np.abs(np.tril(hs)))
+    d = np.abs(hs[ii, jj] - kd[f"SAMPLE_V_{stage}"])
+    # after HNS the diagonal of HS holds diag(H) (CALKPT's swap)
+    if stage == "HAMILT":
+        s_up, h_lo = up, ~up
+    else:
+        s_up, h_lo = ii < jj, ii >= jj
+    assert np.max(d[s_up]) / smax < TOL, f"S samples {stage}"
+    assert np.max(d[h_lo]) / hmax < TOL, f"H samples {stage}"
+    mx = float(kd[f"MAXABS_{stage}"])
+    assert np.max(np.abs(hs.sum(axis=0) - kd[f"COLSUM_{stage}"])) < n * TOL * mx, f"column sums {stage}"
+    assert np.max(np.abs(hs.sum(axis=1) - kd[f"ROWSUM_{stage}"])) < n * TOL * mx, f"row sums {stage}"
+    assert relerr(hd, kd[f"HSDIAG_{stage}"]) < TOL, f"HSDIAG {stage}"
+
+
+@pytest.mark.big
+@pytest.mark.parametrize("name", LARGE)
+def test_assembly_at_size_matches_reference_samples(name):
+    """HAMILT and HNS of the large cells (N = 5951 for the 64-atom cell)
+    against the reference's sampled entries, column/row sums and HSDIAG."""
+    fx = Fixture(name)
+    kd = fx.k(0)
+    ctx = _ctx(fx)
+    ctx.set_basis(fx.basis(0))
+    ctx.hamilt()
+    hs, hd = ctx.get_hs()
+    _check_sampled(hs, hd, kd, "HAMILT")
+    del hs
+    ctx.swap_diag()
+    ctx.hns()
+    hs, hd = ctx.get_hs()
+    _check_sampled(hs, hd, kd, "HNS")
     ctx.close()
 
 

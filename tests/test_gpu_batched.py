@@ -98,6 +98,7 @@ def test_batched_reuses_streams_and_subsets():
         assert np.array_equal(out2[i][2], out1[q][2]) and np.array_equal(out2[i][3], out1[q][3])
 
 
+@pytest.mark.big
 def test_batched_c4_sixty_four_kpoints():
     """The batched-k cell: 64 complex k-points, N from 1936 to 1978 (no local
     orbitals), all in one batch; NE, NBELW and eigenvalues against the

@@ -19,7 +19,7 @@ import pytest
 
 from conftest import GOLDEN, REPO
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.big]
 EXE = {False: os.path.join(REPO, "oracle", "_ref", "lapw1_gpu_real"),
        True: os.path.join(REPO, "oracle", "_ref", "lapw1_gpu_complex")}
 

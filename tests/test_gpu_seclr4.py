@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 import scipy.linalg as sla
 
-from conftest import Fixture, PKG
+from conftest import EMU, Fixture, PKG
 
 pytestmark = pytest.mark.gpu
 L = PKG.lapw1
@@ -96,8 +96,8 @@ _MULTI_K_LO = ("a device fault was seen once in this chain (LO basis, basis grow
     ("lo_mult2", 1),
     ("mgo", 1),
     ("lo_cplx", 1),
-    pytest.param("mgo", 2, marks=pytest.mark.skip(reason=_MULTI_K_LO)),
-    pytest.param("lo_mult2", 2, marks=pytest.mark.skip(reason=_MULTI_K_LO)),
+    pytest.param("mgo", 2, marks=pytest.mark.skipif(not EMU, reason=_MULTI_K_LO)),
+    pytest.param("lo_mult2", 2, marks=pytest.mark.skipif(not EMU, reason=_MULTI_K_LO)),
 ])
 def test_calkpt_chain_matches_reference(name, nk):
     """HAMILT -> swap -> HNS -> SECLR4 on the device from the basis alone,
@@ -264,6 +264,26 @@ def test_seclr4_not_positive_definite_reports_info():
     ctx.close()
 
 
+@pytest.mark.parametrize("cpx", [False, True])
+def test_seclr4_reports_unconverged_vectors(cpx, monkeypatch):
+    """DSTEIN's non-convergence path (dsyevx2.f:352-354, seclr4.F:304-309):
+    with the growth threshold forced out of reach from the host, every vector
+    runs its five solves and is counted, and SECLR4 returns INFO = the number
+    of vectors; without it the same pencil solves with INFO = 0."""
+    n = 80
+    rng = np.random.default_rng(21)
+    X = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cpx else 0)
+    S = X @ X.conj().T / n + np.eye(n)
+    Y = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cpx else 0)
+    H = (Y + Y.conj().T) / 2
+    ne, nb, ev, z = _run_pencil(H, S, -1e9, 1e9, 30)
+    assert ne == 30
+    monkeypatch.setenv("W2K_TEST_STEIN_FAIL", "1")
+    with pytest.raises(L.Lapw1Error) as ei:
+        _run_pencil(H, S, -1e9, 1e9, 30)
+    assert ei.value.code == 30
+
+
 @pytest.mark.parametrize("bad", [5, 70])
 def test_seclr4_complex_not_positive_definite_reports_info(bad):
     """ZPOTRF INFO from the complex block Cholesky: a non-positive pivot in
@@ -284,6 +304,7 @@ def test_seclr4_complex_not_positive_definite_reports_info(bad):
     ctx.close()
 
 
+@pytest.mark.big
 def test_c2_rocksalt_chain_matches_reference():
     """64-atom rocksalt cell at RKMAX 7 (N = 5951): the full device chain
     against the reference eigenvalues, NBELW and the first/last vectors."""

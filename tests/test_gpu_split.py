@@ -147,6 +147,7 @@ def test_split_calkpt_eigenvalues(name, slices):
     ctx.close()
 
 
+@pytest.mark.big
 def test_split_c2_rocksalt_eigenvalues():
     """The 64-atom rocksalt k-point (N=5951) with the split-precision HNS
     (4 digits, the setting bench.py reports)."""
