@@ -2,25 +2,30 @@
 """Seconds per k-point of lapw1's hot path (HAMILT + HNS + SECLR4) on one MI355X,
 plain FP64 and with the split-precision HNS update (INT8 digit slices).
 
-Workload: the 64-atom rocksalt supercell (2x2x2, Mg/O, RMT 2.0/1.6, RKmax 7,
-real/inversion-symmetric, one k-point at Gamma, N = 5951 = 5695 plane waves +
-256 local orbitals, lowest ~20% of the spectrum = 1184 eigenpairs), built from
-the seeded WIEN2k input deck whose reference outputs are stored in
-tests/golden/c2_rs64.npz.  One step is one k-point: HAMILT, the diagonal swap,
-HNS and SECLR4 through the library's C-ABI with the per-run tables and the
-basis already resident on the device (uploaded before the timed region).
-
-The CPU baseline is the reference lapw1 itself (compiled from its sources by
-oracle/Makefile.ref into oracle/_ref/), run on this host on the same deck for
-the same k-point, timed by its own phase clocks; it runs before the GPU is
-touched.
+Headline workload: the 100-atom complex cell without symmetry (3 species,
+seeded positions, RKmax 6.5, one general k-point, N = NV + 400 local
+orbitals, lowest 20% of the spectrum), built from the seeded WIEN2k input deck
+whose reference outputs are stored in tests/golden/c3_lowsym100.npz.  One
+step is one k-point: HAMILT, the diagonal swap, HNS and SECLR4 through the
+library's C-ABI with the per-run tables and the basis already resident on the
+device (uploaded before the timed region).  The line also carries the 64-atom
+rocksalt cell (real, N = 5951) and the 64 k-points of the 8-atom cell
+(batched against a serial loop) as secondary measurements.
 
 The k-point is timed twice: with every product in FP64 ("fp64") and with the
 HNS rank-K update in split precision (--split-slices INT8 digits per element,
 FP64 recombination).  `value` is the split-precision time when its
 eigenvalues stay within 1e-8 Ry of the reference with identical NE/NBELW
-(checked in the run), else the FP64 time; both are in the line.  Prints one
-JSON line.
+(checked in the run), else the FP64 time; both are in the line, with the
+split path's eigenvector deviation from the FP64 path, its S-orthonormality
+and its residual.
+
+CPU baseline: the reference lapw1 itself (compiled from its sources by
+oracle/Makefile.ref into oracle/_ref/), run on this host before the GPU is
+touched.  The 100-atom k-point takes the reference about an hour, so the
+sample is the same cell at a reduced RKmax (the basis is the leading part of
+the |k+G|-sorted list), timed by the reference's own phase clocks and scaled
+to the full basis by N^2 (HAMILT, HNS) and N^3 (SECLR4).  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -28,6 +33,7 @@ import argparse
 import importlib.util
 import json
 import os
+import re
 import resource
 import subprocess
 import sys
@@ -37,14 +43,19 @@ import time
 import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
-FIXTURE = os.path.join(REPO, "tests", "golden", "c2_rs64.npz")
-REF_EXE = os.path.join(REPO, "oracle", "_ref", "lapw1_real")
+GOLDEN = os.path.join(REPO, "tests", "golden")
+C3_FIXTURE = os.path.join(GOLDEN, "c3_lowsym100.npz")
+C2_FIXTURE = os.path.join(GOLDEN, "c2_rs64.npz")
+BATCH_FIXTURE = os.path.join(GOLDEN, "c4_ortho8.npz")
+REF_DIR = os.path.join(REPO, "oracle", "_ref")
 
 METRIC = "s per k-point (H/S setup+eigensolve), split-prec vs FP64 at ΔE≤1e-8 Ry"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_INT8_TOPS = 5000.0
 DE_BAR = 1e-8  # Ry: split precision must stay this close to the reference eigenvalues
 PEAK_HBM_GBS = 8000.0
+# reduced RKmax of the CPU sample of the 100-atom cell (N about 2100)
+C3_SAMPLE_RKMAX = 3.5
 
 
 def load_package():
@@ -57,22 +68,29 @@ def load_package():
     return mod
 
 
-def load_workload(L):
-    raw = np.load(FIXTURE)
+def load_workload(L, path):
+    raw = np.load(path)
     run = L.RunData({k[4:]: raw[k] for k in raw.files if k.startswith("run/")}, bool(int(raw["meta/complex"])))
     kd = {k[3:]: raw[k] for k in raw.files if k.startswith("k0/")}
+    meta = {k[5:]: raw[k] for k in raw.files if k.startswith("meta/")}
     basis = L.Basis(int(kd["NV"]), int(kd["NLO"]), kd["KZZ"], kd["XK"], kd["YK"], kd["ZK"], kd["RK"])
-    return run, basis, kd
+    return run, basis, kd, meta
 
 
-def cpu_baseline(kd, threads: int):
-    """The reference lapw1 on the same deck (same seeded inputs, same window)."""
-    if not os.path.exists(REF_EXE):
-        return None
-    sys.path.insert(0, os.path.join(REPO, "tests", "golden"))
+def _progress(msg: str):
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def _run_reference(case, complex_: bool, threads: int, timeout: int):
+    """Reference lapw1 (no dumps) on a written deck: its phase clocks, the
+    matrix size it printed and the process wall time."""
+    sys.path.insert(0, GOLDEN)
     import w2k_inputs as W
-    case = W.CASES["c2_rs64"]()
-    case.window = (float(kd["EL"]), float(kd["EU"]))
+    sys.path.pop(0)
+    exe = os.path.join(REF_DIR, "lapw1_complex" if complex_ else "lapw1_real")
+    if not os.path.exists(exe):
+        return None
     with tempfile.TemporaryDirectory(prefix="w2k_ref_") as d:
         W.write_case(case, d)
         env = dict(os.environ, OPENBLAS_NUM_THREADS=str(threads), OMP_NUM_THREADS=str(threads))
@@ -81,30 +99,69 @@ def cpu_baseline(kd, threads: int):
             resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
         t0 = time.time()
-        r = subprocess.run([REF_EXE, "nodump"], cwd=d, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                           preexec_fn=unlimited_stack, timeout=900)
+        r = subprocess.run([exe, "nodump"], cwd=d, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                           preexec_fn=unlimited_stack, timeout=timeout)
         wall = time.time() - t0
         tf = os.path.join(d, "ref_times.txt")
         if r.returncode != 0 or not os.path.exists(tf):
             sys.stderr.write(r.stdout.decode(errors="replace")[-2000:])
             return None
         t = {k: float(v) for k, v in (l.split() for l in open(tf))}
-    per_k = t["hamilt_wall_s"] + t["hns_wall_s"] + t["seclr4_wall_s"]
+        m = re.search(r"MATRIX SIZE\s*(\d+)", open(os.path.join(d, "case.output1")).read())
+        t["n"] = int(m.group(1)) if m else 0
+        t["wall"] = wall
+    return t
+
+
+def cpu_baseline_c3(kd, threads: int):
+    """The reference on the 100-atom cell at RKmax C3_SAMPLE_RKMAX, scaled to
+    the full basis: HAMILT and HNS by (N/N_s)^2, SECLR4 by (N/N_s)^3.  The
+    window keeps about the same fraction of the spectrum as the full k-point:
+    its upper end is the full basis' eigenvalue at that fraction of N_s."""
+    sys.path.insert(0, GOLDEN)
+    import w2k_inputs as W
+    sys.path.pop(0)
+    n_full = int(kd["HSROWS"])
+    case = W.case_c3_lowsym100(rkmax=C3_SAMPLE_RKMAX)
+    nlo = int(kd["NLO"])
+    ns_est = nlo + (n_full - nlo) * (C3_SAMPLE_RKMAX / 6.5) ** 3
+    ev = np.asarray(kd["EIGVAL"])
+    j = min(len(ev) - 1, int(round(ns_est * len(ev) / n_full)))
+    case.window = (float(kd["EL"]), float(ev[j]))
+    t = _run_reference(case, True, threads, timeout=600)
+    if t is None or t["n"] <= 0:
+        return None
+    q = n_full / t["n"]
+    parts = {"hamilt": t["hamilt_wall_s"] * q * q, "hns": t["hns_wall_s"] * q * q, "seclr4": t["seclr4_wall_s"] * q ** 3}
+    per_k = sum(parts.values())
     return {
-        "value": round(per_k, 3),
+        "value": round(per_k, 2),
         "unit": "s/k-point",
         "cores": threads,
         "kind": "reference",
-        "sample": (f"the whole workload: one k-point of the 64-atom rocksalt deck through the reference lapw1 "
-                   f"(flang build of SRC_lapw1); HAMILT {t['hamilt_wall_s']:.2f} s and HNS {t['hns_wall_s']:.2f} s "
-                   f"on 1 thread (serial in the reference), SECLR4 {t['seclr4_wall_s']:.2f} s with OpenBLAS on "
-                   f"{threads} threads; process wall {wall:.1f} s"),
+        "sample": (f"the reference lapw1 (flang build of SRC_lapw1, complex) on the same 100-atom cell and k-point at "
+                   f"RKmax {C3_SAMPLE_RKMAX} (N_s = {t['n']}, the leading part of the |k+G|-sorted basis): HAMILT "
+                   f"{t['hamilt_wall_s']:.2f} s and HNS {t['hns_wall_s']:.2f} s on 1 thread (serial in the "
+                   f"reference), SECLR4 {t['seclr4_wall_s']:.2f} s with OpenBLAS on {threads} threads, process "
+                   f"wall {t['wall']:.1f} s; scaled to N = {n_full} by (N/N_s)^2 for HAMILT and HNS and (N/N_s)^3 "
+                   f"for SECLR4: {parts['hamilt']:.0f} + {parts['hns']:.0f} + {parts['seclr4']:.0f} s"),
     }
 
 
-def _progress(msg: str):
-    sys.stderr.write(f"[bench] {msg}\n")
-    sys.stderr.flush()
+def cpu_baseline_c2(kd, threads: int):
+    """The reference on the whole 64-atom k-point (same deck, same window)."""
+    sys.path.insert(0, GOLDEN)
+    import w2k_inputs as W
+    sys.path.pop(0)
+    case = W.CASES["c2_rs64"]()
+    case.window = (float(kd["EL"]), float(kd["EU"]))
+    t = _run_reference(case, False, threads, timeout=900)
+    if t is None:
+        return None
+    per_k = t["hamilt_wall_s"] + t["hns_wall_s"] + t["seclr4_wall_s"]
+    return {"value": round(per_k, 3), "unit": "s/k-point", "cores": threads, "kind": "reference",
+            "sample": (f"the whole k-point: HAMILT {t['hamilt_wall_s']:.2f} s and HNS {t['hns_wall_s']:.2f} s on 1 "
+                       f"thread, SECLR4 {t['seclr4_wall_s']:.2f} s on {threads} threads")}
 
 
 def copy_calibration(L):
@@ -122,59 +179,99 @@ def copy_calibration(L):
             "GB/s": round(16.0 * n / (ms.value * 1e-3) / 1e9, 1)}
 
 
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "c2_rs64_hbm_counters.json")
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "hbm_counters.json")
 
 
-def measured_traffic(kname: str):
+def measured_traffic(workload: str, kname: str):
     """HBM bytes per launch of the roofline kernel from the committed counter
     passes (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each scaled by the
     same counter's reading on the calibration copy), or None."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
     with open(TRAFFIC_FILE) as f:
-        rec = json.load(f).get("kernels", {}).get(kname)
+        rec = json.load(f).get(workload, {}).get("kernels", {}).get(kname)
     return None if rec is None else rec.get("hbm_bytes_per_launch")
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--no-kernel-timing", action="store_true",
-                    help="skip the extra k-point with per-launch SYMV events (for profiler passes)")
-    ap.add_argument("--split-slices", type=int, default=4,
-                    help="INT8 digits per element for the split-precision pass (0: FP64 only)")
-    ap.add_argument("--no-batched", action="store_true",
-                    help="skip the batched-k measurement (64 k-points of the 8-atom cell)")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run a plain 512 MiB device copy (calibrates the profiler's byte counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(k-points shard across GPUs with no exchange; run one process per GPU)", file=sys.stderr)
-        sys.exit(2)
+def _sum_lindex(run) -> int:
+    """sum over all equivalent atoms of the number of distinct (L0,M0) of HNS."""
+    lqns = run.arr["lqns"]
+    lqind = run.arr["lqind"]
+    mult = run.arr["mult"]
+    tot = 0
+    for a in range(run.nat):
+        seen = set()
+        for q in range(int(lqind[a])):
+            seen.add((int(lqns[2, q, a]), int(lqns[5, q, a])))
+        tot += len(seen) * int(mult[a])
+    return tot
 
-    pkg = load_package()
-    L = pkg.lapw1
-    run, basis, kd = load_workload(L)
+
+def _isolated(ev, tol=1e-6):
+    g = np.diff(ev)
+    left = np.concatenate([[np.inf], g])
+    right = np.concatenate([g, [np.inf]])
+    return np.nonzero(np.minimum(left, right) > tol)[0]
+
+
+def vector_errors(hs, hd, ev_f, z_f, ev_s, z_s, complex_: bool):
+    """Split-precision eigenvectors against the FP64 path's (isolated
+    eigenvalues, up to a unit phase, relative to the largest entry), and their
+    S-orthonormality and residual against the FP64 pencil, on the device
+    (torch, FP64)."""
+    import torch
+    dev = torch.device("cuda:0")
+    idx = _isolated(ev_f)
+    zf = torch.from_numpy(np.ascontiguousarray(z_f[:, idx])).to(dev)
+    zs = torch.from_numpy(np.ascontiguousarray(z_s[:, idx])).to(dev)
+    ph = (zs.conj() * zf).sum(dim=0)
+    ph = ph / ph.abs().clamp_min(1e-300)
+    dz = ((zs * ph) - zf).abs().amax(dim=0) / zf.abs().amax(dim=0)
+    vec_err = float(dz.max()) if len(idx) else 0.0
+    del zf, zs
+    A = torch.from_numpy(hs).to(dev)
+    if complex_:
+        # H from the conjugated lower triangle, S from the upper triangle and HSDIAG
+        Hl = torch.tril(A, -1).conj()
+        H = Hl + Hl.conj().T + torch.diag(torch.diagonal(A).real.to(A.dtype))
+        Su = torch.triu(A, 1)
+        S = Su + Su.conj().T + torch.diag(torch.from_numpy(np.asarray(hd).real.copy()).to(dev).to(A.dtype))
+        del Hl
+    else:
+        H = torch.tril(A) + torch.tril(A, -1).T
+        Su = torch.triu(A, 1)
+        S = Su + Su.T + torch.diag(torch.from_numpy(np.asarray(hd)).to(dev))
+    del A, Su
+    Z = torch.from_numpy(z_s).to(dev)
+    SZ = S @ Z
+    G = Z.conj().T @ SZ
+    orth = float((G - torch.eye(G.shape[0], dtype=G.dtype, device=dev)).abs().max())
+    w = torch.from_numpy(ev_s).to(dev).to(Z.dtype)
+    R = H @ Z - SZ * w
+    res = float(R.abs().max() / H.abs().max())
+    del H, S, Z, SZ, G, R
+    torch.cuda.empty_cache()
+    return {"max_vec_dev_vs_fp64_rel": vec_err, "isolated_vectors_compared": int(len(idx)),
+            "s_orthonormality_max_abs": orth, "residual_max_rel": res}
+
+
+def measure(L, run, basis, kd, args, label: str, workload: str, full_checks: bool):
+    """W warmup and K timed k-points in FP64 and in split precision, one
+    k-point with per-launch SYMV events, checks against the reference."""
     el, eu, nume = float(kd["EL"]), float(kd["EU"]), int(kd["NUME"])
-
-    cpu = None if args.no_cpu_baseline else cpu_baseline(kd, args.cpu_threads)
-
-    copy = None
-    if args.copy_calibration:
-        copy = copy_calibration(L)
-        _progress(f"copy calibration: {copy['GB/s']} GB/s")
     ctx = L.Lapw1(run)
     ctx.set_basis(basis)
     names = L.Lapw1.TIMING_NAMES
+    hs0 = hd0 = None
+    if full_checks and args.split_slices > 0:
+        # the pencil of the FP64 path (for the split path's S-orthonormality
+        # and residual), outside the timed region
+        ctx.hamilt()
+        ctx.swap_diag()
+        ctx.hns()
+        hs0, hd0 = ctx.get_hs()
 
     def timed(slices: int):
-        """W warmup and K timed k-points with the HNS rank-K update in FP64
-        (slices = 0) or in split precision (INT8 digit slices)."""
         ctx.set_split_precision(slices)
         for _ in range(args.warmup):
             ctx.calkpt(el, eu, nume)
@@ -186,10 +283,12 @@ def main():
             acc += ctx.timings()
         ctx.sync()
         t1 = time.perf_counter()
-        ev, _ = ctx.get_eig()
+        ev, z = ctx.get_eig()
         tag = "FP64" if slices == 0 else f"split precision ({slices} digits)"
-        _progress(f"{args.warmup} warmup + {args.steps} timed k-point(s) done, {tag}")
-        return {"ms": (t1 - t0) * 1e3 / args.steps, "dev": acc / args.steps, "ne": ne, "nb": nb, "ev": ev}
+        _progress(f"{label}: {args.warmup} warmup + {args.steps} timed k-point(s) done, {tag}, "
+                  f"{(t1 - t0) / args.steps:.4f} s/k-point")
+        return {"ms": (t1 - t0) * 1e3 / args.steps, "dev": acc / args.steps, "ne": ne, "nb": nb, "ev": ev,
+                "z": z if full_checks else None}
 
     fp = timed(0)
     sp = timed(args.split_slices) if args.split_slices > 0 else None
@@ -209,9 +308,8 @@ def main():
         return ok, (float(np.max(np.abs(r["ev"] - kd["EIGVAL"]))) if ok else float("nan"))
 
     ok, dmax = check(fp)
-    ne, nb, dev, ms_fp64 = fp["ne"], fp["nb"], fp["dev"], fp["ms"]
     split = None
-    ms_step = ms_fp64
+    ms_step = fp["ms"]
     if sp is not None:
         sok, sd = check(sp)
         de_fp = float(np.max(np.abs(sp["ev"] - fp["ev"]))) if sok else float("nan")
@@ -220,88 +318,175 @@ def main():
                  "max_abs_eig_dev_vs_reference_ry": sd, "max_abs_eig_dev_vs_fp64_ry": de_fp,
                  "ne": sp["ne"], "nbelw": sp["nb"], "meets_de_bar": usable,
                  "hns_rank_k_ms": round(float(sp["dev"][names.index("hns_gemm")]), 3)}
+        if full_checks and sok and hs0 is not None:
+            split["eigenvectors"] = vector_errors(hs0, hd0, fp["ev"], fp["z"], sp["ev"], sp["z"], run.is_complex)
         if usable:
             ms_step = sp["ms"]
+    del hs0, hd0
 
     nv, nlo, n = basis.nv, basis.nlo, basis.n
     nat, lt = run.nat, run.ltop
+    cx = 4.0 if run.is_complex else 1.0
+    esz = 16.0 if run.is_complex else 8.0
     sum_lindex = _sum_lindex(run)
     f_hamilt = 8.0 * nv * nv * nat * lt
-    f_hns = 4.0 * n * n * sum_lindex
-    t = dict(zip(names, dev))
+    f_hns = (8.0 if run.is_complex else 4.0) * n * n * sum_lindex
+    ne = fp["ne"]
+    f_seclr4 = cx * (n ** 3 / 3.0 + n ** 3 + 4.0 * n ** 3 / 3.0 + 3.0 * n * n * ne)
+    t = dict(zip(names, fp["dev"]))
     kernels = {
-        "k_hamilt_pw": ("mfma", f_hamilt, t["hamilt_pw_kernel"], 1),
+        "k_hamilt_pw": ("valu", f_hamilt, t["hamilt_pw_kernel"], 1),
         "k_dgemm (HNS rank-K)": ("mfma", f_hns, t["hns_gemm"], 1),
     }
+    symv = "k_ztri_symv" if run.is_complex else "k_tri_symv"
     if tk is not None and tk["tri_symv_launches"] > 0:
         # algorithmic bytes of the SYMV at step i: the lower triangle
-        # (diagonal included) of the trailing order m = n - i - 1, FP64
+        # (diagonal included) of the trailing order m = n - i - 1
         nsym = int(tk["tri_symv_launches"])
-        sym_bytes = sum(8.0 * m * (m + 1) / 2 for m in range(n - 1, n - 1 - nsym, -1))
-        kernels["k_tri_symv"] = ("hbm", sym_bytes, tk["tri_symv_total"], nsym)
+        sym_bytes = sum(esz * m * (m + 1) / 2 for m in range(n - 1, n - 1 - nsym, -1))
+        kernels[symv] = ("hbm", sym_bytes, tk["tri_symv_total"], nsym)
     kname, (bound, work, kms, nl) = max(kernels.items(), key=lambda kv: kv[1][2])
-    if bound == "hbm":
-        achieved, peak, unit = work / (kms * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s"
-    else:
-        achieved, peak, unit = work / (kms * 1e-3) / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
+
+    def rate(bd, wk, ms_):
+        if bd == "hbm":
+            return wk / (ms_ * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s"
+        return wk / (ms_ * 1e-3) / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
+
+    achieved, peak, unit = rate(bound, work, kms)
     others = {}
     for kn, (bd, wk, ms_, nl_) in kernels.items():
         if kn == kname or ms_ <= 0:
             continue
-        if bd == "hbm":
-            others[kn] = {"ms": round(ms_, 3), "GB/s": round(wk / (ms_ * 1e-3) / 1e9, 1)}
-        else:
-            others[kn] = {"ms": round(ms_, 3), "TFLOP/s": round(wk / (ms_ * 1e-3) / 1e12, 2)}
+        a_, p_, u_ = rate(bd, wk, ms_)
+        others[kn] = {"ms": round(ms_, 3), u_: round(a_, 2), "frac": round(a_ / p_, 4)}
     if split is not None and split["hns_rank_k_ms"] > 0:
         # the same FP64-equivalent work; the INT8 rate counts the S(S+1)/2
-        # digit-pair products actually issued
+        # digit-pair products actually issued (x4 for the complex product's
+        # four real products, already in f_hns)
         sl, tms = split["slices"], split["hns_rank_k_ms"]
         others["k_split_gemm (HNS rank-K, INT8 digits)"] = {
             "ms": tms, "FP64-equivalent TFLOP/s": round(f_hns / (tms * 1e-3) / 1e12, 2),
             "INT8 TOP/s": round(f_hns * sl * (sl + 1) / 2 / (tms * 1e-3) / 1e12, 1), "INT8 peak TOP/s": PEAK_INT8_TOPS}
     use_split = split is not None and split["meets_de_bar"]
+    f_total = f_hamilt + f_hns + f_seclr4
+    res = {
+        "ms_step": ms_step,
+        "use_split": use_split,
+        "roofline": {"kernel": kname, "bound": "mfma" if bound == "valu" else bound, "achieved": round(achieved, 2),
+                     "peak": peak, "unit": unit, "frac": round(achieved / peak, 4),
+                     "traffic": measured_traffic(workload, kname) if bound == "hbm" else None,
+                     "work_per_launch": work / nl, "ms_per_launch": round(kms / nl, 4), "launches": nl,
+                     "other_kernels": others},
+        "canonical_flops": {"hamilt": f_hamilt, "hns": f_hns, "seclr4": f_seclr4, "total": f_total,
+                            "fp64_tflops_whole_kpoint": round(f_total / (fp["ms"] * 1e-3) / 1e12, 2)},
+        "phases_ms": {k: round(v, 2) for k, v in t.items() if k != "tri_launch_mode"},
+        "check": {"ne": fp["ne"], "nbelw": fp["nb"], "ne_ref": int(kd["NE"]), "nbelw_ref": int(kd["NBELW"]),
+                  "max_abs_eig_dev_ry": dmax, "ok": ok},
+        "fp64": {"value": round(fp["ms"] / 1e3, 4), "ms_per_step": round(fp["ms"], 2), "max_abs_eig_dev_ry": dmax},
+        "split_precision": split,
+        "value_is": "split precision" if use_split else "fp64",
+        "sizes": {"n": n, "nv": nv, "nlo": nlo, "ne": int(fp["ne"]), "nat": nat, "lt": lt, "sum_lindex": sum_lindex},
+    }
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="skip the extra k-point with per-launch SYMV events (for profiler passes)")
+    ap.add_argument("--split-slices", type=int, default=4,
+                    help="INT8 digits per element for the split-precision pass (0: FP64 only)")
+    ap.add_argument("--workload", choices=["c3", "c2"], default="c3",
+                    help="headline workload: the 100-atom complex cell (c3) or the 64-atom rocksalt cell (c2)")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the 64-atom rocksalt line")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-k measurement (64 k-points of the 8-atom cell)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 512 MiB device copy (calibrates the profiler's byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(k-points shard across GPUs with no exchange; run one process per GPU)", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = load_package()
+    L = pkg.lapw1
+    workload = args.workload
+    if workload == "c3" and not os.path.exists(C3_FIXTURE):
+        _progress("tests/golden/c3_lowsym100.npz is missing: the 64-atom rocksalt cell is the headline")
+        workload = "c2"
+    path = C3_FIXTURE if workload == "c3" else C2_FIXTURE
+    run, basis, kd, meta = load_workload(L, path)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        _progress("CPU baseline (reference lapw1) ...")
+        cpu = cpu_baseline_c3(kd, args.cpu_threads) if workload == "c3" else cpu_baseline_c2(kd, args.cpu_threads)
+        if cpu is not None and workload == "c3" and "ref_seconds" in meta:
+            cpu["full_kpoint_build_host"] = {
+                "s": round(float(meta["hamilt_wall_s"] + meta["hns_wall_s"] + meta["seclr4_wall_s"]), 1),
+                "host": "the 8-core container that wrote the fixture (OpenBLAS on 8 threads)"}
+        _progress(f"CPU baseline: {cpu['value'] if cpu else None} s/k-point")
+
+    copy = None
+    if args.copy_calibration:
+        copy = copy_calibration(L)
+        _progress(f"copy calibration: {copy['GB/s']} GB/s")
+
+    label = "100-atom cell" if workload == "c3" else "64-atom cell"
+    r = measure(L, run, basis, kd, args, label, workload, full_checks=(workload == "c3"))
+    sz = r["sizes"]
+    if workload == "c3":
+        wl = (f"100-atom cell, 3 species (Ti/Mg/O), seeded positions, no symmetry, complex, one general k-point, "
+              f"RKmax 6.5, N={sz['n']} (NV {sz['nv']} + NLO {sz['nlo']}), NE={sz['ne']} (lowest 20%)")
+    else:
+        wl = (f"64-atom rocksalt 2x2x2 (Mg/O), real, Gamma, RKmax 7, N={sz['n']} (NV {sz['nv']} + NLO {sz['nlo']}), "
+              f"NE={sz['ne']}")
     out = {
         "metric": METRIC,
-        "value": round(ms_step / 1e3, 4),
+        "value": round(r["ms_step"] / 1e3, 4),
         "unit": "s/k-point",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 2),
+        "ms_per_step": round(r["ms_step"], 2),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if not use_split else "f64 (HNS rank-K: int8 digit slices, FP64 recombination)",
+        "dtype": "f64" if not r["use_split"] else "f64 (HNS rank-K: int8 digit slices, FP64 recombination)",
         "data": "synthetic: seeded WIEN2k input deck (tests/golden/w2k_inputs.py)",
-        "config": {"workload": f"64-atom rocksalt 2x2x2 (Mg/O), real, Gamma, RKmax 7, N={n} (NV {nv} + NLO {nlo}), "
-                               f"NE={ne}", "nat": nat, "lt": lt, "sum_lindex": sum_lindex},
-        "roofline": {"kernel": kname, "bound": bound, "achieved": round(achieved, 2), "peak": peak,
-                     "unit": unit, "frac": round(achieved / peak, 4), "traffic": measured_traffic(kname),
-                     "work_per_launch": work / nl, "ms_per_launch": round(kms / nl, 4), "launches": nl,
-                     "other_kernels": others},
+        "config": {"workload": wl, "nat": sz["nat"], "lt": sz["lt"], "sum_lindex": sz["sum_lindex"]},
+        "roofline": r["roofline"],
         "cpu_baseline": cpu,
-        "phases_ms": {k: round(v, 2) for k, v in t.items() if k != "tri_launch_mode"},
-        "tri_launch_mode": {0: "direct", 1: "hip graph replay", 2: "hip graph capture"}.get(int(t.get("tri_launch_mode", 0)), "?"),
-        "check": {"ne": ne, "nbelw": nb, "ne_ref": int(kd["NE"]), "max_abs_eig_dev_ry": dmax},
-        "fp64": {"value": round(ms_fp64 / 1e3, 4), "ms_per_step": round(ms_fp64, 2), "max_abs_eig_dev_ry": dmax},
-        "split_precision": split,
-        "value_is": "split precision" if use_split else "fp64",
     }
+    for k in ("canonical_flops", "phases_ms", "check", "fp64", "split_precision", "value_is"):
+        out[k] = r[k]
     if copy is not None:
         out["copy_calibration"] = copy
+    if workload == "c3" and not args.no_secondary:
+        run2, basis2, kd2, _ = load_workload(L, C2_FIXTURE)
+        r2 = measure(L, run2, basis2, kd2, args, "64-atom cell", "c2", full_checks=False)
+        out["secondary"] = {
+            "workload": (f"64-atom rocksalt 2x2x2 (Mg/O), real, Gamma, RKmax 7, N={r2['sizes']['n']}, "
+                         f"NE={r2['sizes']['ne']}"),
+            "value": round(r2["ms_step"] / 1e3, 4), "value_is": r2["value_is"], "fp64": r2["fp64"],
+            "split_precision": r2["split_precision"], "phases_ms": r2["phases_ms"], "check": r2["check"],
+            "roofline": r2["roofline"], "canonical_flops": r2["canonical_flops"]}
     if not args.no_batched:
         out["batched"] = batched_kpoints(L)
     print(json.dumps(out))
 
 
-BATCH_FIXTURE = os.path.join(REPO, "tests", "golden", "c4_ortho8.npz")
-
-
 def batched_kpoints(L):
-    """64 k-points of the 8-atom orthorhombic complex cell (N 1936-1978):
-    a serial loop of single k-point calls against one batched call, both over
-    every k-point with its own window; eigenvalues checked against the
-    reference (1e-10 Ry) and the batch against the loop (bit-identical)."""
+    """64 k-points of the 8-atom orthorhombic complex cell: a serial loop of
+    single k-point calls against one batched call, both over every k-point
+    with its own window; eigenvalues checked against the reference (1e-10 Ry)
+    and the batch against the loop (bit-identical)."""
     raw = np.load(BATCH_FIXTURE)
     run = L.RunData({k[4:]: raw[k] for k in raw.files if k.startswith("run/")}, bool(int(raw["meta/complex"])))
     nk = int(raw["meta/nk"])
@@ -345,20 +530,6 @@ def batched_kpoints(L):
             "serial_s_per_kpoint": round(t_serial / nk, 4), "batched_s_per_kpoint": round(t_batch / nk, 4),
             "speedup": round(t_serial / t_batch, 2), "max_abs_eig_dev_ry": dmax, "batched_equals_serial": same,
             "cpu_reference_s_per_kpoint_8_threads_build_host": round(float(raw["meta/ref_seconds"]) / nk, 3)}
-
-
-def _sum_lindex(run) -> int:
-    """sum over all equivalent atoms of the number of distinct (L0,M0) of HNS."""
-    lqns = run.arr["lqns"]
-    lqind = run.arr["lqind"]
-    mult = run.arr["mult"]
-    tot = 0
-    for a in range(run.nat):
-        seen = set()
-        for q in range(int(lqind[a])):
-            seen.add((int(lqns[2, q, a]), int(lqns[5, q, a])))
-        tot += len(seen) * int(mult[a])
-    return tot
 
 
 if __name__ == "__main__":

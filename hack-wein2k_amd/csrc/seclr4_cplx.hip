@@ -811,6 +811,14 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
   g.partr = B.part; g.parti = B.part + (size_t)nts * n;
   g.dotr = B.tdots; g.doti = B.tdots + (size_t)nts * 2 * TNB;
   g.vy = B.tvy; g.norm = B.tnorm; g.counter = B.ticket;
+  // optional per-launch events around the SYMV (kernel-level roofline)
+  const bool kt = c.ktime != 0;
+  if (kt && c.kev.size() < (size_t)2 * n) {
+    const size_t old = c.kev.size();
+    c.kev.resize((size_t)2 * n);
+    for (size_t q = old; q < c.kev.size(); q++) W2K_HIP(hipEventCreate(&c.kev[q]));
+  }
+  int nsymv = 0;
   for (int p0 = 0; p0 < n; p0 += TNB) {
     const int nb = std::min(TNB, n - p0);
     g.p0 = p0;
@@ -824,8 +832,11 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
       nref = k + 1;
       const int rows = n - (i + 1);
       const int nt = (rows + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
+      if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv], s));
       hipLaunchKernelGGL(k_ztri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
       W2K_STAGE(s, "k_ztri_symv");
+      if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv + 1], s));
+      nsymv++;
       const int next = (k + 1 < nb) ? i + 1 : -1;
       hipLaunchKernelGGL(k_ztri_col, dim3((rows + ZCOL_ROWS - 1) / ZCOL_ROWS), dim3(256), 0, s, g, i, next, nt, nt, nblk);
       W2K_STAGE(s, "k_ztri_col");
@@ -845,6 +856,23 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
     }
   }
   W2K_HIP(hipEventRecord(c.ev[9], s));
+  if (kt) {
+    // [10] summed SYMV kernel time, [11] number of SYMV launches
+    W2K_HIP(hipStreamSynchronize(s));
+    double tot = 0.0;
+    int cntl = 0;
+    for (int q = 0; q < nsymv; q++) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, c.kev[2 * q], c.kev[2 * q + 1]) != hipSuccess) {
+        (void)hipGetLastError();
+        break;
+      }
+      tot += ms;
+      cntl++;
+    }
+    c.last_ms[10] = tot;
+    c.last_ms[11] = cntl;
+  }
   // ---- 4./5. real tridiagonal eigenpairs (shared with the real build)
   double *Zt = Xr;  // n x m real, then the real part of Q Zt
   double *Zi = Xi;  // imaginary part of Q Zt

@@ -6,6 +6,8 @@ sign); for degenerate groups the spanned subspace (projector) within 1e-8;
 S-orthonormality and residual of every vector within 1e-10 / 1e-9."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 import scipy.linalg as sla
@@ -323,4 +325,50 @@ def test_c2_rocksalt_chain_matches_reference():
             r = kd[key]
             v = z[:, j] * np.sign(np.dot(z[:, j], r))
             assert np.max(np.abs(v - r)) < 1e-8 * np.max(np.abs(r)), key
+    ctx.close()
+
+
+_C3 = os.path.exists(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_lowsym100.npz"))
+
+
+def _phase_err(v, r):
+    """max |v e^{i phi} - r| / max |r| over the unit phase that aligns v to r."""
+    ph = np.vdot(v, r)
+    ph = ph / abs(ph) if abs(ph) > 0 else 1.0
+    return float(np.max(np.abs(v * ph - r)) / np.max(np.abs(r)))
+
+
+@pytest.mark.big
+@pytest.mark.skipif(not _C3, reason="c3_lowsym100 fixture not generated (tests/golden/make_fixtures.py c3_lowsym100)")
+def test_c3_lowsym100_chain_matches_reference():
+    """100-atom complex cell without symmetry (N about 12400, lowest 20% of the
+    spectrum): the full device chain in FP64 against the reference's
+    eigenvalues (1e-10 Ry), NE/NBELW and three isolated eigenvectors (1e-8),
+    then the same k-point with the split-precision HNS update (4 INT8 digits):
+    eigenvalues within 1e-8 Ry of the reference and its eigenvectors against
+    the FP64 path's."""
+    fx = Fixture("c3_lowsym100")
+    kd = fx.k(0)
+    ctx = L.Lapw1(fx.run())
+    ctx.set_basis(fx.basis(0))
+    el, eu, nume = float(kd["EL"]), float(kd["EU"]), int(kd["NUME"])
+    ne, nb = ctx.calkpt(el, eu, nume)
+    assert ne == int(kd["NE"]) and nb == int(kd["NBELW"])
+    ev, z = ctx.get_eig()
+    assert np.max(np.abs(ev - kd["EIGVAL"])) < 1e-10
+    gaps = np.diff(ev)
+    mid = int(kd["Z_MID_INDEX"])
+    for j, key in ((0, "Z_FIRST"), (mid, "Z_MID"), (ne - 1, "Z_LAST")):
+        g = min(gaps[j - 1] if j > 0 else np.inf, gaps[j] if j < ne - 1 else np.inf)
+        if g > 1e-6:
+            assert _phase_err(z[:, j], kd[key]) < 1e-8, key
+    ctx.set_split_precision(4)
+    ne2, nb2 = ctx.calkpt(el, eu, nume)
+    assert ne2 == ne and nb2 == nb
+    ev2, z2 = ctx.get_eig()
+    assert np.max(np.abs(ev2 - kd["EIGVAL"])) < 1e-8
+    for j in (0, mid, ne - 1):
+        g = min(gaps[j - 1] if j > 0 else np.inf, gaps[j] if j < ne - 1 else np.inf)
+        if g > 1e-6:
+            assert _phase_err(z2[:, j], z[:, j]) < 1e-6, j
     ctx.close()
