@@ -214,45 +214,40 @@ def _isolated(ev, tol=1e-6):
     return np.nonzero(np.minimum(left, right) > tol)[0]
 
 
-def vector_errors(hs, hd, ev_f, z_f, ev_s, z_s, complex_: bool):
+def vector_errors(hs, hd, ev_f, z_f, ev_s, z_s, complex_: bool, nsample: int = 256):
     """Split-precision eigenvectors against the FP64 path's (isolated
-    eigenvalues, up to a unit phase, relative to the largest entry), and their
-    S-orthonormality and residual against the FP64 pencil, on the device
-    (torch, FP64)."""
-    import torch
-    dev = torch.device("cuda:0")
+    eigenvalues, up to a unit phase, relative to the largest entry), and the
+    S-orthonormality and residual of an evenly spaced sample of them against
+    the FP64 pencil (host, FP64)."""
     idx = _isolated(ev_f)
-    zf = torch.from_numpy(np.ascontiguousarray(z_f[:, idx])).to(dev)
-    zs = torch.from_numpy(np.ascontiguousarray(z_s[:, idx])).to(dev)
-    ph = (zs.conj() * zf).sum(dim=0)
-    ph = ph / ph.abs().clamp_min(1e-300)
-    dz = ((zs * ph) - zf).abs().amax(dim=0) / zf.abs().amax(dim=0)
-    vec_err = float(dz.max()) if len(idx) else 0.0
-    del zf, zs
-    A = torch.from_numpy(hs).to(dev)
+    vec_err = 0.0
+    for j in idx:
+        v, r = z_s[:, j], z_f[:, j]
+        ph = np.vdot(v, r)
+        ph = ph / abs(ph) if abs(ph) > 0 else 1.0
+        vec_err = max(vec_err, float(np.max(np.abs(v * ph - r)) / np.max(np.abs(r))))
     if complex_:
-        # H from the conjugated lower triangle, S from the upper triangle and HSDIAG
-        Hl = torch.tril(A, -1).conj()
-        H = Hl + Hl.conj().T + torch.diag(torch.diagonal(A).real.to(A.dtype))
-        Su = torch.triu(A, 1)
-        S = Su + Su.conj().T + torch.diag(torch.from_numpy(np.asarray(hd).real.copy()).to(dev).to(A.dtype))
+        Hl = np.tril(hs, -1).conj()
+        H = Hl + Hl.conj().T
         del Hl
+        H[np.diag_indices_from(H)] = np.diag(hs).real
+        S = np.triu(hs, 1)
+        S = S + S.conj().T
+        S[np.diag_indices_from(S)] = np.asarray(hd).real
     else:
-        H = torch.tril(A) + torch.tril(A, -1).T
-        Su = torch.triu(A, 1)
-        S = Su + Su.T + torch.diag(torch.from_numpy(np.asarray(hd)).to(dev))
-    del A, Su
-    Z = torch.from_numpy(z_s).to(dev)
+        H = np.tril(hs) + np.tril(hs, -1).T
+        S = np.triu(hs, 1)
+        S = S + S.T
+        S[np.diag_indices_from(S)] = hd
+    cols = np.unique(np.linspace(0, len(ev_s) - 1, min(nsample, len(ev_s))).round().astype(int))
+    Z = np.ascontiguousarray(z_s[:, cols])
     SZ = S @ Z
     G = Z.conj().T @ SZ
-    orth = float((G - torch.eye(G.shape[0], dtype=G.dtype, device=dev)).abs().max())
-    w = torch.from_numpy(ev_s).to(dev).to(Z.dtype)
-    R = H @ Z - SZ * w
-    res = float(R.abs().max() / H.abs().max())
-    del H, S, Z, SZ, G, R
-    torch.cuda.empty_cache()
+    orth = float(np.max(np.abs(G - np.eye(len(cols)))))
+    R = H @ Z - SZ * ev_s[cols]
+    res = float(np.max(np.abs(R)) / np.max(np.abs(H)))
     return {"max_vec_dev_vs_fp64_rel": vec_err, "isolated_vectors_compared": int(len(idx)),
-            "s_orthonormality_max_abs": orth, "residual_max_rel": res}
+            "sampled_vectors": int(len(cols)), "s_orthonormality_max_abs": orth, "residual_max_rel": res}
 
 
 def measure(L, run, basis, kd, args, label: str, workload: str, full_checks: bool):

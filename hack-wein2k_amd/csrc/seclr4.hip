@@ -36,6 +36,8 @@
 #include <utility>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "seclr4_shared.h"
 
 namespace w2k {
@@ -1666,10 +1668,12 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
           const int rows = n - (i + 1);
           const int base = (i + 1) & ~15;  // the SYMV's aligned tile origin
           const int nt = (n - base + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
-          if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv], s));
-          hipLaunchKernelGGL(k_tri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
+          // timed launches carry their events in the dispatch itself, so the
+          // duration is the kernel's own (no marker packets between launches)
+          if (kt) hipExtLaunchKernelGGL(k_tri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, c.kev[2 * nsymv],
+                                        c.kev[2 * nsymv + 1], 0, g, i, nt);
+          else hipLaunchKernelGGL(k_tri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
           W2K_STAGE(s, "k_tri_symv");
-          if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv + 1], s));
           nsymv++;
           const int next = (k + 1 < nb) ? i + 1 : -1;
           hipLaunchKernelGGL(k_tri_col, dim3((rows + COL_ROWS - 1) / COL_ROWS), dim3(256), 0, s, g, i, next, nt,

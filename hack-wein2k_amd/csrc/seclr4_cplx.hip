@@ -18,6 +18,8 @@
 #include <cmath>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "seclr4_shared.h"
 
 namespace w2k {
@@ -601,8 +603,12 @@ __global__ __launch_bounds__(256) void k_ztri_col(ZTriWork g, int fin, int col, 
       const bool use = k < jj && k != ii;
       const double wcr_ = use ? wcr[min(k, TNB - 1)] : 0.0, wci_ = use ? wci[min(k, TNB - 1)] : 0.0;
       const double vcr_ = use ? vcr[min(k, TNB - 1)] : 0.0, vci_ = use ? vci[min(k, TNB - 1)] : 0.0;
-      a0r += vkr[u] * wcr_ + vki[u] * wci_ + wkr[u] * vcr_ + wki[u] * vci_;
-      a0i += vki[u] * wcr_ - vkr[u] * wci_ + wki[u] * vcr_ - wkr[u] * vci_;
+      // panel columns k >= jj hold values of an earlier panel (or never
+      // written memory): selected away, never multiplied by zero
+      const double vr_ = use ? vkr[u] : 0.0, vi_ = use ? vki[u] : 0.0;
+      const double wr_ = use ? wkr[u] : 0.0, wi_ = use ? wki[u] : 0.0;
+      a0r += vr_ * wcr_ + vi_ * wci_ + wr_ * vcr_ + wi_ * vci_;
+      a0i += vi_ * wcr_ - vr_ * wci_ + wi_ * vcr_ - wr_ * vci_;
     }
     a0r = oct_sum(a0r);
     a0i = oct_sum(a0i);
@@ -832,10 +838,11 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
       nref = k + 1;
       const int rows = n - (i + 1);
       const int nt = (rows + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
-      if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv], s));
-      hipLaunchKernelGGL(k_ztri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
+      // timed launches carry their events in the dispatch itself
+      if (kt) hipExtLaunchKernelGGL(k_ztri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, c.kev[2 * nsymv],
+                                    c.kev[2 * nsymv + 1], 0, g, i, nt);
+      else hipLaunchKernelGGL(k_ztri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
       W2K_STAGE(s, "k_ztri_symv");
-      if (kt) W2K_HIP(hipEventRecord(c.kev[2 * nsymv + 1], s));
       nsymv++;
       const int next = (k + 1 < nb) ? i + 1 : -1;
       hipLaunchKernelGGL(k_ztri_col, dim3((rows + ZCOL_ROWS - 1) / ZCOL_ROWS), dim3(256), 0, s, g, i, next, nt, nt, nblk);

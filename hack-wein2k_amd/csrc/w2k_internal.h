@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "../../include/w2k_lapw1.h"
@@ -47,8 +48,17 @@ template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
   int alloc(size_t count);
   void release();
+  // exchange the buffers of two handles (nothing allocated or freed)
+  void swap(DBuf &o)
+  {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
   ~DBuf() { release(); }
 };
 

@@ -10,7 +10,8 @@
 // Device memory: every allocation gets its own mapping, with the buffer placed
 // against a PROT_NONE guard page (its end by default, its start with
 // W2K_EMU_ALIGN=start) and filled with the byte 0x41 (doubles of about 2e6,
-// ints of about 1.1e9, so a never-written index runs far out of bounds).  A
+// ints of about 1.1e9, so a never-written index runs far out of bounds), or
+// with 0xFF (NaN doubles, ints of -1) when W2K_EMU_POISON=nan.  A
 // freed buffer is unmapped to PROT_NONE and its address range is never
 // reused.  A fault prints the kernel, block, work-item and the buffer that
 // the address belongs to (or lies past), then ends the process with status 86.
@@ -544,6 +545,18 @@ struct Init {
 
 }  // namespace
 
+// fill byte of new allocations: 0x41 by default; W2K_EMU_POISON=nan fills
+// with 0xFF (every double a NaN, every int -1), so that a value read before
+// it was written and then "masked" by a multiplication with zero still shows
+static int poison_byte()
+{
+  static const int b = [] {
+    const char *e = getenv("W2K_EMU_POISON");
+    return (e && strcmp(e, "nan") == 0) ? 0xFF : 0x41;
+  }();
+  return b;
+}
+
 hipError_t hipMalloc(void **p, size_t bytes)
 {
   if (!p) return hipErrorInvalidValue;
@@ -557,7 +570,7 @@ hipError_t hipMalloc(void **p, size_t bytes)
     return hipErrorOutOfMemory;
   }
   char *user = align_start() ? base + PAGE : base + PAGE + pages - data;
-  memset(base + PAGE, 0x41, pages);
+  memset(base + PAGE, poison_byte(), pages);
   {
     std::lock_guard<std::mutex> lk(g_mem_mu);
     g_regions[base] = Region{base, total, user, bytes, false};

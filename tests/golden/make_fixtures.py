@@ -228,7 +228,11 @@ def make_c3(name: str, workdir: str, seconds: float, frac: float = 0.2):
 
 def main(argv):
     names = argv or ["mgo", "c1_fcc", "lo_mult2", "lo_cplx", "rs8"]
-    tmp = tempfile.mkdtemp(prefix="w2kfix_")
+    # W2K_FIXTURE_WORKDIR: run the reference there and keep its outputs (the
+    # 100-atom cell takes about an hour)
+    keep = os.environ.get("W2K_FIXTURE_WORKDIR")
+    tmp = keep or tempfile.mkdtemp(prefix="w2kfix_")
+    os.makedirs(tmp, exist_ok=True)
     try:
         for name in names:
             if name == "mgo":
@@ -258,8 +262,12 @@ def main(argv):
                 # the window (EL, EU'] of the lowest ~20% of the spectrum is
                 # cut from it: same matrices, same eigenvalues (bisection to
                 # ABSTOL = 0 does not depend on the window)
-                W.write_case(case, d)
-                sec = run_reference(d, True)
+                done = os.path.join(d, "ref_times.txt")
+                if keep and os.path.exists(done) and os.path.exists(os.path.join(d, "k0001.bin")):
+                    sec = float(dict(l.split() for l in open(done))["total_wall_s"])  # kept run
+                else:
+                    W.write_case(case, d)
+                    sec = run_reference(d, True)
                 make_c3(name, d, sec)
                 continue
             if name == "c2_rs64":
@@ -279,7 +287,8 @@ def main(argv):
             nk_full = {"c2_rs64": 0, "c1_fcc": 2}.get(name, 1)
             save_case(name, d, nk_full, case.complex, sec)
     finally:
-        shutil.rmtree(tmp, ignore_errors=True)
+        if not keep:
+            shutil.rmtree(tmp, ignore_errors=True)
 
 
 if __name__ == "__main__":

@@ -18,11 +18,11 @@ def read_dump(path: str) -> dict:
     while off < len(buf):
         name = buf[off:off + 32].decode().strip()
         off += 32
-        kind, ndim = np.frombuffer(buf, np.int32, 2, off)
+        kind, ndim = (int(x) for x in np.frombuffer(buf, np.int32, 2, off))
         off += 8
         dims = tuple(int(x) for x in np.frombuffer(buf, np.int32, ndim, off))
         off += 4 * ndim
-        dt = np.dtype(_DT[int(kind)])
+        dt = np.dtype(_DT[kind])
         cnt = int(np.prod(dims))
         arr = np.frombuffer(buf, dt, cnt, off).reshape(dims, order="F").copy()
         off += cnt * dt.itemsize
