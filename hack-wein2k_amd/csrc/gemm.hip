@@ -419,7 +419,8 @@ int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, co
   }
   int variant = (m >= 256 && n >= 256 && k >= 256) ? 1 : 0;
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  if (m >= 128 && n >= 128 && k >= 64 && tuning_on() && hipStreamIsCapturing(s, &cap) == hipSuccess &&
+  // (not inside the allocation-free entry point, which creates no events)
+  if (m >= 128 && n >= 128 && k >= 64 && tuning_on() && !g_arena && hipStreamIsCapturing(s, &cap) == hipSuccess &&
       cap == hipStreamCaptureStatusNone) {
     int dev = 0;
     (void)hipGetDevice(&dev);

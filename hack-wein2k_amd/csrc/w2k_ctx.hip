@@ -5,12 +5,20 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
 
 #include "w2k_internal.h"
 #include "../../include/w2k_lapw1_test.h"
+
+// Scratch of w2k_seclr4_noalloc: every SECLR4 workspace of that entry point
+// is carved from this array (each device holds its own copy for the life of
+// the process).  Host code never names it except to look up its device
+// address once per device (noalloc_scratch_init); the kernels receive that
+// address through the workspaces carved from it.
+static __device__ __align__(256) unsigned char scratch[W2K_NOALLOC_SCRATCH_BYTES];
 
 namespace w2k {
 
@@ -54,12 +62,27 @@ int stage_check(hipStream_t s, const char *name, const char *file, int line)
   return 0;
 }
 
+thread_local Arena *g_arena = nullptr;
+
 template <class T>
 int DBuf<T>::alloc(size_t count)
 {
   if (count <= n && p) return 0;
   release();
   if (count == 0) count = 1;
+  if (g_arena) {
+    const size_t bytes = (sizeof(T) * count + 255) & ~(size_t)255;
+    if (g_arena->used + bytes > g_arena->size) {
+      set_last_error("scratch of the allocation-free entry point exhausted (%zu of %zu bytes in use, %zu more asked)",
+                     g_arena->used, g_arena->size, bytes);
+      return W2K_ENOMEM;
+    }
+    p = reinterpret_cast<T *>(g_arena->base + g_arena->used);
+    g_arena->used += bytes;
+    n = count;
+    view = true;
+    return 0;
+  }
   if (hipMalloc(&p, sizeof(T) * count) != hipSuccess) {
     p = nullptr;
     n = 0;
@@ -71,9 +94,10 @@ int DBuf<T>::alloc(size_t count)
 template <class T>
 void DBuf<T>::release()
 {
-  if (p) (void)hipFree(p);
+  if (p && !view) (void)hipFree(p);
   p = nullptr;
   n = 0;
+  view = false;
 }
 template struct DBuf<double>;
 template struct DBuf<int>;
@@ -460,6 +484,51 @@ static int ctx_open(w2k_ctx **out, int device)
   return 0;
 }
 
+// device address of `scratch` per device, looked up once per device with
+// that device current (at the first context created on it)
+static std::mutex g_scratch_mu;
+static std::vector<void *> g_scratch_ptr;
+
+static int noalloc_scratch_init(int device)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return 0;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return w2k_hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+  g_scratch_ptr[device] = ptr;
+  return 0;
+}
+
+static void *noalloc_scratch(int device)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
+
+// an upper bound of the device memory (bytes) SECLR4 takes from the arena
+// for order n and at most m eigenpairs: every workspace of seclr4_real /
+// seclr4_complex, seclr4_buffers and tridiag_eigen, each rounded to 256 bytes
+static size_t noalloc_bytes(size_t n, size_t m, int cplx)
+{
+  const size_t z = cplx ? 2 : 1, lda = (n + 15) & ~(size_t)15, nt = (n + 63) / 64;
+  auto r = [](size_t doubles) { return (doubles * 8 + 255) & ~(size_t)255; };
+  size_t b = 0;
+  b += cplx ? 4 * r(2 * n * n) : 3 * r(n * n) + r(lda * n);                  // U, H/C, W, X
+  b += r((size_t)1 << 22);                                                   // split-K partials
+  b += r(12 * n + z * 2 * n + z * 3 * n * 32 + z * nt * n + z * nt * 64 + nt * (nt + 1) / 2 + 2 + nt + n / 32 +
+         64 + 64 + 768);                                                     // tridiagonal work
+  b += r((6 * n + 32 + 1) / 2);                                              // ints
+  b += r(m + 1);                                                             // eigenvalues
+  const size_t zp = ((n + 2) / 2) * 2 * m;
+  b += r(5 * zp) + r((zp + 1) / 2);                                          // inverse iteration
+  b += r(2 * n * 64 + 2 * 64 * 64 + m * 64 + m + 1);                         // clusters
+  b += r(z * n * (m + 1));                                                   // Z
+  b += cplx ? r(2 * (2 * n + 3 * 64 + 2 * m) * 64 + 64 * 64 * 64) : r((2 * n + 2 * m + 3 * 128) * 128 + 32 * 128 * 128);
+  return b;
+}
+
 int w2k_create(w2k_ctx **out, int device, const w2k_run_desc *run)
 {
   if (!out || !run) return W2K_EINVAL;
@@ -468,6 +537,7 @@ int w2k_create(w2k_ctx **out, int device, const w2k_run_desc *run)
   int rc = ctx_open(&x, device);
   if (rc) return rc;
   rc = run_upload(x->c, run);
+  if (!rc) rc = noalloc_scratch_init(device);
   if (rc) {
     w2k_destroy(x);
     return rc;
@@ -612,6 +682,82 @@ int w2k_seclr4(w2k_ctx *x, double el, double eu, int nume, int *ne, int *nbelw)
   if (!rc) rc = rc2;
   W2K_HIP(hipStreamSynchronize(c.stream));
   take_ms(c, 2, 4, 5);
+  return rc;
+}
+
+int w2k_seclr4_noalloc(w2k_ctx *x, int n, double *hs, double *hsdiag, double el, double eu, int nume, int *ne,
+                       int *nbelw, double *eigval, double *z, int ldz)
+{
+  if (!x || n < 1 || !hs || !hsdiag || !ne || !nbelw || !eigval || !z || ldz < n || nume < 0) return W2K_EINVAL;
+  Ctx &c = x->c;
+  const int cplx = c.run().is_complex;
+  const size_t m = (size_t)(nume > 0 ? std::min(nume, n) : n);
+  const size_t need = noalloc_bytes((size_t)n, m, cplx);
+  if (n > W2K_NOALLOC_MAXN || need > (size_t)W2K_NOALLOC_SCRATCH_BYTES) {
+    set_last_error("w2k_seclr4_noalloc: order %d with up to %zu eigenpairs needs %zu bytes of scratch; the scratch holds "
+                   "%zu bytes (orders up to %d always fit)", n, m, need, (size_t)W2K_NOALLOC_SCRATCH_BYTES,
+                   W2K_NOALLOC_MAXN);
+    return W2K_EINVAL;
+  }
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  void *base = noalloc_scratch(c.device);
+  if (!base) return W2K_ESTATE;
+  // the context's own per-k buffers and graph key are set aside (not freed)
+  // and restored afterwards; HS/HSDIAG are the caller's buffers, every
+  // workspace and the eigenpairs come from the scratch
+  DBuf<double> *dbufs[] = {&c.d_hs, &c.d_hsdiag, &c.d_eig, &c.d_z, &c.w_a, &c.w_b, &c.w_c, &c.w_v, &c.w_x,
+                           &c.w_small, &c.w_fac, &c.w_bt, &c.w_cl, &c.w_gk};
+  DBuf<int> *ibufs[] = {&c.w_int, &c.w_piv};
+  constexpr int ND = sizeof(dbufs) / sizeof(dbufs[0]), NI = sizeof(ibufs) / sizeof(ibufs[0]);
+  DBuf<double> dsave[ND];
+  DBuf<int> isave[NI];
+  for (int q = 0; q < ND; q++) dbufs[q]->swap(dsave[q]);
+  for (int q = 0; q < NI; q++) ibufs[q]->swap(isave[q]);
+  Ctx::TriGraph tg_save = c.tri_graph;
+  c.tri_graph = Ctx::TriGraph();
+  const int n_save = c.n, ne_save = c.ne, nbelw_save = c.nbelw, graphs_save = c.graphs, kt_save = c.ktime;
+  const int hb = c.have_basis, hh = c.have_hamilt, ho = c.have_hns_ops;
+  c.d_hs.p = hs; c.d_hs.n = (size_t)(cplx ? 2 : 1) * n * n; c.d_hs.view = true;
+  c.d_hsdiag.p = hsdiag; c.d_hsdiag.n = (size_t)(cplx ? 2 : 1) * n; c.d_hsdiag.view = true;
+  c.n = n;
+  c.graphs = 0;
+  c.ktime = 0;
+  Arena arena;
+  arena.base = static_cast<char *>(base);
+  arena.size = W2K_NOALLOC_SCRATCH_BYTES;
+  g_arena = &arena;
+  int rc = w2k::seclr4(c, el, eu, nume, ne, nbelw);
+  if (rc == 0 && *ne > 0) {
+    const size_t es = sizeof(double) * (cplx ? 2 : 1);
+    hipError_t e = hipMemcpy2DAsync(z, es * ldz, c.d_z.p, es * n, es * n, *ne, hipMemcpyDeviceToDevice, c.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(eigval, c.d_eig.p, sizeof(double) * *ne, hipMemcpyDeviceToDevice, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) rc = w2k_hip_fail(e, "w2k_seclr4_noalloc: result copy", __FILE__, __LINE__);
+  }
+  if (rc != 0) {
+    // nothing may still read the scratch when the call returns
+    (void)hipStreamSynchronize(c.stream);
+    (void)hipStreamSynchronize(c.aux);
+  }
+  g_arena = nullptr;
+  for (int q = 0; q < ND; q++) {
+    dbufs[q]->release();  // views only: nothing is freed
+    dbufs[q]->swap(dsave[q]);
+  }
+  for (int q = 0; q < NI; q++) {
+    ibufs[q]->release();
+    ibufs[q]->swap(isave[q]);
+  }
+  c.tri_graph = tg_save;
+  c.n = n_save;
+  c.ne = ne_save;
+  c.nbelw = nbelw_save;
+  c.graphs = graphs_save;
+  c.ktime = kt_save;
+  c.have_basis = hb;
+  c.have_hamilt = hh;
+  c.have_hns_ops = ho;
   return rc;
 }
 

@@ -44,10 +44,21 @@ int stage_check(hipStream_t s, const char *name, const char *file, int line);
   } while (0)
 
 // ---------------------------------------------------------------- device buffers
+// Bump arena of the allocation-free entry point (w2k_seclr4_noalloc): while
+// g_arena is set on a thread, DBuf::alloc on that thread carves its buffer
+// from the arena (256-byte pieces) instead of calling hipMalloc, and fails
+// with W2K_ENOMEM when the arena is exhausted.
+struct Arena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+};
+extern thread_local Arena *g_arena;
+
 template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0;
+  bool view = false;  // memory not owned (arena piece or caller's buffer): never freed
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
@@ -58,6 +69,7 @@ struct DBuf {
   {
     std::swap(p, o.p);
     std::swap(n, o.n);
+    std::swap(view, o.view);
   }
   ~DBuf() { release(); }
 };

@@ -122,6 +122,27 @@ int w2k_hns(w2k_ctx *ctx);
  * number of eigenvalues <= el.  HS is destroyed (as in the reference).
  * Eigenvalues/vectors stay on the device; fetch with w2k_get_eig. */
 int w2k_seclr4(w2k_ctx *ctx, double el, double eu, int nume, int *ne, int *nbelw);
+/* Allocation-free SECLR4 for callers that cannot call hipMalloc (code under
+ * HIP graph capture elsewhere in their program, tight serving loops,
+ * harnesses that forbid allocation).  It has no counterpart in the reference:
+ * it is w2k_seclr4 on caller-owned device buffers, with every workspace
+ * carved from a static device array of W2K_NOALLOC_SCRATCH_BYTES that each
+ * device holds for the life of the process; it calls no hipMalloc, hipFree
+ * or allocating library routine (the context itself is created beforehand
+ * with w2k_create).  hs (N x N, ld = N) and hsdiag are the device HS/HSDIAG in
+ * the reference layout (complex interleaved) and are destroyed, as by
+ * w2k_seclr4; on return eigval[0..*ne) and Z(1:N, 1:*ne) at z (leading
+ * dimension ldz >= N) are written on the device.  Results equal w2k_seclr4's
+ * on the same HS.  Orders up to W2K_NOALLOC_MAXN always fit the scratch; a
+ * larger order, or one whose workspaces would not fit, returns W2K_EINVAL
+ * before any work is queued (w2k_last_error says why).  Every call on a
+ * device shares that device's scratch, so calls that overlap on one device
+ * (from different threads or contexts) race; calls on one context are
+ * serial.  The context's own HS and eigenpairs are left untouched. */
+#define W2K_NOALLOC_SCRATCH_BYTES (128u << 20)
+#define W2K_NOALLOC_MAXN 768
+int w2k_seclr4_noalloc(w2k_ctx *ctx, int n, double *hs, double *hsdiag, double el, double eu, int nume,
+                       int *ne, int *nbelw, double *eigval, double *z, int ldz);
 /* The whole CALKPT hot path on the device: HAMILT, diag swap, HNS (unless
  * nohns != 0), SECLR4 (calkpt.F:155-196). */
 int w2k_calkpt(w2k_ctx *ctx, int nohns, double el, double eu, int nume,
