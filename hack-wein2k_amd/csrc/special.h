@@ -2,9 +2,9 @@
 // reference algorithms so that tables built on the GPU agree with HAMILT/HNS:
 //   sphbes  -- sphbes.f:47-142 (downward recursion normalised by the sum
 //              rule; power series for x <= 0.1)
-//   dvbes1  -- dvbes1.f:167-184
+//   dvbes1  -- dvbes1.f:32-49
 //   ylm     -- ylm.f:160-263 (Condon-Shortley recursion)
-//   rotate  -- rotate.f:123-129
+//   rotate  -- rotate.f:22-28
 #pragma once
 #include <hip/hip_runtime.h>
 

@@ -5,10 +5,10 @@
  *
  * Follows, statement by statement where the order of floating-point
  * operations matters:
- *   SPHBES  sphbes.f:47-142        DVBES1  dvbes1.f:167-184
- *   YLM     ylm.f:160-263          ROTATE  rotate.f:123-129
+ *   SPHBES  sphbes.f:47-142        DVBES1  dvbes1.f:32-49
+ *   YLM     ylm.f:160-263          ROTATE  rotate.f:22-28
  *   HAMILT  hamilt.F:142-818       WARPIN  warpin.f:31-37
- *   USTPHX  ustphx.F:74-95         LMSORT  lmsort.f (sort + dedupe semantics)
+ *   USTPHX  ustphx.F:31-52         LMSORT  lmsort.f (sort + dedupe semantics)
  *   LOHNS   lohns.f                HNS     hns.F:150-488
  *   DSYR2M/ZHER2M (lower, 'N','T') dsyr2m.f:66-77, zher2m.f:69-79
  * Compiled with gcc -O2 for baseline x86-64 (no FMA contraction), like the
@@ -108,7 +108,7 @@ void oracle_sphbes(int n, double x, double *fj)
   }
 }
 
-/* dvbes1.f:167-184 (nt values, derivative with respect to x = sm*ri) */
+/* dvbes1.f:32-49 (nt values, derivative with respect to x = sm*ri) */
 void oracle_dvbes1(const double *fj, double *dj, double sm, double ri, int nt)
 {
   const double *FJ = fj - 1;
@@ -208,7 +208,7 @@ void oracle_ylm(const double *v, int lmax, cplx *y)
   }
 }
 
-/* rotate.f:123-129: rotvec(j) = sum_k vector(k) * rotmat(j,k) */
+/* rotate.f:22-28: rotvec(j) = sum_k vector(k) * rotmat(j,k) */
 static void rotate3(const double *vec, const double *rot, double *out)
 {
   for (int jc = 0; jc < 3; jc++) {

@@ -72,7 +72,7 @@ class Fixture:
         return f"k{i}/HS_HAMILT" in self.raw
 
 
-CASES = ["c1_fcc", "mgo", "lo_mult2", "lo_cplx", "rs8"]
+CASES = ["c1_fcc", "mgo", "lo_mult2", "lo_cplx", "rs8", "ins_lo", "ins_lo_cplx"]
 
 
 @pytest.fixture(scope="session", params=CASES)

@@ -70,6 +70,10 @@ class Case:
     alpha: Tuple[float, float, float] = (90.0, 90.0, 90.0)
     vns_amp: float = 1.0e-2
     potk_amp: float = 0.05
+    # False: an empty case.vns, so that ATPAR's read of it ends at once and
+    # INS = IOSTAT /= 0 (atpar.f:105): no non-spherical potential and no
+    # warped interstitial box, HAMILT takes its USTPHX branch (hamilt.F:348-364)
+    vns: bool = True
 
 
 def ops_identity() -> List[Tuple[np.ndarray, np.ndarray]]:
@@ -366,7 +370,10 @@ def write_case(case: Case, d: str) -> None:
     write_in1(case, os.path.join(d, f"{c}.in1"))
     write_klist(case, os.path.join(d, f"{c}.klist"))
     write_vsp(case, os.path.join(d, f"{c}.vsp"), rng)
-    write_vns(case, os.path.join(d, f"{c}.vns"), rng)
+    if case.vns:
+        write_vns(case, os.path.join(d, f"{c}.vns"), rng)
+    else:
+        open(os.path.join(d, f"{c}.vns"), "w").close()
     with open(os.path.join(d, "lapw1.def"), "w") as f:
         f.write(DEF_TEMPLATE.format(c=c))
 
@@ -381,7 +388,8 @@ def _vlm_set(lnsmax: int, rng: np.random.Generator, count: int) -> List[Tuple[in
 
 def case_c1_fcc(nk: int = 10) -> Case:
     """Monatomic fcc a=7.60 bohr, Z=13, RMT 2.50, RKmax 7, lmax 10, lnsmax 4,
-    10 seeded k-points on a 1/8 grid, real (inversion symmetric)."""
+    10 seeded k-points on a 1/8 grid, real (inversion symmetric); the window
+    (-9, 4] Ry holds the lowest 20 to 23 eigenvalues of every k-point."""
     rng = np.random.Generator(np.random.PCG64(SEED_BASE + 0))
     ks = [(0, 0, 0, 8)]
     while len(ks) < nk:
@@ -391,7 +399,7 @@ def case_c1_fcc(nk: int = 10) -> Case:
     at = AtomType("Al", 13.0, 2.50, [(0.0, 0.0, 0.0)], iatnr=-1,
                   vlm=[(2, 0), (4, 0), (4, 4), (6, 0), (6, 4)])
     return Case("c1_fcc", "fcc Al-like cell, seeded potential", "F", (7.60, 7.60, 7.60), [at],
-                ops_inversion(), kpoints=ks, window=(-9.0, 3.0), seed=SEED_BASE + 0)
+                ops_inversion(), kpoints=ks, window=(-9.0, 4.0), seed=SEED_BASE + 0)
 
 
 def case_lo_mult2(complex_: bool = False) -> Case:
@@ -414,6 +422,16 @@ def case_lo_mult2(complex_: bool = False) -> Case:
     return Case("lo_cplx" if complex_ else "lo_mult2", "LO/APW+lo/MULT test cell", "P", (6.1, 6.6, 7.2), atoms,
                 ops, rkmax=5.5, lmax=8, kpoints=[(1, 2, 3, 8), (0, 0, 0, 1)] if complex_ else [(0, 0, 0, 1), (1, 0, 2, 6)],
                 window=(-4.0, 4.0), seed=seed, complex=complex_)
+
+
+def case_ins_lo(complex_: bool = False) -> Case:
+    """The LO/APW+lo/MULT cell without a non-spherical potential file: INS /= 0,
+    so HAMILT builds the interstitial part with USTPHX and no warped box."""
+    c = case_lo_mult2(complex_)
+    c.name = "ins_lo_cplx" if complex_ else "ins_lo"
+    c.title = "LO/APW+lo/MULT test cell, no vns (INS /= 0)"
+    c.vns = False
+    return c
 
 
 def rocksalt_positions(n: int) -> Tuple[list, list]:
@@ -536,4 +554,6 @@ CASES = {
     "c2_rs64": lambda: _with_window(case_rocksalt(2, 7.0, "c2_rs64", SEED_BASE + 1), (-9.0, 12.0)),
     "c4_ortho8": case_c4_ortho8,
     "c3_lowsym100": case_c3_lowsym100,
+    "ins_lo": lambda: case_ins_lo(False),
+    "ins_lo_cplx": lambda: case_ins_lo(True),
 }

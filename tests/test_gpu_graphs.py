@@ -9,9 +9,9 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import Fixture, PKG
+from conftest import EMU, Fixture, PKG
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(EMU, reason="HIP graph capture is not part of the CPU build (W2K_TEST_EMU=1)")]
 L = PKG.lapw1
 MODE = L.Lapw1.TIMING_NAMES.index("tri_launch_mode")
 
