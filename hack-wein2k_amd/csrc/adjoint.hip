@@ -115,6 +115,22 @@ int adjoint_gamma(Ctx &c, const double *d_f, int rows, double *Gr, double *Gi)
 
 int adjoint_warp(Ctx &c, const double *f_host, double *e_band, double *g_warp)
 {
+  const double *box = nullptr;
+  int rc = adjoint_warp_box(c, f_host, e_band, g_warp ? &box : nullptr);
+  if (rc || !g_warp) return rc;
+  RunDev &R = c.run();
+  const size_t nbox = (size_t)(2 * R.warp1 + 1) * (2 * R.warp2 + 1) * (2 * R.warp3 + 1);
+  if (!box) {
+    std::fill(g_warp, g_warp + 2 * nbox, 0.0);
+    return 0;
+  }
+  W2K_HIP(hipMemcpyAsync(g_warp, box, sizeof(double) * 2 * nbox, hipMemcpyDeviceToHost, c.stream));
+  W2K_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int adjoint_warp_box(Ctx &c, const double *f_host, double *e_band, const double **d_box)
+{
   RunDev &R = c.run();
   const int n = c.n, nv = c.nv, ne = c.ne;
   const int cpx = R.is_complex;
@@ -129,11 +145,9 @@ int adjoint_warp(Ctx &c, const double *f_host, double *e_band, double *g_warp)
   double e = 0.0;
   for (int k = 0; k < ne; k++) e += f_host[k] * ev[k];
   if (e_band) *e_band = e;
-  if (!g_warp) return 0;
-  if (ne == 0 || nv == 0 || R.ins != 0) {
-    std::fill(g_warp, g_warp + 2 * nbox, 0.0);
-    return 0;
-  }
+  if (!d_box) return 0;
+  *d_box = nullptr;  // no box: every gradient is zero
+  if (ne == 0 || nv == 0 || R.ins != 0) return 0;
   // Gamma on the lower triangle of the plane-wave block
   const size_t gsz = (size_t)nv * nv;
   if ((rc = c.w_a.alloc(gsz)) || (cpx && (rc = c.w_b.alloc(gsz)))) return rc;
@@ -162,8 +176,7 @@ int adjoint_warp(Ctx &c, const double *f_host, double *e_band, double *g_warp)
   hipLaunchKernelGGL(k_adj_warp_grad, dim3((unsigned)((nbox + 127) / 128)), dim3(128), 0, s, nv, c.d_kzz.p,
                      c.w_int.p, lo[0], lo[1], lo[2], dx, dy, dz, w1, w2, w3, cpx, Gr, cpx ? Gi : Gr, c.w_c.p);
   W2K_STAGE(s, "k_adj_warp_grad");
-  W2K_HIP(hipMemcpyAsync(g_warp, c.w_c.p, sizeof(double) * 2 * nbox, hipMemcpyDeviceToHost, s));
-  W2K_HIP(hipStreamSynchronize(s));
+  *d_box = c.w_c.p;
   return 0;
 }
 

@@ -158,6 +158,14 @@ int run_upload(Ctx &c, const w2k_run_desc *r)
           (r->ilo[l + (W2K_LOMAX + 1) * a] > 0 && l >= r->ltop))
         return W2K_EINVAL;
   R.is_complex = r->is_complex ? 1 : 0;
+  // host copies that ATPAR's radial part reads (energies, local orbitals)
+  R.e.assign(r->e, r->e + (size_t)W2K_LMAX * r->nat);
+  R.elo.assign(r->elo, r->elo + (size_t)W2K_LOMAXP * W2K_NLOAT * r->nat);
+  R.ilo.assign(r->ilo, r->ilo + (size_t)W2K_LOMAXP * r->nat);
+  R.loor.assign(r->loor, r->loor + (size_t)W2K_LOMAXP * r->nat);
+  // radial functions of an earlier w2k_atpar stay valid for a run of the
+  // same atom types (the V_LM adjoint reads them)
+  if (r->nat != R.nat) R.at_ready = 0;
   R.nat = r->nat;
   R.ndf = r->ndf;
   R.ltop = r->ltop;
@@ -866,6 +874,64 @@ int w2k_adjoint_vns(w2k_ctx *x, const double *f, double *g_vns1, double *g_vns2,
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
   double *const g[6] = {g_vns1, g_vns2, g_vns3, g_vns1lo, g_vns2lo, g_vns3lo};
   return adjoint_vns(c, f, g);
+}
+
+int w2k_set_stars(w2k_ctx *x, const w2k_stars *st)
+{
+  if (!x || !st) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  drop_workers(c);  // they read the box being replaced
+  c.have_hamilt = 0;
+  return set_stars(c, st);
+}
+
+int w2k_get_warp(w2k_ctx *x, int *w1, int *w2, int *w3, double *warp)
+{
+  if (!x || !w1 || !w2 || !w3) return W2K_EINVAL;
+  Ctx &c = x->c;
+  RunDev &R = c.run();
+  *w1 = R.warp1;
+  *w2 = R.warp2;
+  *w3 = R.warp3;
+  if (!warp) return 0;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  const size_t nbox = (size_t)(2 * R.warp1 + 1) * (2 * R.warp2 + 1) * (2 * R.warp3 + 1);
+  W2K_HIP(hipMemcpyAsync(warp, R.d_warp.p, sizeof(double) * 2 * nbox, hipMemcpyDeviceToHost, c.stream));
+  W2K_HIP(hipStreamSynchronize(c.stream));
+  return 0;
+}
+
+int w2k_adjoint_stars(w2k_ctx *x, const double *f, double *e_band, double *g_potk)
+{
+  if (!x || (!f && x->c.ne > 0)) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  return adjoint_stars(c, f, e_band, g_potk);
+}
+
+int w2k_atpar(w2k_ctx *x, const w2k_radial_desc *rad, double *p, double *dp, double *pe, double *dpe, double *pei,
+              double *e_used, double *plo, double *dplo, double *pi12lo, double *pe12lo, double *vns1, double *vns2,
+              double *vns3, double *vns1lo, double *vns2lo, double *vns3lo)
+{
+  if (!x || !rad) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  double *const out[16] = {p, dp, pe, dpe, pei, e_used, plo, dplo, pi12lo, pe12lo, vns1, vns2, vns3, vns1lo, vns2lo,
+                           vns3lo};
+  int rc = atpar(c, rad, out);
+  if (!rc) c.at_lmmxd = rad->lmmxd;
+  return rc;
+}
+
+int w2k_adjoint_vlm(w2k_ctx *x, const double *f, double *g_vlm)
+{
+  if (!x || !g_vlm || (!f && x->c.ne > 0)) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  return adjoint_vlm(c, f, c.at_lmmxd, g_vlm);
 }
 
 int w2k_get_hs(w2k_ctx *x, double *hs, int ld, double *hsdiag)

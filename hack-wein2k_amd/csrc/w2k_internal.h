@@ -92,6 +92,18 @@ struct RunDev {
   DBuf<double> d_rad;     // [a][6][LMAX]: u, up, du, dup, chn, e
   DBuf<int> d_lapw;       // [a][LMAX]
   DBuf<double> d_warp;    // complex box, interleaved
+  // interstitial stars (w2k_set_stars): members of every star from STERN
+  int nkk = 0, iord = 0;
+  DBuf<int> d_kkk;        // [nkk][iord][3] member vectors (first nst[s] valid)
+  DBuf<double> d_taup;    // [nkk][iord][2] phase factors TAUP
+  DBuf<int> d_nst;        // [nkk] star sizes
+  DBuf<double> d_potk;    // [nkk][2] POTK
+  // ATPAR's radial functions (w2k_atpar), kept for the V_LM(r) adjoint
+  int at_ready = 0, at_rel = 0, at_jrid = 0, at_nt = 0;
+  DBuf<int> d_at_jrj;
+  DBuf<double> d_at_ro, d_at_dh, d_at_zz;
+  DBuf<double> d_at_fn;    // [a][l < NSLMAX][4][jrid]: A1, B1, AE1, BE1
+  DBuf<double> d_at_fnlo;  // [a][l <= LOMAX][2][jrid]: A1LO, B1LO
   // HNS coupling recipe: every term GFAC * VNS that run_upload adds into a
   // coupling coefficient T_c(a,k,p), kept for the band-energy adjoint
   // (dE/dVNS = Re(GFAC * dE/dT) summed over the terms that read it)
@@ -155,6 +167,7 @@ struct Ctx {
   DBuf<double> h_re, h_im;      // complex build: planar products
   int have_hns_ops = 0;         // L/R operands of the last HNS are valid (same basis)
   int hns_slices = 0;           // 0: FP64 rank-K update; 3..7: INT8 digit slices (split precision)
+  int at_lmmxd = 0;             // second dimension of V_LM of the last w2k_atpar
   DBuf<signed char> w_dig;      // split precision: digit planes and row exponents
   // eigen results
   int ne = 0, nbelw = 0;
@@ -200,6 +213,21 @@ int hns(Ctx &c);
 int seclr4(Ctx &c, double el, double eu, int nume, int *ne, int *nbelw);
 // band-energy adjoint with respect to the warped interstitial box (after SECLR4)
 int adjoint_warp(Ctx &c, const double *f_host, double *e_band, double *g_warp);
+// the same gradient left on the device (*d_box: interleaved box of the run's
+// layout in a context workspace, or nullptr when every entry is zero)
+int adjoint_warp_box(Ctx &c, const double *f_host, double *e_band, const double **d_box);
+// STERN + SETWAR on the device: the run's warped box from its stars
+int set_stars(Ctx &c, const w2k_stars *st);
+// band-energy gradient with respect to the star coefficients POTK (after SECLR4)
+int adjoint_stars(Ctx &c, const double *f_host, double *e_band, double *g_potk);
+// ATPAR's radial part on the device; outputs in the reference layouts
+// (host, any may be NULL): out[0..5] P, DP, PE, DPE, PEI, E (LMAX, NAT);
+// out[6..9] PLO, DPLO, PI12LO, PE12LO (LOMAX+1, NAT); out[10..15] VNS1..3,
+// VNS1LO..3LO (run descriptor layouts)
+int atpar(Ctx &c, const w2k_radial_desc *rd, double *const out[16]);
+// band-energy gradient with respect to V_LM(r) (after a CALKPT with HNS and
+// w2k_atpar on this run): g (jrid, lmmxd, NAT)
+int adjoint_vlm(Ctx &c, const double *f_host, int lmmxd, double *g_vlm);
 // Gamma = Z diag(f) Z^H on the lower triangle of the leading `rows` rows,
 // planar (Gi only for the complex build); d_f on the device
 int adjoint_gamma(Ctx &c, const double *d_f, int rows, double *Gr, double *Gi);

@@ -165,6 +165,16 @@ class _BasisDesc(ctypes.Structure):
                 ("zk", ctypes.POINTER(ctypes.c_double)), ("rk", ctypes.POINTER(ctypes.c_double))]
 
 
+class _StarsDesc(ctypes.Structure):
+    _fields_ = [("nkk", ctypes.c_int), ("ja", _c_int_p), ("jb", _c_int_p), ("jc", _c_int_p),
+                ("potk", _c_dbl_p), ("iord", ctypes.c_int), ("imat", _c_int_p), ("tau", _c_dbl_p)]
+
+
+class _RadialDesc(ctypes.Structure):
+    _fields_ = [("rel", ctypes.c_int), ("jrid", ctypes.c_int), ("jrj", _c_int_p), ("ro", _c_dbl_p), ("dh", _c_dbl_p),
+                ("zz", _c_dbl_p), ("vr", _c_dbl_p), ("lmmxd", ctypes.c_int), ("vlm", _c_dbl_p)]
+
+
 class Lapw1Error(RuntimeError):
     def __init__(self, where: str, code: int, detail: str = ""):
         msg = f"{where} failed with status {code}"
@@ -211,6 +221,11 @@ def load_library(path: Optional[str] = None):
     lib.w2k_set_graphs.argtypes = [V, ctypes.c_int]
     lib.w2k_adjoint_warp.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p]
     lib.w2k_adjoint_vns.argtypes = [V, _c_dbl_p] + [_c_dbl_p] * 6
+    lib.w2k_set_stars.argtypes = [V, ctypes.POINTER(_StarsDesc)]
+    lib.w2k_get_warp.argtypes = [V, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p]
+    lib.w2k_adjoint_stars.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p]
+    lib.w2k_atpar.argtypes = [V, ctypes.POINTER(_RadialDesc)] + [_c_dbl_p] * 16
+    lib.w2k_adjoint_vlm.argtypes = [V, _c_dbl_p, _c_dbl_p]
     lib.w2k_calkpt_batched.argtypes = [V, ctypes.c_int, ctypes.POINTER(_BasisDesc), ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                        ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p,
                                        ctypes.c_int, ctypes.POINTER(_c_dbl_p), _c_int_p]
@@ -235,6 +250,12 @@ class Lapw1:
             raise Lapw1Error("w2k_create", rc)
         self.basis: Optional[Basis] = None
         self.ne = 0
+        self._nkk = 0
+
+    def set_run(self, run: RunData):
+        """Replace the per-run data (w2k_set_run)."""
+        self.run = run
+        self._chk("w2k_set_run", self.lib.w2k_set_run(self.ctx, ctypes.byref(run.desc())))
 
     def close(self):
         if self.ctx:
@@ -322,9 +343,47 @@ class Lapw1:
         if f.shape != (self.ne,):
             raise ValueError(f"need {self.ne} weights")
         e = ctypes.c_double(0.0)
-        w1, w2, w3 = self.run.warp
+        w1, w2, w3 = self.warp_dims()
         g = np.zeros((2 * w1 + 1, 2 * w2 + 1, 2 * w3 + 1), np.complex128, order="F")
         self._chk("w2k_adjoint_warp", self.lib.w2k_adjoint_warp(self.ctx, _dp(f), ctypes.byref(e), _dp(g)))
+        return e.value, g
+
+    def set_stars(self, ja, jb, jc, potk, imat, tau):
+        """STERN + SETWAR on the device: the run's warped box from the
+        interstitial stars (JA/JB/JC, POTK) and the space group (IMAT(3,3,IORD),
+        TAU(3,IORD)), replacing the box of the run data."""
+        ja, jb, jc = (np.ascontiguousarray(np.asarray(x, np.int32).ravel()) for x in (ja, jb, jc))
+        potk = np.ascontiguousarray(np.asarray(potk, np.complex128).ravel())
+        imat = np.asfortranarray(np.asarray(imat, np.int32))
+        tau = np.asfortranarray(np.asarray(tau, np.float64))
+        d = _StarsDesc(len(ja), ja.ctypes.data_as(_c_int_p), jb.ctypes.data_as(_c_int_p), jc.ctypes.data_as(_c_int_p),
+                       _dp(potk), int(imat.shape[2]), imat.ctypes.data_as(_c_int_p), _dp(tau))
+        self._chk("w2k_set_stars", self.lib.w2k_set_stars(self.ctx, ctypes.byref(d)))
+        self._nkk = len(ja)
+
+    def warp_dims(self) -> Tuple[int, int, int]:
+        w = [ctypes.c_int(0) for _ in range(3)]
+        self._chk("w2k_get_warp", self.lib.w2k_get_warp(self.ctx, *[ctypes.byref(x) for x in w],
+                                                        ctypes.cast(None, _c_dbl_p)))
+        return tuple(x.value for x in w)
+
+    def get_warp(self) -> np.ndarray:
+        """The warped box HAMILT reads: complex (2 w1+1, 2 w2+1, 2 w3+1)."""
+        w1, w2, w3 = self.warp_dims()
+        g = np.zeros((2 * w1 + 1, 2 * w2 + 1, 2 * w3 + 1), np.complex128, order="F")
+        w = [ctypes.c_int(0) for _ in range(3)]
+        self._chk("w2k_get_warp", self.lib.w2k_get_warp(self.ctx, *[ctypes.byref(x) for x in w], _dp(g)))
+        return g
+
+    def adjoint_stars(self, f) -> Tuple[float, np.ndarray]:
+        """Band energy of the last solved k-point and its gradient with respect
+        to the star coefficients POTK (complex: dE/dRe + i dE/dIm)."""
+        f = np.ascontiguousarray(np.asarray(f, np.float64))
+        if f.shape != (self.ne,):
+            raise ValueError(f"need {self.ne} weights")
+        e = ctypes.c_double(0.0)
+        g = np.zeros(self._nkk, np.complex128)
+        self._chk("w2k_adjoint_stars", self.lib.w2k_adjoint_stars(self.ctx, _dp(f), ctypes.byref(e), _dp(g)))
         return e.value, g
 
     def adjoint_vns(self, f) -> dict:
@@ -339,6 +398,44 @@ class Lapw1:
         self._chk("w2k_adjoint_vns", self.lib.w2k_adjoint_vns(
             self.ctx, _dp(f), *[_dp(out[k]) for k in ("vns1", "vns2", "vns3", "vns1lo", "vns2lo", "vns3lo")]))
         return out
+
+    ATPAR_NAMES = ("p", "dp", "pe", "dpe", "pei", "e", "plo", "dplo", "pi12lo", "pe12lo",
+                   "vns1", "vns2", "vns3", "vns1lo", "vns2lo", "vns3lo")
+
+    def atpar(self, rel: bool, jrj, ro, dh, zz, vr, vlm=None) -> dict:
+        """ATPAR's radial part on the device from the radial mesh (JRJ, RO, DH),
+        the nuclear charges, r V00(r) in Hartree (jrid, NAT) and V_LM(r) in Ry
+        (jrid, lmmx, NAT).  Returns the radial parameters and the VNS
+        integrals (entries the coupling list reads) in the reference layouts."""
+        jrj = np.ascontiguousarray(np.asarray(jrj, np.int32).ravel())
+        ro, dh, zz = (np.ascontiguousarray(np.asarray(x, np.float64).ravel()) for x in (ro, dh, zz))
+        vr = np.asfortranarray(np.asarray(vr, np.float64))
+        nat = self.run.nat
+        lmaxp, lomaxp = 13, 4
+        shapes = {k: (lmaxp, nat) for k in ("p", "dp", "pe", "dpe", "pei", "e")}
+        shapes.update({k: (lomaxp, nat) for k in ("plo", "dplo", "pi12lo", "pe12lo")})
+        for k in ("vns1", "vns2", "vns3", "vns1lo", "vns2lo", "vns3lo"):
+            shapes[k] = self.run.arr[k].shape
+        out = {k: np.zeros(sh, np.float64, order="F") for k, sh in shapes.items()}
+        if vlm is not None:
+            vlm = np.asfortranarray(np.asarray(vlm, np.float64))
+        d = _RadialDesc(int(bool(rel)), int(vr.shape[0]), jrj.ctypes.data_as(_c_int_p), _dp(ro), _dp(dh), _dp(zz),
+                        _dp(vr), int(vlm.shape[1]) if vlm is not None else 0,
+                        _dp(vlm) if vlm is not None else ctypes.cast(None, _c_dbl_p))
+        self._chk("w2k_atpar", self.lib.w2k_atpar(self.ctx, ctypes.byref(d),
+                                                  *[_dp(out[k]) for k in self.ATPAR_NAMES]))
+        self._vlm_shape = (int(vr.shape[0]), int(vlm.shape[1]) if vlm is not None else 0, nat)
+        return out
+
+    def adjoint_vlm(self, f) -> np.ndarray:
+        """Gradient of the band energy of the last solved k-point (with HNS)
+        with respect to V_LM(r_k): (jrid, lmmx, NAT) of the last atpar call."""
+        f = np.ascontiguousarray(np.asarray(f, np.float64))
+        if f.shape != (self.ne,):
+            raise ValueError(f"need {self.ne} weights")
+        g = np.zeros(self._vlm_shape, np.float64, order="F")
+        self._chk("w2k_adjoint_vlm", self.lib.w2k_adjoint_vlm(self.ctx, _dp(f), _dp(g)))
+        return g
 
     def get_hs(self) -> Tuple[np.ndarray, np.ndarray]:
         n = self.basis.n

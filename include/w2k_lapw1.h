@@ -201,6 +201,72 @@ int w2k_adjoint_warp(w2k_ctx *ctx, const double *f, double *e_band, double *g_wa
 int w2k_adjoint_vns(w2k_ctx *ctx, const double *f, double *g_vns1, double *g_vns2, double *g_vns3,
                     double *g_vns1lo, double *g_vns2lo, double *g_vns3lo);
 
+/* Interstitial potential stars (module totpot, read by rdswar.f:19-36) and
+ * the space group (module out: IORD, IMAT, TAU). */
+typedef struct w2k_stars {
+  int nkk;                   /* NKK stars                                       */
+  const int *ja, *jb, *jc;   /* JA/JB/JC(NKK): representative of each star      */
+  const double *potk;        /* COMPLEX POTK(NKK), interleaved                  */
+  int iord;                  /* IORD symmetry operations (<= 48)                */
+  const int *imat;           /* IMAT(3,3,IORD)                                  */
+  const double *tau;         /* TAU(3,IORD), non-primitive translations         */
+} w2k_stars;
+/* SETWAR (setwar.f:40-74) with STERN (stern.F:1-97) on the device: the
+ * stars are expanded into the warped interstitial box of the context's run,
+ * WARP(K) = POTK_s conj(TAUP(K)) for every member K of star s, replacing the
+ * box of w2k_run_desc (its extent is that of the star members).  The run
+ * must have INS = 0.  Also keeps the star tables for w2k_adjoint_stars. */
+int w2k_set_stars(w2k_ctx *ctx, const w2k_stars *stars);
+/* The run's warped box as HAMILT reads it: *w1..*w3 its half-widths and, when
+ * warp is non-NULL, the (2 w1+1)(2 w2+1)(2 w3+1) complex entries (Fortran
+ * order, interleaved).  No counterpart in the reference (module out). */
+int w2k_get_warp(w2k_ctx *ctx, int *w1, int *w2, int *w3, double *warp);
+/* Band-energy adjoint with respect to the star coefficients (an extended
+ * form with no counterpart in the reference), after a solved k-point on a
+ * run whose box came from w2k_set_stars: E_band = sum_n f[n] EIGVAL(n) to
+ * *e_band and, when g_potk is non-NULL, dE/dRe POTK_s + i dE/dIm POTK_s for
+ * every star (interleaved; the real build reads the box through its real
+ * part, so its imaginary parts are 0) = sum over the members K of g(K)
+ * TAUP(K), g the warped-box gradient of w2k_adjoint_warp. */
+int w2k_adjoint_stars(w2k_ctx *ctx, const double *f, double *e_band, double *g_potk);
+/* Radial input of ATPAR per atom type (modules potnlc and radial,
+ * atpar.f:89-101 and 352-375). */
+typedef struct w2k_radial_desc {
+  int rel;                /* REL: scalar-relativistic (1) or not (0)         */
+  int jrid;               /* leading dimension of vr and vlm (>= every JRJ)  */
+  const int *jrj;         /* JRJ(NAT): radial mesh points                    */
+  const double *ro;       /* RO(NAT): first mesh point                       */
+  const double *dh;       /* DH(NAT): logarithmic step                       */
+  const double *zz;       /* nuclear charge of every atom type               */
+  const double *vr;       /* VR(jrid, NAT): r V_00(r) in Hartree             */
+  int lmmxd;              /* second dimension of vlm                         */
+  const double *vlm;      /* VLM(jrid, lmmxd, NAT): V_LM(r) in Ry, LM order of
+                           * LQNS(4,:,:) (after CBCOMB's combination for
+                           * cubic sites); NULL: no non-spherical integrals  */
+} w2k_radial_desc;
+/* ATPAR's radial part on the device (atpar.f:212-330, 470-561; OUTWIN
+ * outwin.f:1-113; RINT13 rint13.f:1-64): for every atom type and l < LTOP
+ * the radial function at E_l (the run's E, Ry), its energy derivative and
+ * their boundary values and norms; the local-orbital functions at ELO(l,
+ * ILO(l)) where LOOR(l); and, with vlm, the non-spherical integrals that the
+ * run's HNS coupling list reads.  Outputs (host, any may be NULL) in the
+ * reference layouts: P, DP, PE, DPE, PEI and the energy used (ATPAR lowers E_l
+ * by 0.05 Ry once when P is almost zero) as (LMAX, NAT); PLO, DPLO, PI12LO,
+ * PE12LO as (LOMAX+1, NAT); VNS1..3 and VNS1LO..3LO in the w2k_run_desc
+ * layouts (only the entries the coupling list reads are written).  The
+ * functions stay on the device for w2k_adjoint_vlm. */
+int w2k_atpar(w2k_ctx *ctx, const w2k_radial_desc *rad, double *p, double *dp, double *pe, double *dpe,
+              double *pei, double *e_used, double *plo, double *dplo, double *pi12lo, double *pe12lo,
+              double *vns1, double *vns2, double *vns3, double *vns1lo, double *vns2lo, double *vns3lo);
+/* Band-energy adjoint with respect to the non-spherical potential V_LM(r)
+ * (an extended form with no counterpart in the reference), after a solved
+ * k-point with HNS on a run whose radial functions came from w2k_atpar:
+ * g_vlm(k, lm, a) = dE_band/dV_LM(r_k) of atom type a, layout (jrid, lmmxd,
+ * NAT) of the last w2k_atpar call, = sum over the integrals VNS_x(l, lm, l')
+ * of dE/dVNS_x times RINT13's weight of r_k times the product of the two
+ * radial functions at r_k (large components, and small components times
+ * 1/c^2 when relativistic). */
+int w2k_adjoint_vlm(w2k_ctx *ctx, const double *f, double *g_vlm);
 /* Host <-> device transfers of the reference-layout arrays (ld >= N).  */
 int w2k_get_hs(w2k_ctx *ctx, double *hs, int ld, double *hsdiag);
 int w2k_put_hs(w2k_ctx *ctx, const double *hs, int ld, const double *hsdiag);

@@ -86,7 +86,11 @@ program ref_dump
   use albl, only : AL, BL, END_ALBL
   use loabc, only : ALO, BLO, CLO, ELO, PLO, DPLO, PELO, DPELO, PEILO, PI12LO, PE12LO
   use loint, only : VNS1LO, VNS2LO, VNS3LO
-  use out, only : WARP, WARP1, WARP2, WARP3
+  use out, only : WARP, WARP1, WARP2, WARP3, IORD, IMAT, TAU, NKK
+  use totpot, only : JA, JB, JC, POTK
+  use potnlc, only : JRJ, DH, RO, VR
+  use atspdt, only : LMMAX, LM
+  use coml, only : REL
   implicit none
   include 'param.inc'
   logical :: nmat_only, dodump, eigdump, basisdump
@@ -180,7 +184,25 @@ program ref_dump
   call wd(u, 'VNS1LO', VNS1LO, shape(VNS1LO))
   call wd(u, 'VNS2LO', VNS2LO, shape(VNS2LO))
   call wd(u, 'VNS3LO', VNS3LO, shape(VNS3LO))
+  ! radial mesh and spherical potential (r V in Hartree after ATPAR,
+  ! atpar.f:98-100), the LM list of each atom's V_LM, REL
+  call wi(u, 'JRJ', JRJ, [NAT])
+  call wd(u, 'DH', DH, [NAT])
+  call wd(u, 'RO', RO, [NAT])
+  call wd(u, 'VR', VR, [size(VR,1), NAT])
+  call wi(u, 'LMMAX', LMMAX, [NAT])
+  call wi(u, 'LM', LM, [2, LMMX, NAT])
+  call ws(u, 'REL', merge(1, 0, REL))
   if (INS == 0) then
+    ! interstitial stars (rdswar.f) and the space group (module out)
+    call ws(u, 'IORD', IORD)
+    call wi(u, 'IMAT', IMAT, [3, 3, IORD])
+    call wd(u, 'TAU', TAU, [3, IORD])
+    call ws(u, 'NKK', NKK)
+    call wi(u, 'JA', JA, [NKK])
+    call wi(u, 'JB', JB, [NKK])
+    call wi(u, 'JC', JC, [NKK])
+    call wz(u, 'POTK', POTK, [NKK])
     call ws(u, 'WARP1', WARP1)
     call ws(u, 'WARP2', WARP2)
     call ws(u, 'WARP3', WARP3)

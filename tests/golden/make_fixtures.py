@@ -40,7 +40,7 @@ sys.path.insert(0, HERE)
 import w2k_inputs as W  # noqa: E402
 from w2kdump import read_dump  # noqa: E402
 
-REF_DIR = os.path.join(REPO, "oracle", "_ref")
+REF_DIR = os.environ.get("W2K_REF_DIR", os.path.join(REPO, "oracle", "_ref"))
 MGO_DIR = "/root/reference/SRC_elast/example-MgO/elast/eos"
 
 RUN_KEYS_SCALAR = ["NAT", "NDF", "LTOP", "LNSMAX", "INS", "NLO", "MULTMAX", "LMAXP",
@@ -82,6 +82,19 @@ def crop_run(run: dict) -> dict:
     lmx_used = max(lmx_used, 1)
     for k in ("VNS1", "VNS2", "VNS3", "VNS1LO", "VNS2LO", "VNS3LO"):
         out[k] = run[k][:, :lmx_used, :, :]
+    # radial mesh, spherical potential (r V, Hartree), LM lists, REL
+    if "JRJ" in run:
+        jri = int(np.max(np.atleast_1d(run["JRJ"])))
+        lmmax = np.atleast_1d(run["LMMAX"]).astype(int)
+        nl = max(int(lmmax.max()), 1)
+        out["JRJ"], out["DH"], out["RO"] = run["JRJ"], run["DH"], run["RO"]
+        out["VR"] = run["VR"][:jri, :]
+        out["LMMAX"] = run["LMMAX"]
+        out["LM"] = run["LM"][:, :nl, :]
+        out["REL"] = run["REL"]
+    if int(run["INS"]) == 0 and "NKK" in run:
+        for k in ("IORD", "IMAT", "TAU", "NKK", "JA", "JB", "JC", "POTK"):
+            out[k] = run[k]
     if int(run["INS"]) == 0:
         w = run["WARP"]
         k1, k2, k3 = int(run["WARP1"]), int(run["WARP2"]), int(run["WARP3"])
@@ -133,6 +146,9 @@ def kdata(kd: dict, full: bool, rng: np.random.Generator | None = None) -> dict:
 def save_case(name: str, workdir: str, nk_full: int, complex_: bool, seconds: float, extra: dict | None = None):
     run = read_dump(os.path.join(workdir, "run.bin"))
     payload = {f"run/{k}": v for k, v in crop_run(run).items()}
+    vlm = os.path.join(workdir, "vlm_read.npy")
+    if os.path.exists(vlm):
+        payload["run/VLM"] = np.load(vlm)
     ks = sorted(f for f in os.listdir(workdir) if re.match(r"k\d{4}\.bin$", f))
     rng = np.random.Generator(np.random.PCG64(W.SEED_BASE + 999))
     for i, kf in enumerate(ks):
@@ -284,7 +300,7 @@ def main(argv):
                 shutil.rmtree(d)
             W.write_case(case, d)
             sec = run_reference(d, case.complex)
-            nk_full = {"c2_rs64": 0, "c1_fcc": 2}.get(name, 1)
+            nk_full = {"c2_rs64": 0, "c5_cubic16": 0, "c1_fcc": 2}.get(name, 1)
             save_case(name, d, nk_full, case.complex, sec)
     finally:
         if not keep:

@@ -319,22 +319,28 @@ def warped_potential(case: Case, rng: np.random.Generator):
     return out
 
 
-def write_vns(case: Case, path: str, rng: np.random.Generator) -> None:
+def write_vns(case: Case, path: str, rng: np.random.Generator) -> list:
+    """Writes case.vns; returns per atom the V_LM(r) arrays as the reference
+    reads them back (the written text parsed), in file order."""
     out = ["     TOTAL NONSPHERICAL POTENTIAL OF MT-SPHERES", "     NORM  OF VLM(R) = VLM(R)", ""]
+    vlm_read = []
     for it, at in enumerate(case.atoms, start=1):
         r = radial_mesh(at.rmt)
         out.append(f"   ATOMNUMBER  {it:3d}")
         out.append(f"   NUMBER OF LM{len(at.vlm):3d}")
         out.append("")
         out.append("")
+        vlm_read.append([])
         for (L, M) in at.vlm:
             amp = case.vns_amp * rng.uniform(0.5, 1.5) * (1 if rng.uniform() < 0.5 else -1)
             x = r / at.rmt
             v = amp * x ** L * np.exp(-2.0 * (x - 0.7) ** 2)
             out.append(f"   VLM(R) FOR L{L:3d}   M={M:2d}")
             out.append("")
+            txt = [_fmt_e(val) for val in v]
+            vlm_read[-1].append(np.array([float(t.replace("D", "E")) for t in txt]))
             for j in range(0, NPT, 4):
-                out.append("   " + "".join(_fmt_e(val) for val in v[j:j + 4]))
+                out.append("   " + "".join(txt[j:j + 4]))
             out.append("")
             out.append("")
         out.extend([""] * 4)
@@ -346,6 +352,7 @@ def write_vns(case: Case, path: str, rng: np.random.Generator) -> None:
         out.append(f"   {t[0]:5d}{t[1]:5d}{t[2]:5d}{_fmt_e(v.real)}{_fmt_e(v.imag)}")
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
+    return vlm_read
 
 
 DEF_TEMPLATE = """ 4,'{c}.klist',          'unknown','formatted',0
@@ -371,7 +378,15 @@ def write_case(case: Case, d: str) -> None:
     write_klist(case, os.path.join(d, f"{c}.klist"))
     write_vsp(case, os.path.join(d, f"{c}.vsp"), rng)
     if case.vns:
-        write_vns(case, os.path.join(d, f"{c}.vns"), rng)
+        vlm = write_vns(case, os.path.join(d, f"{c}.vns"), rng)
+        # V_LM(r) as the reference reads it (NPT, max LM count, atom types),
+        # for the radial-integral tests; the reference does not read this file
+        nl = max([len(a) for a in vlm] + [1])
+        arr = np.zeros((NPT, nl, len(vlm)))
+        for it, a in enumerate(vlm):
+            for k, v in enumerate(a):
+                arr[:, k, it] = v
+        np.save(os.path.join(d, "vlm_read.npy"), arr)
     else:
         open(os.path.join(d, f"{c}.vns"), "w").close()
     with open(os.path.join(d, "lapw1.def"), "w") as f:
@@ -541,6 +556,34 @@ def case_c3_lowsym100(rkmax: float = 6.5, name: str = "c3_lowsym100") -> Case:
                 complex=True)
 
 
+def case_c5_cubic16(nk: int = 8) -> Case:
+    """Adjoint cell: a CsCl-type 2x2x2 supercell (16 atoms, Z=12 RMT 2.0 and
+    Z=8 RMT 1.6) in a cubic P cell of 12.8 bohr, every atom its own type,
+    inversion (real build); RKmax 7 (N about 3000 with 64 local orbitals);
+    V_LM with L <= 4; 200 seeded V(G) stars; 8 seeded k-points on a 1/4
+    grid."""
+    rng = np.random.Generator(np.random.PCG64(SEED_BASE + 5))
+    atoms = []
+    for i in range(2):
+        for j in range(2):
+            for k in range(2):
+                atoms.append(AtomType("Mg", 12.0, 2.0, [(i / 2, j / 2, k / 2)], apw_lo=[0, 1],
+                                      vlm=[(2, 0), (2, 2), (4, 0), (4, 4)]))
+    for i in range(2):
+        for j in range(2):
+            for k in range(2):
+                atoms.append(AtomType("O", 8.0, 1.6, [((i + .5) / 2, (j + .5) / 2, (k + .5) / 2)], apw_lo=[0, 1],
+                                      vlm=[(2, 0), (4, 0), (4, 2)]))
+    ks = []
+    while len(ks) < nk:
+        q = tuple(int(x) for x in rng.integers(0, 4, size=3)) + (4,)
+        if q not in ks:
+            ks.append(q)
+    a = 12.8
+    return Case("c5_cubic16", "CsCl-type 2x2x2 supercell (16 atoms), adjoint cell", "P", (a, a, a), atoms,
+                ops_inversion(), rkmax=7.0, kpoints=ks, window=(-9.0, 0.5), nstars=200, seed=SEED_BASE + 5)
+
+
 def _with_window(c: Case, w) -> Case:
     c.window = w
     return c
@@ -554,6 +597,7 @@ CASES = {
     "c2_rs64": lambda: _with_window(case_rocksalt(2, 7.0, "c2_rs64", SEED_BASE + 1), (-9.0, 12.0)),
     "c4_ortho8": case_c4_ortho8,
     "c3_lowsym100": case_c3_lowsym100,
+    "c5_cubic16": case_c5_cubic16,
     "ins_lo": lambda: case_ins_lo(False),
     "ins_lo_cplx": lambda: case_ins_lo(True),
 }
