@@ -305,6 +305,160 @@ __global__ void k_splitk_reduce(int m, int n, int nz, const double *__restrict__
   *cp = v;
 }
 
+// ------------------------------------------------------------------ complex
+// Planar complex product C = alpha op(A) op(B) + beta C (alpha, beta real)
+// in one kernel: each work-group forms a 64x64 tile of both Re C and Im C
+// from the same staged operands (four real MFMA products per k-step, the
+// imaginary parts of op(A) and op(B) signed for conjugation at staging), so
+// C is read and written once instead of four times as by four real GEMMs.
+// op: transpose TA/TB (template), conjugation sa/sb = -1 (runtime); output
+// restricted to a triangle (tri 1 lower, 2 upper) as in k_dgemm.
+struct ZGemmArgs {
+  int m, n, k;
+  double alpha, beta;
+  const double *Ar, *Ai;
+  int lda;
+  const double *Br, *Bi;
+  int ldb;
+  double *Cr, *Ci;
+  int ldc;
+  double sa, sb;  // sign of the imaginary parts of op(A), op(B)
+  int tri;
+};
+
+template <int TA, int TB>
+__global__ __launch_bounds__(256) void k_zgemm(ZGemmArgs g)
+{
+  __shared__ double Ar_s[2][BK][BM + PAD], Ai_s[2][BK][BM + PAD];
+  __shared__ double Br_s[2][BK][BN + PAD], Bi_s[2][BK][BN + PAD];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN;
+  if (g.tri == 1 && i0 + BM - 1 < j0) return;
+  if (g.tri == 2 && j0 + BN - 1 < i0) return;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  d4 cr[2][2], ci[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) {
+      cr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+      ci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+  // this thread's four elements of each operand per slab (rows or k steps as
+  // in k_dgemm), loaded unconditionally from clamped addresses and masked
+  double rar[4], rai[4], rbr[4], rbi[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = TA ? i0 + (tid >> 4) + 16 * r : i0 + (tid & 63);
+      const int kk = TA ? k0 + (tid & 15) : k0 + (tid >> 6) + 4 * r;
+      const bool ok = i < g.m && kk < g.k;
+      const int ic = min(i, g.m - 1), kc = min(kk, g.k - 1);
+      const size_t o = TA ? (size_t)kc + (size_t)g.lda * ic : (size_t)ic + (size_t)g.lda * kc;
+      const double vr = g.Ar[o], vi = g.Ai[o];
+      rar[r] = ok ? vr : 0.0;
+      rai[r] = ok ? g.sa * vi : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int j = TB ? j0 + (tid & 63) : j0 + (tid >> 4) + 16 * r;
+      const int kk = TB ? k0 + (tid >> 6) + 4 * r : k0 + (tid & 15);
+      const bool ok = j < g.n && kk < g.k;
+      const int jc = min(j, g.n - 1), kc = min(kk, g.k - 1);
+      const size_t o = TB ? (size_t)jc + (size_t)g.ldb * kc : (size_t)kc + (size_t)g.ldb * jc;
+      const double vr = g.Br[o], vi = g.Bi[o];
+      rbr[r] = ok ? vr : 0.0;
+      rbi[r] = ok ? g.sb * vi : 0.0;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int ka = TA ? (tid & 15) : (tid >> 6) + 4 * r, ia = TA ? (tid >> 4) + 16 * r : (tid & 63);
+      Ar_s[buf][ka][ia] = rar[r];
+      Ai_s[buf][ka][ia] = rai[r];
+      const int kb = TB ? (tid >> 6) + 4 * r : (tid & 15), jb = TB ? (tid & 63) : (tid >> 4) + 16 * r;
+      Br_s[buf][kb][jb] = rbr[r];
+      Bi_s[buf][kb][jb] = rbi[r];
+    }
+  };
+  if (g.k > 0) {
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < g.k; k0 += BK) {
+      const bool more = (k0 + BK < g.k);
+      if (more) gload(k0 + BK);
+#pragma unroll
+      for (int kq = 0; kq < BK; kq += 4) {
+        double ar[2], ai[2], br[2], bi[2], nbi[2];
+        for (int a = 0; a < 2; a++) {
+          ar[a] = Ar_s[buf][kq + lk][wi + 16 * a + li];
+          ai[a] = Ai_s[buf][kq + lk][wi + 16 * a + li];
+        }
+        for (int b = 0; b < 2; b++) {
+          br[b] = Br_s[buf][kq + lk][wj + 16 * b + li];
+          bi[b] = Bi_s[buf][kq + lk][wj + 16 * b + li];
+          nbi[b] = -bi[b];
+        }
+        for (int a = 0; a < 2; a++)
+          for (int b = 0; b < 2; b++) {
+            cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
+            cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], nbi[b], cr[a][b], 0, 0, 0);
+            ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
+            ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+          }
+      }
+      if (more) {
+        sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+      }
+    }
+  }
+  for (int a = 0; a < 2; a++) {
+    for (int b = 0; b < 2; b++) {
+      const int j = j0 + wj + 16 * b + li;
+      for (int r = 0; r < 4; r++) {
+        const int i = i0 + wi + 16 * a + lk + 4 * r;
+        if (i >= g.m || j >= g.n) continue;
+        if (g.tri == 1 && i < j) continue;
+        if (g.tri == 2 && i > j) continue;
+        const size_t o = (size_t)i + (size_t)g.ldc * j;
+        double vr = g.alpha * cr[a][b][r], vi = g.alpha * ci[a][b][r];
+        if (g.beta != 0.0) {
+          vr += g.beta * g.Cr[o];
+          vi += g.beta * g.Ci[o];
+        }
+        g.Cr[o] = vr;
+        g.Ci[o] = vi;
+      }
+    }
+  }
+}
+
+int zgemm_fused(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *Ar, const double *Ai,
+                int lda, const double *Br, const double *Bi, int ldb, double beta, double *Cr, double *Ci, int ldc,
+                int tri)
+{
+  if (m <= 0 || n <= 0) return 0;
+  ZGemmArgs g;
+  g.m = m; g.n = n; g.k = k > 0 ? k : 0; g.alpha = alpha; g.beta = beta;
+  g.Ar = Ar; g.Ai = Ai; g.lda = lda; g.Br = Br; g.Bi = Bi; g.ldb = ldb; g.Cr = Cr; g.Ci = Ci; g.ldc = ldc;
+  g.sa = (ta == 'C' || ta == 'R') ? -1.0 : 1.0;
+  g.sb = (tb == 'C' || tb == 'R') ? -1.0 : 1.0;
+  g.tri = tri;
+  const bool ttA = (ta == 'T' || ta == 'C'), ttB = (tb == 'T' || tb == 'C');
+  dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
+  if (!ttA && !ttB) hipLaunchKernelGGL((k_zgemm<0, 0>), grid, dim3(256), 0, s, g);
+  else if (ttA && !ttB) hipLaunchKernelGGL((k_zgemm<1, 0>), grid, dim3(256), 0, s, g);
+  else if (!ttA && ttB) hipLaunchKernelGGL((k_zgemm<0, 1>), grid, dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((k_zgemm<1, 1>), grid, dim3(256), 0, s, g);
+  W2K_STAGE(s, "k_zgemm");
+  return 0;
+}
+
 // ------------------------------------------------------------------ tile choice
 // Products that both tile kernels can serve (at least 128 x 128 outputs and
 // K >= 64) get their kernel from a per-device table keyed by the shape class

@@ -30,6 +30,10 @@ static int zgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double al
                  const double *Ai, int lda, const double *Br, const double *Bi, int ldb, double beta, double *Cr,
                  double *Ci, int ldc, int tri = 0, int atri = 0, int btri = 0, double *ws = nullptr, size_t wsn = 0)
 {
+  // products with dense operands and no split K: one fused complex kernel
+  // (C read and written once); short K (rank updates) is where it pays
+  if (atri == 0 && btri == 0 && ws == nullptr && k <= 256)
+    return zgemm_fused(s, ta, tb, m, n, k, alpha, Ar, Ai, lda, Br, Bi, ldb, beta, Cr, Ci, ldc, tri);
   const double sa = (ta == 'C' || ta == 'R') ? -1.0 : 1.0, sb = (tb == 'C' || tb == 'R') ? -1.0 : 1.0;
   const char ra = (ta == 'T' || ta == 'C') ? 'T' : 'N', rb = (tb == 'T' || tb == 'C') ? 'T' : 'N';
   int rc;
@@ -363,6 +367,9 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
   vof(R, vrr, vri);          // v of the lane's row
   double yr = 0.0, yi = 0.0, vyc = 0.0;
   const double *apr = g.ar + (size_t)Rl + (size_t)n * C0, *api = g.ai + (size_t)Rl + (size_t)n * C0;
+  // only the lower triangle of the trailing matrix is kept current: a
+  // diagonal tile uses its lower part by rows and its strict lower part by
+  // columns (A^H v), the column sums folded into the rows they belong to
 #pragma unroll 1
   for (int c0 = 0; c0 < m; c0 += 16) {  // four quarters of 16 columns (wave-uniform)
     double q[32];  // q[j] = Re A(R, C0 + c0 + j), q[16 + j] = Im
@@ -374,27 +381,37 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
     }
 #pragma unroll
     for (int j = 0; j < 16; j++) {
-      const bool ok = live && c0 + j < m;
+      const bool ok = live && c0 + j < m && (off || c0 + j <= lane);
       q[j] = ok ? q[j] : 0.0;
       q[16 + j] = ok ? q[16 + j] : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < 16; j++) cmad(q[j], q[16 + j], dbl_readlane(vcr, c0 + j), dbl_readlane(vci, c0 + j), yr, yi);
-    if (off) {
 #pragma unroll
-      for (int j = 0; j < 16; j++) {  // conj(A(R,c)) v(R)
-        const double ar = q[j], ai = q[16 + j];
-        q[j] = ar * vrr + ai * vri;
-        q[16 + j] = ar * vri - ai * vrr;
-      }
-      // lane l: entry l % 32 summed over the wave (Re of column l % 16 for
-      // l % 32 < 16, Im of column l % 16 above)
-      const double cs = wave_transpose_sum32(q, lane);
+    for (int j = 0; j < 16; j++) {  // conj(A(R,c)) v(R); the diagonal tile's diagonal only by rows
+      const bool use = off || c0 + j < lane;
+      const double ar = q[j], ai = q[16 + j];
+      q[j] = use ? ar * vrr + ai * vri : 0.0;
+      q[16 + j] = use ? ar * vri - ai * vrr : 0.0;
+    }
+    // lane l: entry l % 32 summed over the wave (Re of column l % 16 for
+    // l % 32 < 16, Im of column l % 16 above)
+    const double cs = wave_transpose_sum32(q, lane);
+    if (off) {
       const double co = __shfl_xor(cs, 16, 64);
       const int cl = c0 + (lane & 15);
       if (lane < 32 && cl < m) ((lane & 16) ? g.parti : g.partr)[(size_t)I * n + C0 + cl] = cs;
       const double vcl_r = __shfl(vcr, cl, 64), vcl_i = __shfl(vci, cl, 64);
       if (lane < 16 && cl < m) vyc += vcl_r * cs + vcl_i * co;  // Re(conj(v_c) (A^H v)_c)
+    } else {
+      // column c0 + j of the tile is row C0 + c0 + j: lane c0 + j takes it
+      const int src = lane - c0;
+      const bool mine = src >= 0 && src < 16;
+      const double fr = __shfl(cs, mine ? src : 0, 64), fi = __shfl(cs, mine ? src + 16 : 16, 64);
+      if (mine) {
+        yr += fr;
+        yi += fi;
+      }
     }
   }
   if (live) {
@@ -854,11 +871,11 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
     }
     const int p1 = p0 + nb;
     if (p1 < n) {
-      // A(p1:n, p1:n) -= [V W] [W V]^H
+      // A(p1:n, p1:n) -= [V W] [W V]^H, lower triangle
       const int r = n - p1;
       const size_t o1 = (size_t)p1 + (size_t)n * p1, ow = (size_t)n * nb + p1;
       if ((rc = zgemm(s, 'N', 'C', r, r, 2 * nb, -1.0, g.Pr + p1, g.Pi + p1, n, g.Pr + ow, g.Pi + ow, n, 1.0, Hr + o1,
-                      Hi + o1, n)))
+                      Hi + o1, n, 1)))
         return rc;
     }
   }

@@ -255,6 +255,11 @@ int split_gemm_nt(hipStream_t s, int m, int n, int k, const double *A, int lda, 
 int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
           const double *B, int ldb, double beta, double *C, int ldc, int tri = 0, int atri = 0, int btri = 0,
           double *ws = nullptr, size_t wsn = 0);
+// planar complex C = alpha op(A) op(B) + beta C in one kernel (alpha, beta
+// real; op 'N', 'T', 'C' conjugate transpose, 'R' conjugate); tri as dgemm
+int zgemm_fused(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *Ar, const double *Ai,
+                int lda, const double *Br, const double *Bi, int ldb, double beta, double *Cr, double *Ci, int ldc,
+                int tri);
 // the same product on a given tile kernel (0: 64x64, 1: 128x128; no split K)
 int dgemm_tiles(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
                 const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri, int variant);
