@@ -934,6 +934,25 @@ int w2k_adjoint_vlm(w2k_ctx *x, const double *f, double *g_vlm)
   return adjoint_vlm(c, f, c.at_lmmxd, g_vlm);
 }
 
+int w2k_write_vector(w2k_ctx *x, const char *path, int append, const char *kname, double sx, double sy, double sz,
+                     double weight, const char *ipgr)
+{
+  if (!x || !path) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  return write_vector(c, path, append, kname, sx, sy, sz, weight, ipgr);
+}
+
+int w2k_density_interstitial(w2k_ctx *x, const double *f, double *rho)
+{
+  if (!x || !rho || (!f && x->c.ne > 0)) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (!c.have_basis) return W2K_ESTATE;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  return density_interstitial(c, f, rho);
+}
+
 int w2k_get_hs(w2k_ctx *x, double *hs, int ld, double *hsdiag)
 {
   if (!x) return W2K_EINVAL;

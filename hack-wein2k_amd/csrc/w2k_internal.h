@@ -220,6 +220,11 @@ int adjoint_warp_box(Ctx &c, const double *f_host, double *e_band, const double 
 int set_stars(Ctx &c, const w2k_stars *st);
 // band-energy gradient with respect to the star coefficients POTK (after SECLR4)
 int adjoint_stars(Ctx &c, const double *f_host, double *e_band, double *g_potk);
+// interstitial band density rho(Q) on the run's box (host, interleaved)
+int density_interstitial(Ctx &c, const double *f_host, double *rho_host);
+// the vector-file record of the k-point (TAPEWF's layout) appended to / written at path
+int write_vector(Ctx &c, const char *path, int append, const char *kname, double sx, double sy, double sz,
+                 double weight, const char *ipgr);
 // ATPAR's radial part on the device; outputs in the reference layouts
 // (host, any may be NULL): out[0..5] P, DP, PE, DPE, PEI, E (LMAX, NAT);
 // out[6..9] PLO, DPLO, PI12LO, PE12LO (LOMAX+1, NAT); out[10..15] VNS1..3,

@@ -225,6 +225,9 @@ def load_library(path: Optional[str] = None):
     lib.w2k_get_warp.argtypes = [V, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p]
     lib.w2k_adjoint_stars.argtypes = [V, _c_dbl_p, _c_dbl_p, _c_dbl_p]
     lib.w2k_atpar.argtypes = [V, ctypes.POINTER(_RadialDesc)] + [_c_dbl_p] * 16
+    lib.w2k_write_vector.argtypes = [V, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p]
+    lib.w2k_density_interstitial.argtypes = [V, _c_dbl_p, _c_dbl_p]
     lib.w2k_adjoint_vlm.argtypes = [V, _c_dbl_p, _c_dbl_p]
     lib.w2k_calkpt_batched.argtypes = [V, ctypes.c_int, ctypes.POINTER(_BasisDesc), ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                        ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p,
@@ -436,6 +439,24 @@ class Lapw1:
         g = np.zeros(self._vlm_shape, np.float64, order="F")
         self._chk("w2k_adjoint_vlm", self.lib.w2k_adjoint_vlm(self.ctx, _dp(f), _dp(g)))
         return g
+
+    def write_vector(self, path: str, kname: str, sx: float, sy: float, sz: float, weight: float, ipgr: str,
+                     append: bool = True):
+        """Append (or write) the vector-file record of the last solved k-point
+        in TAPEWF's layout."""
+        self._chk("w2k_write_vector", self.lib.w2k_write_vector(
+            self.ctx, path.encode(), int(bool(append)), kname.encode()[:10], float(sx), float(sy), float(sz),
+            float(weight), ipgr.encode()[:3]))
+
+    def density_interstitial(self, f) -> np.ndarray:
+        """rho(Q) = sum_n f_n sum_{KZZ_J - KZZ_I = Q} conj z_I z_J on the run's box."""
+        f = np.ascontiguousarray(np.asarray(f, np.float64))
+        if f.shape != (self.ne,):
+            raise ValueError(f"need {self.ne} weights")
+        w1, w2, w3 = self.warp_dims()
+        rho = np.zeros((2 * w1 + 1, 2 * w2 + 1, 2 * w3 + 1), np.complex128, order="F")
+        self._chk("w2k_density_interstitial", self.lib.w2k_density_interstitial(self.ctx, _dp(f), _dp(rho)))
+        return rho
 
     def get_hs(self) -> Tuple[np.ndarray, np.ndarray]:
         n = self.basis.n

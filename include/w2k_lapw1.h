@@ -267,6 +267,23 @@ int w2k_atpar(w2k_ctx *ctx, const w2k_radial_desc *rad, double *p, double *dp, d
  * radial functions at r_k (large components, and small components times
  * 1/c^2 when relativistic). */
 int w2k_adjoint_vlm(w2k_ctx *ctx, const double *f, double *g_vlm);
+/* The vector-file record of the last solved k-point exactly as TAPEWF writes
+ * it to unit 10 (tapewf.F:45-56, called from prtkpt.F:163-166): Fortran
+ * unformatted sequential records with 4-byte length markers: (SX, SY, SZ,
+ * KNAME(10 chars), NV+NLO, NE, WEIGHT, IPGR(3 chars)), (KZZ(1:3,I), I=1,NV+NLO)
+ * and per eigenpair (I, EIGVAL(I)) and Z(1:NV+NLO, I) (conjugated in the
+ * complex build).  Written to `path`, or appended when append != 0, so that
+ * lapw2 can read the file the reference would have written.  No counterpart
+ * in the hot path itself (lapw1 writes it from the host after SECLR4). */
+int w2k_write_vector(w2k_ctx *ctx, const char *path, int append, const char *kname, double sx, double sy,
+                     double sz, double weight, const char *ipgr);
+/* The interstitial band density of the last solved k-point in reciprocal
+ * space, as lapw2 forms it (SRC_lapw2/fourir.F through |psi_n(r)|^2 on an FFT
+ * grid), here by pair sums on the device: rho(Q) = sum_n f[n] sum over plane
+ * waves I, J with KZZ_J - KZZ_I = Q of conj z(I,n) z(J,n), for every Q of the
+ * run's warped box (layout of w2k_get_warp, interleaved; the real build
+ * keeps the real part).  No counterpart in lapw1. */
+int w2k_density_interstitial(w2k_ctx *ctx, const double *f, double *rho);
 /* Host <-> device transfers of the reference-layout arrays (ld >= N).  */
 int w2k_get_hs(w2k_ctx *ctx, double *hs, int ld, double *hsdiag);
 int w2k_put_hs(w2k_ctx *ctx, const double *hs, int ld, const double *hsdiag);

@@ -155,6 +155,10 @@ def save_case(name: str, workdir: str, nk_full: int, complex_: bool, seconds: fl
         kd = read_dump(os.path.join(workdir, kf))
         for k, v in kdata(kd, i < nk_full, rng).items():
             payload[f"k{i}/{k}"] = v
+    if name in ("c1_fcc", "lo_cplx"):
+        # the reference's vector file (TAPEWF, unit 10) of every k-point
+        with open(os.path.join(workdir, "case.vector"), "rb") as f:
+            payload["vector/case_vector"] = np.frombuffer(f.read(), np.uint8)
     payload["meta/nk"] = np.asarray(len(ks))
     payload["meta/complex"] = np.asarray(int(complex_))
     payload["meta/ref_seconds"] = np.asarray(seconds)

@@ -170,8 +170,8 @@ def test_c5_warp_gradient_is_the_fft_band_density(cell):
     _, z = ctx.get_eig()
     ctx.close()
     kd = cell.fx.k(q)
-    rho, dims = _fft_density(np.asarray(kd["KZZ"]), z.astype(np.complex128), f, int(kd["NV"]))
     w = [(s - 1) // 2 for s in g.shape]
+    rho, dims = _fft_density(np.asarray(kd["KZZ"]), z.astype(np.complex128), f, int(kd["NV"]), w)
     worst = 0.0
     for qx in range(-w[0], w[0] + 1):
         for qy in range(-w[1], w[1] + 1):

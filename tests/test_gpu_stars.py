@@ -119,13 +119,15 @@ def test_star_gradient_matches_finite_differences(name):
             assert abs(fd - an) <= 1e-6 * abs(an) + 2e-8, (s, part, fd, an)
 
 
-def _fft_density(kzz, z, f, nv):
+def _fft_density(kzz, z, f, nv, box):
     """rho(Q) = sum_n f_n sum over (I, J) with KZZ_J - KZZ_I = Q of
-    conj z_I z_J, through psi_n(r) on an FFT grid (lapw2's fourir.F route)."""
+    conj z_I z_J, through psi_n(r) on an FFT grid (lapw2's fourir.F route).
+    The grid holds every difference of the basis and every Q of the box
+    (half-widths `box`) without wrap-around."""
     k = kzz[:, :nv].T
     lo = k.min(axis=0)
     span = k.max(axis=0) - lo + 1
-    dims = tuple(int(2 * s) for s in span)  # room for every difference without wrap-around
+    dims = tuple(int(max(2 * s, s + w)) for s, w in zip(span, box))
     rho = np.zeros(dims)
     rho = rho.astype(np.complex128)
     idx = tuple((k - lo).T)
@@ -149,8 +151,8 @@ def test_warp_gradient_is_the_fft_band_density(name):
     f = _fermi(ev)
     _, g = ctx.adjoint_warp(f)
     ctx.close()
-    rho, dims = _fft_density(np.asarray(kd["KZZ"]), z.astype(np.complex128), f, b.nv)
     w = [(s - 1) // 2 for s in g.shape]
+    rho, dims = _fft_density(np.asarray(kd["KZZ"]), z.astype(np.complex128), f, b.nv, w)
     worst = 0.0
     scale = abs(rho[0, 0, 0])
     for qx in range(-w[0], w[0] + 1):
