@@ -432,6 +432,36 @@ static int timed_end(Ctx &c, int slot)
   return 0;
 }
 
+int basis_buffers(Ctx &c, int nv, int nlo)
+{
+  const int n = nv + nlo;
+  if (n > c.n_cap) {
+    // a basis larger than any this context has held: drain both streams and
+    // release every per-k buffer and cached key, so that the k-point runs
+    // exactly as on a fresh context (smaller bases reuse the buffers, as the
+    // batched path's streams do)
+    W2K_HIP(hipStreamSynchronize(c.stream));
+    W2K_HIP(hipStreamSynchronize(c.aux));
+    drop_kpoint_state(c);
+    c.n_cap = n;
+  }
+  c.nv = nv;
+  c.nlo = nlo;
+  c.n = n;
+  c.have_basis = 0;
+  c.have_hamilt = 0;
+  c.ne = 0;
+  c.have_hns_ops = 0;
+  int rc;
+  if ((rc = c.d_kzz.alloc((size_t)3 * n)) || (rc = c.d_kvec.alloc((size_t)4 * n))) return rc;
+  const int w = c.run().is_complex ? 2 : 1;
+  if ((rc = c.d_hs.alloc((size_t)w * n * n))) return rc;
+  if ((rc = c.d_hsdiag.alloc((size_t)w * n))) return rc;
+  W2K_HIP(hipMemsetAsync(c.d_hs.p, 0, sizeof(double) * w * (size_t)n * n, c.stream));
+  W2K_HIP(hipMemsetAsync(c.d_hsdiag.p, 0, sizeof(double) * w * (size_t)n, c.stream));
+  return 0;
+}
+
 }  // namespace w2k
 
 using namespace w2k;
@@ -595,21 +625,8 @@ int w2k_set_basis(w2k_ctx *x, int nv, int nlo, const int *kzz, const double *xk,
   if (nlo != 0 && nlo != c.run().nlo_cols) return W2K_EINVAL;
   if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
   const int n = nv + nlo;
-  if (n > c.n_cap) {
-    // a basis larger than any this context has held: drain both streams and
-    // release every per-k buffer and cached key, so that the k-point runs
-    // exactly as on a fresh context (smaller bases reuse the buffers, as the
-    // batched path's streams do)
-    W2K_HIP(hipStreamSynchronize(c.stream));
-    W2K_HIP(hipStreamSynchronize(c.aux));
-    drop_kpoint_state(c);
-    c.n_cap = n;
-  }
-  c.nv = nv;
-  c.nlo = nlo;
-  c.n = n;
-  c.have_hamilt = 0;
-  c.ne = 0;
+  int rc;
+  if ((rc = basis_buffers(c, nv, nlo))) return rc;
   std::vector<double> kv((size_t)4 * n);
   for (int i = 0; i < n; i++) {
     kv[4 * i] = xk[i];
@@ -617,19 +634,28 @@ int w2k_set_basis(w2k_ctx *x, int nv, int nlo, const int *kzz, const double *xk,
     kv[4 * i + 2] = zk[i];
     kv[4 * i + 3] = rk[i];
   }
-  int rc;
   c.h_kzz.assign(kzz, kzz + (size_t)3 * n);
-  c.have_hns_ops = 0;
-  if ((rc = upload(c.d_kzz, kzz, (size_t)3 * n, c.stream))) return rc;
-  if ((rc = upload(c.d_kvec, kv.data(), kv.size(), c.stream))) return rc;
-  const int w = c.run().is_complex ? 2 : 1;
-  if ((rc = c.d_hs.alloc((size_t)w * n * n))) return rc;
-  if ((rc = c.d_hsdiag.alloc((size_t)w * n))) return rc;
-  W2K_HIP(hipMemsetAsync(c.d_hs.p, 0, sizeof(double) * w * (size_t)n * n, c.stream));
-  W2K_HIP(hipMemsetAsync(c.d_hsdiag.p, 0, sizeof(double) * w * (size_t)n, c.stream));
+  W2K_HIP(hipMemcpyAsync(c.d_kzz.p, kzz, sizeof(int) * 3 * (size_t)n, hipMemcpyHostToDevice, c.stream));
+  W2K_HIP(hipMemcpyAsync(c.d_kvec.p, kv.data(), sizeof(double) * kv.size(), hipMemcpyHostToDevice, c.stream));
   W2K_HIP(hipStreamSynchronize(c.stream));
   c.have_basis = 1;
   return 0;
+}
+
+int w2k_coors(w2k_ctx *x, const w2k_lattice *lat, double sx, double sy, double sz, int *nv, int *nlo)
+{
+  if (!x || !lat) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  return coors(c, lat, sx, sy, sz, nv, nlo);
+}
+
+int w2k_get_basis(w2k_ctx *x, int *kzz, double *xk, double *yk, double *zk, double *rk)
+{
+  if (!x) return W2K_EINVAL;
+  Ctx &c = x->c;
+  if (hipSetDevice(c.device) != hipSuccess) return W2K_ENODEV;
+  return get_basis(c, kzz, xk, yk, zk, rk);
 }
 
 int w2k_hamilt(w2k_ctx *x)

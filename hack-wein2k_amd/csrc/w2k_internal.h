@@ -241,6 +241,13 @@ int adjoint_gamma(Ctx &c, const double *d_f, int rows, double *Gr, double *Gi);
 int adjoint_vns(Ctx &c, const double *f_host, double *const g[6]);
 // release every per-k-point buffer and cached key (streams idle)
 void drop_kpoint_state(Ctx &c);
+// per-k buffers of a basis of NV + NLO rows (KZZ, k+G, zeroed HS/HSDIAG), as
+// w2k_set_basis leaves them before the rows are filled in
+int basis_buffers(Ctx &c, int nv, int nlo);
+// COORS + LOPW on the device into the context's basis (w2k_coors)
+int coors(Ctx &c, const w2k_lattice *L, double sx, double sy, double sz, int *nv, int *nlo);
+// the context's basis on the host (any pointer may be NULL)
+int get_basis(Ctx &c, int *kzz, double *xk, double *yk, double *zk, double *rk);
 // elapsed milliseconds between events e0 and e1 into last_ms[slot]
 void take_ms(Ctx &c, int slot, int e0, int e1);
 

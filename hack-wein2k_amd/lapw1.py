@@ -175,6 +175,24 @@ class _RadialDesc(ctypes.Structure):
                 ("zz", _c_dbl_p), ("vr", _c_dbl_p), ("lmmxd", ctypes.c_int), ("vlm", _c_dbl_p)]
 
 
+class Lattice(ctypes.Structure):
+    """w2k_lattice of include/w2k_lapw1.h: what COORS reads besides k
+    (coors.f:1-18)."""
+    _fields_ = [("br2", ctypes.c_double * 9), ("pia", ctypes.c_double * 3), ("ortho", ctypes.c_int),
+                ("cxz", ctypes.c_int), ("rkm", ctypes.c_double), ("nmatmax", ctypes.c_int)]
+
+    @classmethod
+    def make(cls, br2, pia, ortho: bool, cxz: bool, rkm: float, nmatmax: int = 13000) -> "Lattice":
+        """BR2(3,3) as an array (Fortran order is taken from its [i, j] indices)."""
+        b = np.asarray(br2, np.float64).reshape(3, 3)
+        lat = cls()
+        lat.br2[:] = [float(b[i, j]) for j in range(3) for i in range(3)]
+        lat.pia[:] = [float(x) for x in np.asarray(pia, np.float64).ravel()[:3]]
+        lat.ortho, lat.cxz = int(bool(ortho)), int(bool(cxz))
+        lat.rkm, lat.nmatmax = float(rkm), int(nmatmax)
+        return lat
+
+
 class Lapw1Error(RuntimeError):
     def __init__(self, where: str, code: int, detail: str = ""):
         msg = f"{where} failed with status {code}"
@@ -200,6 +218,9 @@ def load_library(path: Optional[str] = None):
     lib.w2k_destroy.argtypes = [V]
     lib.w2k_destroy.restype = None
     lib.w2k_set_basis.argtypes = [V, ctypes.c_int, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
+    lib.w2k_coors.argtypes = [V, ctypes.POINTER(Lattice), ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                              _c_int_p, _c_int_p]
+    lib.w2k_get_basis.argtypes = [V, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
     for fn in ("w2k_hamilt", "w2k_swap_diag", "w2k_hns", "w2k_sync"):
         getattr(lib, fn).argtypes = [V]
     lib.w2k_seclr4.argtypes = [V, ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_int_p, _c_int_p]
@@ -284,6 +305,26 @@ class Lapw1:
         self.basis = b
         self._chk("w2k_set_basis", self.lib.w2k_set_basis(
             self.ctx, b.nv, b.nlo, b.kzz.ctypes.data_as(_c_int_p), _dp(b.xk), _dp(b.yk), _dp(b.zk), _dp(b.rk)))
+
+    def coors(self, lat: Lattice, sx: float, sy: float, sz: float) -> Basis:
+        """COORS + LOPW (coors.f, lopw.f) on the device: the k-point's basis
+        becomes the context's (as set_basis would) and is returned."""
+        nv, nlo = ctypes.c_int(0), ctypes.c_int(0)
+        self._chk("COORS", self.lib.w2k_coors(self.ctx, ctypes.byref(lat), float(sx), float(sy), float(sz),
+                                              ctypes.byref(nv), ctypes.byref(nlo)))
+        self.basis = self.get_basis(nv.value, nlo.value)
+        return self.basis
+
+    def get_basis(self, nv: Optional[int] = None, nlo: Optional[int] = None) -> Basis:
+        """The context's basis (w2k_get_basis)."""
+        if nv is None:
+            nv, nlo = self.basis.nv, self.basis.nlo
+        n = nv + nlo
+        kzz = np.zeros((3, n), np.int32, order="F")
+        xk, yk, zk, rk = (np.zeros(n) for _ in range(4))
+        self._chk("w2k_get_basis", self.lib.w2k_get_basis(self.ctx, kzz.ctypes.data_as(_c_int_p), _dp(xk), _dp(yk),
+                                                          _dp(zk), _dp(rk)))
+        return Basis(nv, nlo, kzz, xk, yk, zk, rk)
 
     def hamilt(self):
         """HAMILT (hamilt.F): H/S spherical + interstitial + LO into HS/HSDIAG."""

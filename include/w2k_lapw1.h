@@ -107,6 +107,34 @@ int w2k_set_basis(w2k_ctx *ctx, int nv, int nlo, const int *kzz,
                   const double *xk, const double *yk, const double *zk,
                   const double *rk);
 
+/* Reciprocal lattice and cut-off that COORS reads from its argument list and
+ * the modules gener/struk/orth/char (coors.f:1-18). */
+typedef struct w2k_lattice {
+  double br2[9];  /* BR2(3,3), Fortran order: column j = reciprocal vector j  */
+  double pia[3];  /* PIA(3) = 2 pi / (a, b, c)                                */
+  int ortho;      /* ORTHO (orthogonal axes: KZZ from PIA, coors.f:123-129)    */
+  int cxz;        /* LATTIC(1:3) == 'CXZ' (coors.f:135-139)                    */
+  double rkm;     /* RKM = RKmax / min RMT: the radius of the k+G sphere      */
+  int nmatmax;    /* NMATMAX (param.inc_r:16): largest order NV + NLO          */
+} w2k_lattice;
+
+/* COORS + LOPW on the device (coors.f:1-236 called at calkpt.F:143, with
+ * LOPW at coors.f:208): the plane waves k+G, k = (sx, sy, sz), with
+ * |k+G| <= RKM in the reference's order (by length; equal lengths in the
+ * order of the box walk JA, JB, JC), cut back at NMATMAX - NLO to the last
+ * gap wider than 1e-5 in length, followed by the NLO local-orbital vectors
+ * LOPW picks by Gram-Schmidt over the plane-wave list.  Becomes the context's
+ * basis as w2k_set_basis would with the same KZZ/XK/YK/ZK/RK (bit for bit the
+ * reference's; RK of the local-orbital rows is what the reference leaves in
+ * those slots).  *nv and *nlo receive NV and NLO (either may be NULL).
+ * LOPW running out of plane waves (lopw.f:139) returns W2K_EINVAL. */
+int w2k_coors(w2k_ctx *ctx, const w2k_lattice *lat, double sx, double sy, double sz,
+              int *nv, int *nlo);
+/* The context's basis on the host: KZZ(3, NV+NLO), XK, YK, ZK, RK(NV+NLO)
+ * (any pointer may be NULL). */
+int w2k_get_basis(w2k_ctx *ctx, int *kzz, double *xk, double *yk, double *zk,
+                  double *rk);
+
 /* HAMILT (hamilt.F:1-845): spherical MT + warped interstitial + LO block into
  * the device HS/HSDIAG, in the reference layout.  Also keeps the AL/BL
  * tables (module albl) for HNS.  */

@@ -73,7 +73,7 @@ program ref_dump
   use lapw_timer, only : init_all_timer, start_timer, stop_timer, time_total, &
                          time_hamilt, time_hns, time_diag, read_cpu_time, read_wall_time
   use cut1, only : cut
-  use matrices, only : HS, HSDIAG, HSROWS, Z, EIGVAL, KZZ, XK, YK, ZK, RK
+  use matrices, only : HS, HSDIAG, HSROWS, Z, EIGVAL, KZZ, XK, YK, ZK, RK, END_MATRICES, END_matrice1
   use lolog, only : nlo, loor, lapw, ilo
   use atspdt, only : INS, LQIND, LQNS, DP, DPE, E, P, PE, PEI, VNS1, VNS2, VNS3, GFAC
   use comi, only : NVAA, NE, NBELW, NAT, NT, LNSMAX
@@ -91,9 +91,13 @@ program ref_dump
   use potnlc, only : JRJ, DH, RO, VR
   use atspdt, only : LMMAX, LM
   use coml, only : REL
+  use gener, only : BR2
+  use orth, only : ORTHO
+  use char, only : LATTIC
+  use struk, only : PIA
   implicit none
   include 'param.inc'
-  logical :: nmat_only, dodump, eigdump, basisdump
+  logical :: nmat_only, dodump, eigdump, basisdump, coorsonly
   integer :: numwk, info, kpoint, u, nn, i, j
   integer, allocatable :: ltmp(:,:)
   character(len=80) :: deffn, fname
@@ -106,6 +110,7 @@ program ref_dump
   dodump = .true.
   eigdump = .false.
   basisdump = .false.
+  coorsonly = .false.
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg)
     if (arg(1:6) == 'nodump') dodump = .false.
@@ -116,6 +121,12 @@ program ref_dump
     if (arg(1:5) == 'basis') then
       dodump = .false.
       basisdump = .true.
+    end if
+    ! 'coors': run state and per-k basis only (COORS/LOPW), no HAMILT/HNS/SECLR4
+    if (arg(1:5) == 'coors') then
+      dodump = .false.
+      basisdump = .true.
+      coorsonly = .true.
     end if
   end if
   call init_parallel
@@ -193,6 +204,12 @@ program ref_dump
   call wi(u, 'LMMAX', LMMAX, [NAT])
   call wi(u, 'LM', LM, [2, LMMX, NAT])
   call ws(u, 'REL', merge(1, 0, REL))
+  ! reciprocal lattice and cutoff of COORS (coors.f:61-148)
+  call wd(u, 'BR2', BR2, [3, 3])
+  call wd(u, 'PIA', PIA, [3])
+  call ws(u, 'ORTHO', merge(1, 0, ORTHO))
+  call ws(u, 'CXZ', merge(1, 0, LATTIC(1:3) == 'CXZ'))
+  call wsd(u, 'RKM', RKM)
   if (INS == 0) then
     ! interstitial stars (rdswar.f) and the space group (module out)
     call ws(u, 'IORD', IORD)
@@ -236,6 +253,15 @@ program ref_dump
     call wsd(u, 'EL', EL)
     call wsd(u, 'EU', EU)
     call ws(u, 'NUME', NUME)
+    end if
+    if (coorsonly) then
+      call ws(u, 'NE', 0)
+      call ws(u, 'NBELW', 0)
+      call wd(u, 'EIGVAL', EIGVAL, [1])
+      close(u)
+      call END_matrice1
+      call END_MATRICES
+      cycle
     end if
     call system_clock(c0, crate)
     call HAMILT(NT, NAT, NV, P, PE, DP, DPE, MULT, RMT, POS, V, VI, PEI, E, INS)

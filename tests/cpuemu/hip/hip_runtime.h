@@ -240,6 +240,13 @@ inline int atomicAdd(int *p, int v)
   *p = o + v;
   return o;
 }
+inline int atomicExch(int *p, int v)
+{
+  const int o = *p;
+  *p = v;
+  return o;
+}
+inline double __dsqrt_rn(double x) { return std::sqrt(x); }
 
 // HIP's global min/max overloads
 inline int min(int a, int b) { return a < b ? a : b; }

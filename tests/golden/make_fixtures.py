@@ -92,6 +92,9 @@ def crop_run(run: dict) -> dict:
         out["LMMAX"] = run["LMMAX"]
         out["LM"] = run["LM"][:, :nl, :]
         out["REL"] = run["REL"]
+    for k in ("BR2", "PIA", "ORTHO", "CXZ", "RKM"):
+        if k in run:
+            out[k] = run[k]
     if int(run["INS"]) == 0 and "NKK" in run:
         for k in ("IORD", "IMAT", "TAU", "NKK", "JA", "JB", "JC", "POTK"):
             out[k] = run[k]
@@ -246,6 +249,29 @@ def make_c3(name: str, workdir: str, seconds: float, frac: float = 0.2):
           f"reference {seconds:.1f} s")
 
 
+def add_lattice(name: str, tmp: str):
+    """Add COORS's lattice (BR2, PIA, ORTHO, CXZ, RKM) to an existing fixture
+    from a COORS-only pass of the reference over the same deck ('coors'
+    dump: run state and bases, no HAMILT/HNS/SECLR4), after checking that the
+    pass reproduces the stored bases."""
+    case = W.CASES[name]()
+    d = os.path.join(tmp, name + "_coors")
+    W.write_case(case, d)
+    run_reference(d, case.complex, mode="coors")
+    run = read_dump(os.path.join(d, "run.bin"))
+    path = os.path.join(HERE, f"{name}.npz")
+    z = np.load(path, allow_pickle=False)
+    payload = {k: z[k] for k in z.files}
+    for i in range(int(payload["meta/nk"])):
+        kd = read_dump(os.path.join(d, f"k{i + 1:04d}.bin"))
+        n = int(kd["HSROWS"])
+        assert np.array_equal(kd["KZZ"][:, :n], payload[f"k{i}/KZZ"]), f"{name} k{i}: bases differ"
+    for k in ("BR2", "PIA", "ORTHO", "CXZ", "RKM"):
+        payload[f"run/{k}"] = np.asarray(run[k])
+    np.savez_compressed(path, **payload)
+    print(f"{name}: lattice added ({os.path.getsize(path)/1024:.0f} KiB)")
+
+
 def main(argv):
     names = argv or ["mgo", "c1_fcc", "lo_mult2", "lo_cplx", "rs8"]
     # W2K_FIXTURE_WORKDIR: run the reference there and keep its outputs (the
@@ -255,6 +281,9 @@ def main(argv):
     os.makedirs(tmp, exist_ok=True)
     try:
         for name in names:
+            if name.startswith("lattice:"):
+                add_lattice(name.split(":", 1)[1], tmp)
+                continue
             if name == "mgo":
                 make_mgo(tmp)
                 continue

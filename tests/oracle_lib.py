@@ -54,6 +54,10 @@ def lib():
         l.oracle_seclr4.argtypes = [ctypes.c_int, ctypes.c_int, _c_dbl_p, ctypes.c_int, _c_dbl_p, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_int, _c_int_p, _c_int_p, _c_dbl_p, _c_dbl_p,
                                     ctypes.c_int]
+        LT = ctypes.POINTER(L.Lattice)
+        l.oracle_coors.argtypes = [LT, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_int_p,
+                                   _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
+        l.oracle_lopw.argtypes = [RD, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
         _lib = l
     return _lib
 
@@ -80,6 +84,25 @@ def ylm(v, lmax: int) -> np.ndarray:
     out = np.zeros((lmax + 1) ** 2, np.complex128)
     lib().oracle_ylm(_dp(v), lmax, _dp(out))
     return out
+
+
+def coors(run, lat, sx: float, sy: float, sz: float, nlo: int):
+    """COORS + LOPW (oracle_coors.c): the Basis of the k-point (RK of the
+    local-orbital rows as the reference's list leaves them)."""
+    L = load_package().lapw1
+    cap = lat.nmatmax + 1
+    kzz = np.zeros((3, cap), np.int32, order="F")
+    xk, yk, zk, rk = (np.zeros(cap) for _ in range(4))
+    ip = kzz.ctypes.data_as(_c_int_p)
+    nv = lib().oracle_coors(ctypes.byref(lat), float(sx), float(sy), float(sz), int(nlo), ip, _dp(xk), _dp(yk),
+                            _dp(zk), _dp(rk))
+    if nv < 1:
+        raise RuntimeError("oracle COORS: empty list")
+    if nlo:
+        rc = lib().oracle_lopw(ctypes.byref(run.desc()), nv, ip, _dp(xk), _dp(yk), _dp(zk))
+        if rc != 0:
+            raise RuntimeError("oracle LOPW: plane waves exhausted")
+    return L.Basis(nv, nlo, kzz, xk, yk, zk, rk)
 
 
 def hamilt(run, basis) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:

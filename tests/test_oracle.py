@@ -182,3 +182,53 @@ def test_seclr4_not_positive_definite_reports_info():
     hd[5] = -1.0  # S no longer positive definite: POTRF INFO > 0 (seclr4.F:200-205)
     rc, *_ = O.seclr4(hs, hd, -1e9, 1e9, 100, False)
     assert rc > 0
+
+
+# ---------------------------------------------------------------- COORS / LOPW
+BASIS_CASES = ["c1_fcc", "mgo", "lo_mult2", "lo_cplx", "rs8", "ins_lo", "ins_lo_cplx", "c2_rs64", "c3_lowsym100",
+               "c4_ortho8", "c5_cubic16"]
+
+
+@pytest.mark.parametrize("name", BASIS_CASES)
+def test_oracle_coors_lopw_match_reference(name):
+    """KZZ, XK, YK, ZK and RK of every k-point, local-orbital rows included,
+    bit for bit (orthogonal and general axes, one to eight equivalent atoms
+    per local orbital, up to N = 12447)."""
+    fx = Fixture(name)
+    r = fx.run_arrays
+    if "BR2" not in r:
+        pytest.skip("fixture without the lattice")
+    L = O.load_package().lapw1
+    lat = L.Lattice.make(r["BR2"], r["PIA"], int(r["ORTHO"]), int(r["CXZ"]), float(r["RKM"]))
+    run = fx.run()
+    for i in range(fx.nk):
+        kd = fx.k(i)
+        b = O.coors(run, lat, float(kd["SX"]), float(kd["SY"]), float(kd["SZ"]), int(kd["NLO"]))
+        ref = fx.basis(i)
+        assert (b.nv, b.nlo) == (ref.nv, ref.nlo), f"k{i}"
+        assert np.array_equal(b.kzz, ref.kzz), f"KZZ k{i}"
+        for f in ("xk", "yk", "zk", "rk"):
+            assert np.array_equal(getattr(b, f), getattr(ref, f)), f"{f} k{i}"
+
+
+def test_oracle_coors_list_order_by_brute_force():
+    """The oracle's list against an independent statement of it: every k+G
+    of a wide box with |k+G| <= RKM, stably sorted by length in (JA, JB, JC)
+    order (numpy lexsort), for a skewed lattice and a general k."""
+    L = O.load_package().lapw1
+    br2 = np.array([[0.9, 0.1, 0.0], [0.3, 1.1, 0.2], [0.0, -0.4, 0.8]])
+    lat = L.Lattice.make(br2, [1.0, 1.0, 1.0], False, False, 4.0)
+    s = np.array([0.11, -0.23, 0.37])
+    fx = Fixture("rs8")
+    b = O.coors(fx.run(), lat, *s, 0)
+    j = np.arange(-15, 16)
+    ja, jb, jc = (x.ravel() for x in np.meshgrid(j, j, j, indexing="ij"))
+    sk = br2 @ s
+    g = sk[:, None] + br2 @ np.vstack([ja, jb, jc])
+    rk = np.sqrt(g[0] ** 2 + g[1] ** 2 + g[2] ** 2)
+    keep = rk <= 4.0
+    order = np.lexsort((np.arange(keep.sum()), rk[keep]))
+    ref = np.vstack([ja, jb, jc])[:, keep][:, order]
+    assert b.nv == keep.sum()
+    assert np.array_equal(b.kzz[:, :b.nv], ref)
+    assert np.allclose(b.rk[:b.nv], rk[keep][order], rtol=0, atol=1e-13)
