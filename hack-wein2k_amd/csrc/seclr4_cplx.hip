@@ -221,7 +221,7 @@ struct ZTriWork {
   int *counter;
 };
 
-constexpr int ZCOL_ROWS = 32;  // rows per block of the column kernel (8 threads per row)
+constexpr int ZCOL_ROWS = 64;  // rows per block of the column kernel (one per lane)
 
 // ZLARFG of column i on (A(i+1,i), A(i+2:n,i)) from the squared-norm
 // partials of the column kernel (one per block of its rows), evaluated by
@@ -316,7 +316,6 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
       vi = ar * si + ai * sr;
     }
   };
-  double t[SYMV_T];
   if (bid >= nblk) {
     const int I = bid - nblk;
     const int R = r0 + SYMV_T * I + lane;
@@ -330,24 +329,19 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
     for (int q = I * SYMV_T + lane; q < r0; q += nt * SYMV_T) { Vr[q] = 0.0; Vi[q] = 0.0; V2r[q] = 0.0; V2i[q] = 0.0; }
     if (ii > 0) {
       double *dr = g.dotr + (size_t)I * 2 * TNB, *di = g.doti + (size_t)I * 2 * TNB;
-#pragma unroll
-      for (int half = 0; half < 2; half++) {  // 0: V, 1: W
+#pragma unroll 1
+      for (int pass = 0; pass < 4; pass++) {  // V then W, each Re then Im of conj(X(R,k)) v(R)
+        const int half = pass >> 1, im = pass & 1;
         const double *Xr = g.Pr + (size_t)n * (half * nb), *Xi = g.Pi + (size_t)n * (half * nb);
+        double t[TNB];  // 32 values per pass keep the branch's registers at the tile loop's level
 #pragma unroll
         for (int k = 0; k < TNB; k++) {
-          t[k] = Xr[(size_t)n * k + Rs];
-          t[TNB + k] = Xi[(size_t)n * k + Rs];
-        }
-#pragma unroll
-        for (int k = 0; k < TNB; k++) {  // conj(X(R,k)) v(R)
+          const double ur = Xr[(size_t)n * k + Rs], ui = Xi[(size_t)n * k + Rs];
           const bool use = live && k < ii;
-          const double ur = t[k], ui = t[TNB + k];
-          t[k] = use ? ur * vrr + ui * vri : 0.0;
-          t[TNB + k] = use ? ur * vri - ui * vrr : 0.0;
+          t[k] = use ? (im ? ur * vri - ui * vrr : ur * vrr + ui * vri) : 0.0;
         }
-        const double sq = wave_transpose_sum(t, lane);  // lane l < 32: Re of column l, 32 + l: Im
-        if (lane < TNB) dr[half * TNB + lane] = sq;
-        else di[half * TNB + lane - TNB] = sq;
+        const double sq = wave_transpose_sum32(t, lane);  // lane l < 32: column l
+        if (lane < TNB) (im ? di : dr)[half * TNB + lane] = sq;
       }
     }
     return;
@@ -367,6 +361,9 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
   vof(R, vrr, vri);          // v of the lane's row
   double yr = 0.0, yi = 0.0, vyc = 0.0;
   const double *apr = g.ar + (size_t)Rl + (size_t)n * C0, *api = g.ai + (size_t)Rl + (size_t)n * C0;
+  // a full off-diagonal tile (all 64 rows and columns inside the matrix)
+  // needs no masks; wave-uniform
+  const bool full = off && m == SYMV_T && r0 + SYMV_T * (I + 1) <= n;
   // only the lower triangle of the trailing matrix is kept current: a
   // diagonal tile uses its lower part by rows and its strict lower part by
   // columns (A^H v), the column sums folded into the rows they belong to
@@ -379,11 +376,13 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
       q[j] = apr[(size_t)n * jj];
       q[16 + j] = api[(size_t)n * jj];
     }
+    if (!full) {
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const bool ok = live && c0 + j < m && (off || c0 + j <= lane);
-      q[j] = ok ? q[j] : 0.0;
-      q[16 + j] = ok ? q[16 + j] : 0.0;
+      for (int j = 0; j < 16; j++) {
+        const bool ok = live && c0 + j < m && (off || c0 + j <= lane);
+        q[j] = ok ? q[j] : 0.0;
+        q[16 + j] = ok ? q[16 + j] : 0.0;
+      }
     }
 #pragma unroll
     for (int j = 0; j < 16; j++) cmad(q[j], q[16 + j], dbl_readlane(vcr, c0 + j), dbl_readlane(vci, c0 + j), yr, yi);
@@ -423,212 +422,248 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
   if (lane == 0) g.vy[bid] = vy;
 }
 
-// sum over the 8 lanes 8q..8q+7, the same value in each
-__device__ __forceinline__ double oct_sum(double v)
-{
-  v += bfly_partner<1>(v);
-  v += bfly_partner<2>(v);
-  v += bfly_partner<4>(v);
-  return v;
-}
-
 // Finish W(:, fin - p0) (fin >= 0) and update column col (col >= 0) with the
 // panel so far (ZLATRD); write the squared-norm partial of its
 // sub-subdiagonal part per block (ZLARFG itself is evaluated by the next
-// SYMV's waves).  Eight threads per row, ZCOL_ROWS rows per block; each
-// thread takes an eighth of its row's SYMV partials and of the panel
-// columns.  Every load that does not depend on a value formed here is issued
-// before any of them is used: the first batch of every sum is unconditional
-// (clamped indices, masked values), the tile partials are read in pairs.
-__global__ __launch_bounds__(256) void k_ztri_col(ZTriWork g, int fin, int col, int ntile, int nch, int nvy)
+// SYMV's waves).  A block takes ZCOL_ROWS = 64 consecutive rows, one per
+// lane, in each of its ZCOL_WAVES waves, so that every load of a SYMV partial
+// or a panel column is one contiguous 512-byte row segment; the waves split
+// the SYMV partials (chunk w, w + 8, ...) and the panel columns (four each)
+// and their row sums meet in LDS.  Loads are issued in batches with clamped
+// indices and masked values, so that a batch is in flight at once.
+constexpr int ZCOL_WAVES = 8;
+__global__ __launch_bounds__(64 * ZCOL_WAVES) void k_ztri_col(ZTriWork g, int fin, int col, int ntile, int nch,
+                                                              int nvy)
 {
-  constexpr int BD = 4;        // first batch of V^H v, W^H v pairs per array (ntile <= 8 BD)
-  constexpr int BV = 6;        // first batch of v^H y pairs (nvy <= 512 BV)
-  constexpr int BP = 8;        // first batch of SYMV partials of a row (nch <= 8 BP)
-  constexpr int KO = TNB / 8;  // panel columns per thread
-  const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  const int q8 = t & 7;
+  constexpr int NT = 64 * ZCOL_WAVES;   // threads
+  constexpr int KW = TNB / ZCOL_WAVES;  // panel columns per wave
+  constexpr int YB = 16;                // SYMV-partial chunks per wave and batch
+  constexpr int VB = 10;                // first batch of v^H y pairs per thread (nvy <= 2 NT VB)
+  constexpr int DG = NT / 32;           // groups of the V^H v, W^H v sums (32 column pairs each)
+  constexpr int DB = 8;                 // first batch of tiles per group (ntile <= DG DB)
+  const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int rbase = fin >= 0 ? fin + 1 : col;
-  const int r = rbase + blockIdx.x * ZCOL_ROWS + (t >> 3);
+  const int r = rbase + blockIdx.x * ZCOL_ROWS + lane;
   const bool live = r < n;
   const int rs = live ? r : rbase;
-  __shared__ double abr[2 * TNB], abi[2 * TNB];  // a_k = V_k^H v, b_k = W_k^H v
-  __shared__ double wcr[TNB], wci[TNB], vcr[TNB], vci[TNB];
-  __shared__ double reddr[8][2 * TNB], reddi[8][2 * TNB];
-  __shared__ double sh[3][4];
   const int ii = fin >= 0 ? fin - p0 : -1;
   const int jj = col - p0;
+  __shared__ double abr[2 * TNB], abi[2 * TNB];  // a_k = V_k^H v, b_k = W_k^H v
+  __shared__ double wcr[TNB], wci[TNB], vcr[TNB], vci[TNB];
+  __shared__ double dred[2][DG][2 * TNB];
+  __shared__ double red[6][ZCOL_WAVES][ZCOL_ROWS];  // per wave: y, z (W a + V b), a0 (re, im)
+  __shared__ double sh[3][ZCOL_WAVES];
   const double *Vr = g.Pr, *Vi = g.Pi, *Wr = g.Pr + (size_t)n * nb, *Wi = g.Pi + (size_t)n * nb;
   double *acr = g.ar + (size_t)n * (col >= 0 ? col : 0), *aci = g.ai + (size_t)n * (col >= 0 ? col : 0);
-  // ---- loads
-  const double ar0 = (col >= 0) ? acr[rs] : 0.0, ai0 = (col >= 0) ? aci[rs] : 0.0;
-  double vc_r = 0.0, vc_i = 0.0, wc_r = 0.0, wc_i = 0.0, pc_r = 0.0, pc_i = 0.0;
+  // ---- per-block scalars: column col of the panel, v^H y, V^H v and W^H v
   if (col >= 0 && t < jj) {
-    vc_r = Vr[(size_t)n * t + col];
-    vc_i = Vi[(size_t)n * t + col];
+    vcr[t] = Vr[(size_t)n * t + col];
+    vci[t] = Vi[(size_t)n * t + col];
     if (t < ii || fin < 0) {
-      wc_r = Wr[(size_t)n * t + col];
-      wc_i = Wi[(size_t)n * t + col];
+      wcr[t] = Wr[(size_t)n * t + col];
+      wci[t] = Wi[(size_t)n * t + col];
     }
   }
-  if (fin >= 0 && col >= 0 && t < nch) {
-    pc_r = g.partr[(size_t)t * n + col];
-    pc_i = g.parti[(size_t)t * n + col];
-  }
-  double vkr[KO], vki[KO], wkr[KO], wki[KO];  // V(r,k), W(r,k) for k = KO q8 + u
-#pragma unroll
-  for (int u = 0; u < KO; u++) {
-    const size_t o = (size_t)n * (KO * q8 + u) + rs;
-    vkr[u] = Vr[o]; vki[u] = Vi[o]; wkr[u] = Wr[o]; wki[u] = Wi[o];
-  }
-  double tr = 0.0, ti = 0.0, alpha2 = 0.0, wrr = 0.0, wri = 0.0, viir = 0.0, viii = 0.0;
-  double yr = 0.0, yi = 0.0, vs = 0.0;
-  const int dq = t & 31, dp = t >> 5;  // dots: columns 2dq, 2dq+1 of tiles dp + 8u
   if (fin >= 0) {
-    tr = g.taur[fin];
-    ti = g.taui[fin];
-    viir = Vr[(size_t)n * ii + rs];
-    viii = Vi[(size_t)n * ii + rs];
-    double2 xdr[BD], xdi[BD], xv[BV];
-    double xpr[BP], xpi[BP];
-    const int vlast = (nvy - 1) & ~1;
-#pragma unroll
-    for (int u = 0; u < BD; u++) {
-      const size_t o = (size_t)min(dp + 8 * u, ntile - 1) * 2 * TNB + 2 * dq;
-      xdr[u] = *reinterpret_cast<const double2 *>(g.dotr + o);
-      xdi[u] = *reinterpret_cast<const double2 *>(g.doti + o);
-    }
-#pragma unroll
-    for (int u = 0; u < BV; u++) xv[u] = *reinterpret_cast<const double2 *>(g.vy + min(2 * t + 512 * u, vlast));
-#pragma unroll
-    for (int u = 0; u < BP; u++) {
-      const size_t o = (size_t)min(q8 + 8 * u, nch - 1) * n + rs;
-      xpr[u] = g.partr[o];
-      xpi[u] = g.parti[o];
-    }
+    // v^H y: every partial once, in pairs
     {
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      const int vlast = (nvy - 1) & ~1;
+      double2 xv[VB];
 #pragma unroll
-      for (int u = 0; u < BD; u++) {
-        const bool k = dp + 8 * u < ntile;
-        s0 += k ? xdr[u].x : 0.0;
-        s1 += k ? xdr[u].y : 0.0;
-        s2 += k ? xdi[u].x : 0.0;
-        s3 += k ? xdi[u].y : 0.0;
-      }
-      for (int b = dp + 8 * BD; b < ntile; b += 8) {
-        const size_t o = (size_t)b * 2 * TNB + 2 * dq;
-        const double2 x = *reinterpret_cast<const double2 *>(g.dotr + o);
-        const double2 z = *reinterpret_cast<const double2 *>(g.doti + o);
-        s0 += x.x; s1 += x.y; s2 += z.x; s3 += z.y;
-      }
-      reddr[dp][2 * dq] = s0; reddr[dp][2 * dq + 1] = s1;
-      reddi[dp][2 * dq] = s2; reddi[dp][2 * dq + 1] = s3;
-    }
-    {
+      for (int u = 0; u < VB; u++) xv[u] = *reinterpret_cast<const double2 *>(g.vy + min(2 * t + 2 * NT * u, vlast));
       double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-      for (int u = 0; u < BV; u++) {
-        const int j = 2 * t + 512 * u;
+      for (int u = 0; u < VB; u++) {
+        const int j = 2 * t + 2 * NT * u;
         s0 += (j < nvy) ? xv[u].x : 0.0;
         s1 += (j + 1 < nvy) ? xv[u].y : 0.0;
       }
-      for (int j = 2 * t + 512 * BV; j < nvy; j += 512) {
+      for (int j = 2 * t + 2 * NT * VB; j < nvy; j += 2 * NT) {
         s0 += g.vy[j];
         s1 += (j + 1 < nvy) ? g.vy[j + 1] : 0.0;
       }
-      vs = s0 + s1;
+      const double vs = wave_sum(s0 + s1);
+      if (lane == 0) sh[0][w] = vs;
     }
+    // row col's SYMV partials (for W(col, ii))
+    if (col >= 0) {
+      double pr = 0.0, pi = 0.0;
+      for (int c = t; c < nch; c += NT) {
+        pr += g.partr[(size_t)c * n + col];
+        pi += g.parti[(size_t)c * n + col];
+      }
+      pr = wave_sum(pr);
+      pi = wave_sum(pi);
+      if (lane == 0) {
+        sh[1][w] = pr;
+        sh[2][w] = pi;
+      }
+    }
+    // V^H v, W^H v: columns 2dq, 2dq + 1 of tiles dp, dp + DG, ...
     {
+      const int dq = t & 31, dp = t >> 5;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      {
+        double2 xd[DB], zd[DB];
 #pragma unroll
-      for (int u = 0; u < BP; u++) {
-        yr += (q8 + 8 * u < nch) ? xpr[u] : 0.0;
-        yi += (q8 + 8 * u < nch) ? xpi[u] : 0.0;
+        for (int u = 0; u < DB; u++) {
+          const size_t o = (size_t)min(dp + DG * u, ntile - 1) * 2 * TNB + 2 * dq;
+          xd[u] = *reinterpret_cast<const double2 *>(g.dotr + o);
+          zd[u] = *reinterpret_cast<const double2 *>(g.doti + o);
+        }
+#pragma unroll
+        for (int u = 0; u < DB; u++) {
+          const bool k = dp + DG * u < ntile;
+          s0 += k ? xd[u].x : 0.0;
+          s1 += k ? xd[u].y : 0.0;
+          s2 += k ? zd[u].x : 0.0;
+          s3 += k ? zd[u].y : 0.0;
+        }
       }
-      for (int c0 = q8 + 8 * BP; c0 < nch; c0 += 8) {
-        yr += g.partr[(size_t)c0 * n + rs];
-        yi += g.parti[(size_t)c0 * n + rs];
+      for (int b = dp + DG * DB; b < ntile; b += DG) {
+        const size_t o = (size_t)b * 2 * TNB + 2 * dq;
+        const double2 x = *reinterpret_cast<const double2 *>(g.dotr + o);
+        const double2 z = *reinterpret_cast<const double2 *>(g.doti + o);
+        s0 += x.x;
+        s1 += x.y;
+        s2 += z.x;
+        s3 += z.y;
       }
+      dred[0][dp][2 * dq] = s0;
+      dred[0][dp][2 * dq + 1] = s1;
+      dred[1][dp][2 * dq] = s2;
+      dred[1][dp][2 * dq + 1] = s3;
     }
-    for (int c = t + blockDim.x; c < nch && col >= 0; c += blockDim.x) {
-      pc_r += g.partr[(size_t)c * n + col];
-      pc_i += g.parti[(size_t)c * n + col];
-    }
-    vs = wave_sum(vs);
-    const double psr = wave_sum(pc_r), psi = wave_sum(pc_i);
-    if (lane == 0) { sh[0][wid] = vs; sh[1][wid] = psr; sh[2][wid] = psi; }
   }
-  if (col >= 0 && t < jj) {
-    vcr[t] = vc_r; vci[t] = vc_i;
-    if (t < ii || fin < 0) { wcr[t] = wc_r; wci[t] = wc_i; }
+  // ---- this wave's share of the rows' sums
+  double kvr[KW], kvi[KW], kwr[KW], kwi[KW];  // V(r, k), W(r, k), k = KW w + u
+#pragma unroll
+  for (int u = 0; u < KW; u++) {
+    const size_t o = (size_t)n * (KW * w + u) + rs;
+    kvr[u] = Vr[o];
+    kvi[u] = Vi[o];
+    kwr[u] = Wr[o];
+    kwi[u] = Wi[o];
+  }
+  double yr = 0.0, yi = 0.0;
+  if (fin >= 0) {
+    for (int c0 = w; c0 < nch; c0 += ZCOL_WAVES * YB) {
+      double xr[YB], xi[YB];
+#pragma unroll
+      for (int u = 0; u < YB; u++) {
+        const size_t o = (size_t)min(c0 + ZCOL_WAVES * u, nch - 1) * n + rs;
+        xr[u] = g.partr[o];
+        xi[u] = g.parti[o];
+      }
+#pragma unroll
+      for (int u = 0; u < YB; u++) {
+        const bool k = c0 + ZCOL_WAVES * u < nch;
+        yr += k ? xr[u] : 0.0;
+        yi += k ? xi[u] : 0.0;
+      }
+    }
   }
   __syncthreads();
   if (fin >= 0 && t < 2 * TNB) {
     double qr = 0.0, qi = 0.0;
 #pragma unroll
-    for (int p = 0; p < 8; p++) { qr += reddr[p][t]; qi += reddi[p][t]; }
+    for (int p = 0; p < DG; p++) {
+      qr += dred[0][p][t];
+      qi += dred[1][p][t];
+    }
     const bool use = (t & (TNB - 1)) < ii;
     abr[t] = use ? qr : 0.0;
     abi[t] = use ? qi : 0.0;
   }
   __syncthreads();
-  double wciir = 0.0, wciii = 0.0;  // W(col, ii)
+  {
+    double zr = 0.0, zi = 0.0, a0r = 0.0, a0i = 0.0;
+#pragma unroll
+    for (int u = 0; u < KW; u++) {
+      const int k = KW * w + u;
+      if (fin >= 0 && k < ii) {  // W(r,k) a_k + V(r,k) b_k
+        zr += kwr[u] * abr[k] - kwi[u] * abi[k] + kvr[u] * abr[TNB + k] - kvi[u] * abi[TNB + k];
+        zi += kwr[u] * abi[k] + kwi[u] * abr[k] + kvr[u] * abi[TNB + k] + kvi[u] * abr[TNB + k];
+      }
+      // V(r,k) conj(W(col,k)) + W(r,k) conj(V(col,k)), k < jj, k != ii; panel
+      // columns k >= jj hold values of an earlier panel (or never written
+      // memory): selected away, never multiplied by zero
+      const bool use = col >= 0 && k < jj && k != ii;
+      const double wcr_ = use ? wcr[min(k, TNB - 1)] : 0.0, wci_ = use ? wci[min(k, TNB - 1)] : 0.0;
+      const double vcr_ = use ? vcr[min(k, TNB - 1)] : 0.0, vci_ = use ? vci[min(k, TNB - 1)] : 0.0;
+      const double vr_ = use ? kvr[u] : 0.0, vi_ = use ? kvi[u] : 0.0;
+      const double wr_ = use ? kwr[u] : 0.0, wi_ = use ? kwi[u] : 0.0;
+      a0r += vr_ * wcr_ + vi_ * wci_ + wr_ * vcr_ + wi_ * vci_;
+      a0i += vi_ * wcr_ - vr_ * wci_ + wi_ * vcr_ - wr_ * vci_;
+    }
+    red[0][w][lane] = yr;
+    red[1][w][lane] = yi;
+    red[2][w][lane] = zr;
+    red[3][w][lane] = zi;
+    red[4][w][lane] = a0r;
+    red[5][w][lane] = a0i;
+  }
+  __syncthreads();
+  // W(q, ii) = 0 above this kernel's rows
   if (fin >= 0) {
-    const double vty = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+    double *Wir = g.Pr + (size_t)n * (nb + fin - p0), *Wii = g.Pi + (size_t)n * (nb + fin - p0);
+    for (int q = blockIdx.x * NT + t; q < rbase; q += gridDim.x * NT) {
+      Wir[q] = 0.0;
+      Wii[q] = 0.0;
+    }
+  }
+  if (w != 0) return;
+  // ---- wave 0 finishes its 64 rows
+  double sy[6];
+#pragma unroll
+  for (int a = 0; a < 6; a++) {
+    double s = 0.0;
+#pragma unroll
+    for (int p = 0; p < ZCOL_WAVES; p++) s += red[a][p][lane];
+    sy[a] = s;
+  }
+  double wrr = 0.0, wri = 0.0, viir = 0.0, viii = 0.0, wciir = 0.0, wciii = 0.0;
+  if (fin >= 0) {
+    const double tr = g.taur[fin], ti = g.taui[fin];
+    viir = Vr[(size_t)n * ii + rs];
+    viii = Vi[(size_t)n * ii + rs];
+    double vty = 0.0;
+#pragma unroll
+    for (int p = 0; p < ZCOL_WAVES; p++) vty += sh[0][p];
     double ssum = 0.0;
 #pragma unroll
     for (int k = 0; k < TNB; k++) ssum += abr[k] * abr[TNB + k] + abi[k] * abi[TNB + k];
-    alpha2 = -0.5 * (tr * tr + ti * ti) * (vty - 2.0 * ssum);
-    // y -= W(r,k) a_k + V(r,k) b_k over this thread's columns
-    double zr = 0.0, zi = 0.0;
-#pragma unroll
-    for (int u = 0; u < KO; u++) {
-      const int k = KO * q8 + u;
-      if (k < ii) {
-        zr += wkr[u] * abr[k] - wki[u] * abi[k] + vkr[u] * abr[TNB + k] - vki[u] * abi[TNB + k];
-        zi += wkr[u] * abi[k] + wki[u] * abr[k] + vkr[u] * abi[TNB + k] + vki[u] * abr[TNB + k];
-      }
-    }
-    yr = oct_sum(yr) - oct_sum(zr);
-    yi = oct_sum(yi) - oct_sum(zi);
-    wrr = tr * yr - ti * yi + alpha2 * viir;
-    wri = tr * yi + ti * yr + alpha2 * viii;
+    const double alpha2 = -0.5 * (tr * tr + ti * ti) * (vty - 2.0 * ssum);
+    const double yr_ = sy[0] - sy[2], yi_ = sy[1] - sy[3];
+    wrr = tr * yr_ - ti * yi_ + alpha2 * viir;
+    wri = tr * yi_ + ti * yr_ + alpha2 * viii;
     double *Wir = g.Pr + (size_t)n * (nb + ii), *Wii = g.Pi + (size_t)n * (nb + ii);
-    if (live && q8 == 0) { Wir[r] = wrr; Wii[r] = wri; }
-    for (int q = blockIdx.x * 256 + t; q < rbase; q += gridDim.x * 256) { Wir[q] = 0.0; Wii[q] = 0.0; }
+    if (live) {
+      Wir[r] = wrr;
+      Wii[r] = wri;
+    }
     if (col >= 0) {
-      // row col = fin + 1 of the same formula (v(col) = 1), in every thread
+      // row col = fin + 1 of the same formula (v(col) = 1), in every lane
       double cr = 0.0, ci = 0.0;
       for (int k = 0; k < ii; k++) {
         cr += wcr[k] * abr[k] - wci[k] * abi[k] + vcr[k] * abr[TNB + k] - vci[k] * abi[TNB + k];
         ci += wcr[k] * abi[k] + wci[k] * abr[k] + vcr[k] * abi[TNB + k] + vci[k] * abr[TNB + k];
       }
-      const double sr = ((sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3])) - cr;
-      const double si = ((sh[2][0] + sh[2][1]) + (sh[2][2] + sh[2][3])) - ci;
+      double pr = 0.0, pi = 0.0;
+#pragma unroll
+      for (int p = 0; p < ZCOL_WAVES; p++) {
+        pr += sh[1][p];
+        pi += sh[2][p];
+      }
+      const double sr = pr - cr, si = pi - ci;
       wciir = tr * sr - ti * si + alpha2;
       wciii = tr * si + ti * sr;
     }
   }
   if (col < 0) return;
-  double xr = ar0, xi = ai0;
+  double xr = acr[rs], xi = aci[rs];
   {
-    // -= V(r,k) conj(W(col,k)) + W(r,k) conj(V(col,k)), k < jj
-    double a0r = 0.0, a0i = 0.0;
-#pragma unroll
-    for (int u = 0; u < KO; u++) {
-      const int k = KO * q8 + u;
-      const bool use = k < jj && k != ii;
-      const double wcr_ = use ? wcr[min(k, TNB - 1)] : 0.0, wci_ = use ? wci[min(k, TNB - 1)] : 0.0;
-      const double vcr_ = use ? vcr[min(k, TNB - 1)] : 0.0, vci_ = use ? vci[min(k, TNB - 1)] : 0.0;
-      // panel columns k >= jj hold values of an earlier panel (or never
-      // written memory): selected away, never multiplied by zero
-      const double vr_ = use ? vkr[u] : 0.0, vi_ = use ? vki[u] : 0.0;
-      const double wr_ = use ? wkr[u] : 0.0, wi_ = use ? wki[u] : 0.0;
-      a0r += vr_ * wcr_ + vi_ * wci_ + wr_ * vcr_ + wi_ * vci_;
-      a0i += vi_ * wcr_ - vr_ * wci_ + wi_ * vcr_ - wr_ * vci_;
-    }
-    a0r = oct_sum(a0r);
-    a0i = oct_sum(a0i);
+    double a0r = sy[4], a0i = sy[5];
     if (fin >= 0) {  // k = ii: this kernel's W(r, ii) and W(col, ii)
       const double vr_ = vcr[ii], vi_ = vci[ii];
       a0r += viir * wciir + viii * wciii + wrr * vr_ + wri * vi_;
@@ -638,23 +673,23 @@ __global__ __launch_bounds__(256) void k_ztri_col(ZTriWork g, int fin, int col, 
     xi -= a0i;
   }
   double nrm = 0.0;
-  if (live && q8 == 0 && r >= col) {
+  if (live && r >= col) {
     if (r == col) xi = 0.0;
     acr[r] = xr;
     aci[r] = xi;
     if (r == col) {
       g.d[col] = xr;
       if (col >= n - 1) {
-        g.taur[col] = 0.0; g.taui[col] = 0.0;
-        g.scr[col] = 0.0; g.sci[col] = 0.0;
+        g.taur[col] = 0.0;
+        g.taui[col] = 0.0;
+        g.scr[col] = 0.0;
+        g.sci[col] = 0.0;
       }
     }
     if (r >= col + 2) nrm = xr * xr + xi * xi;
   }
   nrm = wave_sum(nrm);
-  if (lane == 0) sh[0][wid] = nrm;  // sh[0] was last read before the second barrier
-  __syncthreads();
-  if (t == 0) g.norm[blockIdx.x] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+  if (lane == 0) g.norm[blockIdx.x] = nrm;
 }
 
 __global__ void k_ztri_store(ZTriWork g, int ncols)
@@ -804,17 +839,44 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
     // this equals inverting conj(U_kk))
     hipLaunchKernelGGL(k_negate, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nn, Wi);
     W2K_STAGE(s, "k_negate");
-    for (int j0 = 0; j0 < n; j0 += NB) {
-      const int kb = std::min(NB, n - j0);
-      if (j0 > 0) {
-        // T = W(0:j0,0:j0) conj(U)(0:j0, j0:j0+kb); W(0:j0, j0:) = -T W_jj
-        const size_t oc = (size_t)n * j0, ojj = (size_t)j0 + (size_t)n * j0;
-        if ((rc = zgemm(s, 'N', 'R', j0, kb, j0, 1.0, Wr, Wi, n, Ur + oc, Ui + oc, n, 0.0, Xr, Xi, n, 0, 1, 0, ws, wsn)))
-          return rc;
-        if ((rc = zgemm(s, 'N', 'N', j0, kb, kb, -1.0, Xr, Xi, n, Wr + ojj, Wi + ojj, n, 0.0, Wr + oc, Wi + oc, n, 0, 0,
-                        1)))
-          return rc;
+    // V^-1 by recursive halving over the NB-blocks: with V = [V11 V12; 0 V22]
+    // and the inverses W11, W22 of the halves, W12 = -W11 V12 W22.  Most of
+    // the arithmetic lands in a few large products (the top level alone is
+    // three quarters of it) instead of one thin product per block column.
+    struct Range {
+      int lo, hi;
+    };
+    std::vector<Range> order;  // halves in post-order: both halves before their parent
+    {
+      std::vector<std::pair<Range, int>> stack{{{0, n}, 0}};
+      while (!stack.empty()) {
+        auto [rg, seen] = stack.back();
+        stack.pop_back();
+        const int nblk = (rg.hi - rg.lo + NB - 1) / NB;
+        if (nblk <= 1) continue;
+        const int mid = rg.lo + (nblk / 2) * NB;
+        if (seen) {
+          order.push_back(rg);
+          continue;
+        }
+        stack.push_back({rg, 1});
+        stack.push_back({{mid, rg.hi}, 0});
+        stack.push_back({{rg.lo, mid}, 0});
       }
+    }
+    for (const Range &rg : order) {
+      const int nblk = (rg.hi - rg.lo + NB - 1) / NB;
+      const int mid = rg.lo + (nblk / 2) * NB;
+      const int s1 = mid - rg.lo, s2 = rg.hi - mid;
+      const size_t o11 = (size_t)rg.lo + (size_t)n * rg.lo, o12 = (size_t)rg.lo + (size_t)n * mid;
+      const size_t o22 = (size_t)mid + (size_t)n * mid;
+      // T = W11 conj(U12) (W11 upper triangular); W12 = -T W22 (W22 upper)
+      if ((rc = zgemm(s, 'N', 'R', s1, s2, s1, 1.0, Wr + o11, Wi + o11, n, Ur + o12, Ui + o12, n, 0.0, Xr, Xi, n, 0, 1,
+                      0, ws, wsn)))
+        return rc;
+      if ((rc = zgemm(s, 'N', 'N', s1, s2, s2, -1.0, Xr, Xi, n, Wr + o22, Wi + o22, n, 0.0, Wr + o12, Wi + o12, n, 0,
+                      0, 1, ws, wsn)))
+        return rc;
     }
     if ((rc = zgemm(s, 'N', 'N', n, n, n, 1.0, Hr, Hi, n, Wr, Wi, n, 0.0, Xr, Xi, n, 0, 0, 1))) return rc;
     if ((rc = zgemm(s, 'C', 'N', n, n, n, 1.0, Wr, Wi, n, Xr, Xi, n, 0.0, Hr, Hi, n, 1, 2, 0))) return rc;
@@ -846,7 +908,8 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
     const int nb = std::min(TNB, n - p0);
     g.p0 = p0;
     g.nb = nb;
-    hipLaunchKernelGGL(k_ztri_col, dim3((n - p0 + ZCOL_ROWS - 1) / ZCOL_ROWS), dim3(256), 0, s, g, -1, p0, 0, 0, 0);
+    hipLaunchKernelGGL(k_ztri_col, dim3((n - p0 + ZCOL_ROWS - 1) / ZCOL_ROWS), dim3(64 * ZCOL_WAVES), 0, s, g, -1, p0, 0,
+                       0, 0);
     W2K_STAGE(s, "k_ztri_col");
     int nref = 0;
     for (int k = 0; k < nb; k++) {
@@ -862,7 +925,8 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
       W2K_STAGE(s, "k_ztri_symv");
       nsymv++;
       const int next = (k + 1 < nb) ? i + 1 : -1;
-      hipLaunchKernelGGL(k_ztri_col, dim3((rows + ZCOL_ROWS - 1) / ZCOL_ROWS), dim3(256), 0, s, g, i, next, nt, nt, nblk);
+      hipLaunchKernelGGL(k_ztri_col, dim3((rows + ZCOL_ROWS - 1) / ZCOL_ROWS), dim3(64 * ZCOL_WAVES), 0, s, g, i,
+                         next, nt, nt, nblk);
       W2K_STAGE(s, "k_ztri_col");
     }
     if (nref > 0) {
