@@ -294,6 +294,23 @@ __device__ __forceinline__ double wave_transpose_sum32(double *t, int lane)
   return t[0] + __shfl_xor(t[0], 32, 64);
 }
 
+// t[0..15] per lane -> returns, in lane l, the sum over the 64 lanes of
+// t[l % 16] (four halving steps within each 16-lane group, then the groups
+// added).  Overwrites t.
+__device__ __forceinline__ double wave_transpose_sum16(double *t, int lane)
+{
+  bfly_step<8>(t, lane);
+  bfly_step<4>(t, lane);
+  bfly_step<2>(t, lane);
+  bfly_step<1>(t, lane);
+  const double s = t[0] + __shfl_xor(t[0], 16, 64);
+  return s + __shfl_xor(s, 32, 64);
+}
+
+// columns of the tile taken per step of the SYMV's tile loop (8-column steps
+// at four or five waves per SIMD measured slower on the 100-atom cell)
+constexpr int ZQC = 16;
+
 __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
 {
   const int n = g.n, r0 = i + 1, ii = i - g.p0, nb = g.nb;
@@ -317,32 +334,32 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
     }
   };
   if (bid >= nblk) {
-    const int I = bid - nblk;
+    // four waves per row block: V then W, each Re then Im of conj(X(R,k)) v(R)
+    const int I = (bid - nblk) >> 2, pass = (bid - nblk) & 3;
     const int R = r0 + SYMV_T * I + lane;
     const bool live = R < n;
     const int Rs = live ? R : r0;
     double vrr, vri;
     vof(R, vrr, vri);
-    double *Vr = g.Pr + (size_t)n * ii, *Vi = g.Pi + (size_t)n * ii;
-    double *V2r = g.Pr + (size_t)n * (2 * nb + ii), *V2i = g.Pi + (size_t)n * (2 * nb + ii);
-    if (live) { Vr[R] = vrr; Vi[R] = vri; V2r[R] = vrr; V2i[R] = vri; }
-    for (int q = I * SYMV_T + lane; q < r0; q += nt * SYMV_T) { Vr[q] = 0.0; Vi[q] = 0.0; V2r[q] = 0.0; V2i[q] = 0.0; }
+    if (pass == 0) {
+      double *Vr = g.Pr + (size_t)n * ii, *Vi = g.Pi + (size_t)n * ii;
+      double *V2r = g.Pr + (size_t)n * (2 * nb + ii), *V2i = g.Pi + (size_t)n * (2 * nb + ii);
+      if (live) { Vr[R] = vrr; Vi[R] = vri; V2r[R] = vrr; V2i[R] = vri; }
+      for (int q = I * SYMV_T + lane; q < r0; q += nt * SYMV_T) { Vr[q] = 0.0; Vi[q] = 0.0; V2r[q] = 0.0; V2i[q] = 0.0; }
+    }
     if (ii > 0) {
       double *dr = g.dotr + (size_t)I * 2 * TNB, *di = g.doti + (size_t)I * 2 * TNB;
-#pragma unroll 1
-      for (int pass = 0; pass < 4; pass++) {  // V then W, each Re then Im of conj(X(R,k)) v(R)
-        const int half = pass >> 1, im = pass & 1;
-        const double *Xr = g.Pr + (size_t)n * (half * nb), *Xi = g.Pi + (size_t)n * (half * nb);
-        double t[TNB];  // 32 values per pass keep the branch's registers at the tile loop's level
+      const int half = pass >> 1, im = pass & 1;
+      const double *Xr = g.Pr + (size_t)n * (half * nb), *Xi = g.Pi + (size_t)n * (half * nb);
+      double t[TNB];  // 32 values per wave keep the branch's registers at the tile loop's level
 #pragma unroll
-        for (int k = 0; k < TNB; k++) {
-          const double ur = Xr[(size_t)n * k + Rs], ui = Xi[(size_t)n * k + Rs];
-          const bool use = live && k < ii;
-          t[k] = use ? (im ? ur * vri - ui * vrr : ur * vrr + ui * vri) : 0.0;
-        }
-        const double sq = wave_transpose_sum32(t, lane);  // lane l < 32: column l
-        if (lane < TNB) (im ? di : dr)[half * TNB + lane] = sq;
+      for (int k = 0; k < TNB; k++) {
+        const double ur = Xr[(size_t)n * k + Rs], ui = Xi[(size_t)n * k + Rs];
+        const bool use = live && k < ii;
+        t[k] = use ? (im ? ur * vri - ui * vrr : ur * vrr + ui * vri) : 0.0;
       }
+      const double sq = wave_transpose_sum32(t, lane);  // lane l < 32: column l
+      if (lane < TNB) (im ? di : dr)[half * TNB + lane] = sq;
     }
     return;
   }
@@ -368,45 +385,46 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
   // diagonal tile uses its lower part by rows and its strict lower part by
   // columns (A^H v), the column sums folded into the rows they belong to
 #pragma unroll 1
-  for (int c0 = 0; c0 < m; c0 += 16) {  // four quarters of 16 columns (wave-uniform)
-    double q[32];  // q[j] = Re A(R, C0 + c0 + j), q[16 + j] = Im
+  for (int c0 = 0; c0 < m; c0 += ZQC) {  // steps of ZQC columns (wave-uniform)
+    double q[2 * ZQC];  // q[j] = Re A(R, C0 + c0 + j), q[ZQC + j] = Im
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
+    for (int j = 0; j < ZQC; j++) {
       const int jj = min(c0 + j, m - 1);
       q[j] = apr[(size_t)n * jj];
-      q[16 + j] = api[(size_t)n * jj];
+      q[ZQC + j] = api[(size_t)n * jj];
     }
     if (!full) {
 #pragma unroll
-      for (int j = 0; j < 16; j++) {
+      for (int j = 0; j < ZQC; j++) {
         const bool ok = live && c0 + j < m && (off || c0 + j <= lane);
         q[j] = ok ? q[j] : 0.0;
-        q[16 + j] = ok ? q[16 + j] : 0.0;
+        q[ZQC + j] = ok ? q[ZQC + j] : 0.0;
       }
     }
 #pragma unroll
-    for (int j = 0; j < 16; j++) cmad(q[j], q[16 + j], dbl_readlane(vcr, c0 + j), dbl_readlane(vci, c0 + j), yr, yi);
+    for (int j = 0; j < ZQC; j++)
+      cmad(q[j], q[ZQC + j], dbl_readlane(vcr, c0 + j), dbl_readlane(vci, c0 + j), yr, yi);
 #pragma unroll
-    for (int j = 0; j < 16; j++) {  // conj(A(R,c)) v(R); the diagonal tile's diagonal only by rows
+    for (int j = 0; j < ZQC; j++) {  // conj(A(R,c)) v(R); the diagonal tile's diagonal only by rows
       const bool use = off || c0 + j < lane;
-      const double ar = q[j], ai = q[16 + j];
+      const double ar = q[j], ai = q[ZQC + j];
       q[j] = use ? ar * vrr + ai * vri : 0.0;
-      q[16 + j] = use ? ar * vri - ai * vrr : 0.0;
+      q[ZQC + j] = use ? ar * vri - ai * vrr : 0.0;
     }
-    // lane l: entry l % 32 summed over the wave (Re of column l % 16 for
-    // l % 32 < 16, Im of column l % 16 above)
-    const double cs = wave_transpose_sum32(q, lane);
+    // lane l: entry l % (2 ZQC) summed over the wave (Re of column l % ZQC
+    // in the lower half of each group of 2 ZQC lanes, Im in the upper half)
+    const double cs = ZQC == 16 ? wave_transpose_sum32(q, lane) : wave_transpose_sum16(q, lane);
     if (off) {
-      const double co = __shfl_xor(cs, 16, 64);
-      const int cl = c0 + (lane & 15);
-      if (lane < 32 && cl < m) ((lane & 16) ? g.parti : g.partr)[(size_t)I * n + C0 + cl] = cs;
+      const double co = __shfl_xor(cs, ZQC, 64);
+      const int cl = c0 + (lane & (ZQC - 1));
+      if (lane < 2 * ZQC && cl < m) ((lane & ZQC) ? g.parti : g.partr)[(size_t)I * n + C0 + cl] = cs;
       const double vcl_r = __shfl(vcr, cl, 64), vcl_i = __shfl(vci, cl, 64);
-      if (lane < 16 && cl < m) vyc += vcl_r * cs + vcl_i * co;  // Re(conj(v_c) (A^H v)_c)
+      if (lane < ZQC && cl < m) vyc += vcl_r * cs + vcl_i * co;  // Re(conj(v_c) (A^H v)_c)
     } else {
       // column c0 + j of the tile is row C0 + c0 + j: lane c0 + j takes it
       const int src = lane - c0;
-      const bool mine = src >= 0 && src < 16;
-      const double fr = __shfl(cs, mine ? src : 0, 64), fi = __shfl(cs, mine ? src + 16 : 16, 64);
+      const bool mine = src >= 0 && src < ZQC;
+      const double fr = __shfl(cs, mine ? src : 0, 64), fi = __shfl(cs, mine ? src + ZQC : ZQC, 64);
       if (mine) {
         yr += fr;
         yi += fi;
@@ -919,9 +937,9 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
       const int rows = n - (i + 1);
       const int nt = (rows + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
       // timed launches carry their events in the dispatch itself
-      if (kt) hipExtLaunchKernelGGL(k_ztri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, c.kev[2 * nsymv],
+      if (kt) hipExtLaunchKernelGGL(k_ztri_symv, dim3(nblk + 4 * nt), dim3(SYMV_T), 0, s, c.kev[2 * nsymv],
                                     c.kev[2 * nsymv + 1], 0, g, i, nt);
-      else hipLaunchKernelGGL(k_ztri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
+      else hipLaunchKernelGGL(k_ztri_symv, dim3(nblk + 4 * nt), dim3(SYMV_T), 0, s, g, i, nt);
       W2K_STAGE(s, "k_ztri_symv");
       nsymv++;
       const int next = (k + 1 < nb) ? i + 1 : -1;
