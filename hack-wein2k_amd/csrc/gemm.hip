@@ -327,7 +327,7 @@ struct ZGemmArgs {
 };
 
 template <int TA, int TB>
-__global__ __launch_bounds__(256) void k_zgemm(ZGemmArgs g)
+__global__ __launch_bounds__(256, 2) void k_zgemm(ZGemmArgs g)
 {
   __shared__ double Ar_s[2][BK][BM + PAD], Ai_s[2][BK][BM + PAD];
   __shared__ double Br_s[2][BK][BN + PAD], Bi_s[2][BK][BN + PAD];
@@ -338,6 +338,24 @@ __global__ __launch_bounds__(256) void k_zgemm(ZGemmArgs g)
   const int lane = tid & 63, wave = tid >> 6;
   const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
+  // C (beta != 0) is read before the K loop, from clamped addresses, so that
+  // its loads overlap the products instead of following them
+  double pcr[2][2][4], pci[2][2][4];
+  if (g.beta != 0.0) {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        const int j = min(j0 + wj + 16 * b + li, g.n - 1);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = min(i0 + wi + 16 * a + lk + 4 * r, g.m - 1);
+          const size_t o = (size_t)i + (size_t)g.ldc * j;
+          pcr[a][b][r] = g.Cr[o];
+          pci[a][b][r] = g.Ci[o];
+        }
+      }
+  }
   d4 cr[2][2], ci[2][2];
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++) {
@@ -428,8 +446,8 @@ __global__ __launch_bounds__(256) void k_zgemm(ZGemmArgs g)
         const size_t o = (size_t)i + (size_t)g.ldc * j;
         double vr = g.alpha * cr[a][b][r], vi = g.alpha * ci[a][b][r];
         if (g.beta != 0.0) {
-          vr += g.beta * g.Cr[o];
-          vi += g.beta * g.Ci[o];
+          vr += g.beta * pcr[a][b][r];
+          vi += g.beta * pci[a][b][r];
         }
         g.Cr[o] = vr;
         g.Ci[o] = vi;

@@ -292,40 +292,36 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
   const int nblk = nt * (nt + 1) / 2;
   const int bid = blockIdx.x;
   const double *ai = g.a + (size_t)lda * i;
-  double t[SYMV_T];
   double xn[NORM_B], alpha;
   tri_norm_load(g, i, lane, xn, alpha);
   if (bid >= nblk) {
-    // row block I: V column and panel dot products
-    const int I = bid - nblk;
+    // row block I, two waves: V column and V^T v (half 0), W^T v (half 1)
+    const int I = (bid - nblk) >> 1, half = (bid - nblk) & 1;
     const int R = base + SYMV_T * I + lane;
     const bool live = R >= r0 && R < n;
     const int Rs = live ? R : r0;
     const double a_r = ai[Rs];
+    double t[TNB];  // 32 values per wave keep the branch's registers at the tile loop's level
     if (ii > 0) {
 #pragma unroll
-      for (int k = 0; k < TNB; k++) {
-        t[k] = g.P[(size_t)n * k + Rs];
-        t[TNB + k] = g.P[(size_t)n * (nb + k) + Rs];
-      }
+      for (int k = 0; k < TNB; k++) t[k] = g.P[(size_t)n * (half * nb + k) + Rs];
     }
     const double sc = tri_reflector(g, i, lane, false, xn, alpha);
     const double vr = live ? (R == r0 ? 1.0 : a_r * sc) : 0.0;
-    if (live) {
+    if (live && half == 0) {
       g.P[(size_t)n * ii + R] = vr;
       g.P[(size_t)n * (2 * nb + ii) + R] = vr;
     }
     if (ii > 0) {
-      // t[k] = V(R,k) v(R), t[TNB+k] = W(R,k) v(R) for k < ii, summed over
-      // the lanes: lane q ends with a_q (q < TNB) or b_(q-TNB)
+      // t[k] = X(R,k) v(R) for k < ii, summed over the lanes: lane q < TNB
+      // ends with the sum of column q
 #pragma unroll
       for (int k = 0; k < TNB; k++) {
         const bool use = live && k < ii;
         t[k] = use ? t[k] * vr : 0.0;
-        t[TNB + k] = use ? t[TNB + k] * vr : 0.0;
       }
-      const double dq = wave_transpose_sum(t, lane);
-      g.dots[(size_t)I * 2 * TNB + lane] = dq;
+      const double dq = wave_transpose_sum32(t, lane);
+      if (lane < TNB) g.dots[(size_t)I * 2 * TNB + half * TNB + lane] = dq;
     }
     return;
   }
@@ -345,6 +341,7 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
   const double a_c = ai[min(C0 + lane, n - 1)];
   const double a_r = ai[Rl];
   const double *ap = g.a + (size_t)Rl + (size_t)lda * C0;
+  double t[SYMV_T];
 #pragma unroll
   for (int j = 0; j < SYMV_T; j++) t[j] = ap[(size_t)lda * min(j, m - 1)];
   const double sc = tri_reflector(g, i, lane, bid == 0, xn, alpha);
@@ -1670,9 +1667,9 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
           const int nt = (n - base + SYMV_T - 1) / SYMV_T, nblk = nt * (nt + 1) / 2;
           // timed launches carry their events in the dispatch itself, so the
           // duration is the kernel's own (no marker packets between launches)
-          if (kt) hipExtLaunchKernelGGL(k_tri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, c.kev[2 * nsymv],
+          if (kt) hipExtLaunchKernelGGL(k_tri_symv, dim3(nblk + 2 * nt), dim3(SYMV_T), 0, s, c.kev[2 * nsymv],
                                         c.kev[2 * nsymv + 1], 0, g, i, nt);
-          else hipLaunchKernelGGL(k_tri_symv, dim3(nblk + nt), dim3(SYMV_T), 0, s, g, i, nt);
+          else hipLaunchKernelGGL(k_tri_symv, dim3(nblk + 2 * nt), dim3(SYMV_T), 0, s, g, i, nt);
           W2K_STAGE(s, "k_tri_symv");
           nsymv++;
           const int next = (k + 1 < nb) ? i + 1 : -1;

@@ -281,32 +281,6 @@ __device__ __forceinline__ double dbl_readlane(double v, int src)
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// t[0..31] per lane -> returns, in lane l, the sum over the 64 lanes of
-// t[l % 32] (five halving steps within each 32-lane half, then the halves
-// added).  Overwrites t.
-__device__ __forceinline__ double wave_transpose_sum32(double *t, int lane)
-{
-  bfly_step<16>(t, lane);
-  bfly_step<8>(t, lane);
-  bfly_step<4>(t, lane);
-  bfly_step<2>(t, lane);
-  bfly_step<1>(t, lane);
-  return t[0] + __shfl_xor(t[0], 32, 64);
-}
-
-// t[0..15] per lane -> returns, in lane l, the sum over the 64 lanes of
-// t[l % 16] (four halving steps within each 16-lane group, then the groups
-// added).  Overwrites t.
-__device__ __forceinline__ double wave_transpose_sum16(double *t, int lane)
-{
-  bfly_step<8>(t, lane);
-  bfly_step<4>(t, lane);
-  bfly_step<2>(t, lane);
-  bfly_step<1>(t, lane);
-  const double s = t[0] + __shfl_xor(t[0], 16, 64);
-  return s + __shfl_xor(s, 32, 64);
-}
-
 // columns of the tile taken per step of the SYMV's tile loop (8-column steps
 // at four or five waves per SIMD measured slower on the 100-atom cell)
 constexpr int ZQC = 16;

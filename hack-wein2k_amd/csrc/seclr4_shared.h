@@ -70,6 +70,32 @@ __device__ __forceinline__ double wave_transpose_sum(double *t, int lane)
   return t[0];
 }
 
+// t[0..31] per lane -> returns, in lane l, the sum over the 64 lanes of
+// t[l % 32] (five halving steps within each 32-lane half, then the halves
+// added).  Overwrites t.
+__device__ __forceinline__ double wave_transpose_sum32(double *t, int lane)
+{
+  bfly_step<16>(t, lane);
+  bfly_step<8>(t, lane);
+  bfly_step<4>(t, lane);
+  bfly_step<2>(t, lane);
+  bfly_step<1>(t, lane);
+  return t[0] + __shfl_xor(t[0], 32, 64);
+}
+
+// t[0..15] per lane -> returns, in lane l, the sum over the 64 lanes of
+// t[l % 16] (four halving steps within each 16-lane group, then the groups
+// added).  Overwrites t.
+__device__ __forceinline__ double wave_transpose_sum16(double *t, int lane)
+{
+  bfly_step<8>(t, lane);
+  bfly_step<4>(t, lane);
+  bfly_step<2>(t, lane);
+  bfly_step<1>(t, lane);
+  const double s = t[0] + __shfl_xor(t[0], 16, 64);
+  return s + __shfl_xor(s, 32, 64);
+}
+
 // device work buffers of one SECLR4 call (carved from Ctx::w_small / w_int);
 // complex quantities are planar: real parts, then imaginary parts at +stride
 struct Seclr4Bufs {
