@@ -48,6 +48,75 @@ static int zgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double al
   return 0;
 }
 
+// D (rows x cols, ld rows) = X + s Y (ld ldx)
+__global__ void k_zsum(int rows, int cols, const double *__restrict__ X, const double *__restrict__ Y, int ldx,
+                       double s, double *__restrict__ D)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= rows || j >= cols) return;
+  const size_t o = (size_t)i + (size_t)ldx * j;
+  D[(size_t)i + (size_t)rows * j] = X[o] + s * Y[o];
+}
+
+// three-product combination on the output (triangle tri): mode 0 after T =
+// A_r B_r: Cr = a T + b Cr, Ci -= a T; mode 1 after T = Im(op A) Im(op B):
+// Cr -= a T, Ci -= a T
+__global__ void k_z3m_comb(int m, int n, const double *__restrict__ T, double a, double b, double *__restrict__ Cr,
+                           double *__restrict__ Ci, int ldc, int tri, int mode)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= m || j >= n) return;
+  if ((tri == 1 && i < j) || (tri == 2 && i > j)) return;
+  const double t = a * T[(size_t)i + (size_t)m * j];
+  const size_t o = (size_t)i + (size_t)ldc * j;
+  if (mode == 0) {
+    Cr[o] = (b != 0.0 ? b * Cr[o] : 0.0) + t;
+    Ci[o] -= t;
+  } else {
+    Cr[o] -= t;
+    Ci[o] -= t;
+  }
+}
+
+// zgemm for the large products: three real products instead of four
+// (Ci = (Ar + Im op A)(Br + Im op B) - Ar Br - Im op A Im op B), which takes a
+// quarter of the arithmetic off at the cost of three elementwise passes and
+// an error of the imaginary part bounded by |A| |B| rather than by the
+// products of the parts.  Smaller products, and calls inside the
+// allocation-free entry point, go to zgemm.
+static int zgemm3(Ctx &c, hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *Ar,
+                  const double *Ai, int lda, const double *Br, const double *Bi, int ldb, double beta, double *Cr,
+                  double *Ci, int ldc, int tri = 0, int atri = 0, int btri = 0, double *ws = nullptr, size_t wsn = 0)
+{
+  const bool big = (double)m * n * k >= 1073741824.0 && k > 256;
+  if (!big || g_arena || m <= 0 || n <= 0 || k <= 0)
+    return zgemm(s, ta, tb, m, n, k, alpha, Ar, Ai, lda, Br, Bi, ldb, beta, Cr, Ci, ldc, tri, atri, btri, ws, wsn);
+  const double sa = (ta == 'C' || ta == 'R') ? -1.0 : 1.0, sb = (tb == 'C' || tb == 'R') ? -1.0 : 1.0;
+  const char ra = (ta == 'T' || ta == 'C') ? 'T' : 'N', rb = (tb == 'T' || tb == 'C') ? 'T' : 'N';
+  // stored extents of A and B
+  const int arows = ra == 'T' ? k : m, acols = ra == 'T' ? m : k;
+  const int brows = rb == 'T' ? n : k, bcols = rb == 'T' ? k : n;
+  const size_t na = (size_t)arows * acols, nbb = (size_t)brows * bcols, nt = (size_t)m * n;
+  int rc;
+  if ((rc = c.w_3m.alloc(na + nbb + nt))) return rc;
+  double *SA = c.w_3m.p, *SB = SA + na, *T = SB + nbb;
+  hipLaunchKernelGGL(k_zsum, dim3((arows + 255) / 256, acols), dim3(256), 0, s, arows, acols, Ar, Ai, lda, sa, SA);
+  W2K_STAGE(s, "k_zsum");
+  hipLaunchKernelGGL(k_zsum, dim3((brows + 255) / 256, bcols), dim3(256), 0, s, brows, bcols, Br, Bi, ldb, sb, SB);
+  W2K_STAGE(s, "k_zsum");
+  const dim3 gc((m + 255) / 256, n);
+  if ((rc = dgemm(s, ra, rb, m, n, k, alpha, SA, arows, SB, brows, beta, Ci, ldc, tri, atri, btri, ws, wsn))) return rc;
+  if ((rc = dgemm(s, ra, rb, m, n, k, 1.0, Ar, lda, Br, ldb, 0.0, T, m, tri, atri, btri, ws, wsn))) return rc;
+  hipLaunchKernelGGL(k_z3m_comb, gc, dim3(256), 0, s, m, n, T, alpha, beta, Cr, Ci, ldc, tri, 0);
+  W2K_STAGE(s, "k_z3m_comb");
+  if ((rc = dgemm(s, ra, rb, m, n, k, sa * sb, Ai, lda, Bi, ldb, 0.0, T, m, tri, atri, btri, ws, wsn))) return rc;
+  hipLaunchKernelGGL(k_z3m_comb, gc, dim3(256), 0, s, m, n, T, alpha, beta, Cr, Ci, ldc, tri, 1);
+  W2K_STAGE(s, "k_z3m_comb");
+  return 0;
+}
+
 __device__ __forceinline__ void cmad(double ar, double ai, double br, double bi, double &cr, double &ci)
 {
   cr += ar * br - ai * bi;
@@ -863,15 +932,15 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
       const size_t o11 = (size_t)rg.lo + (size_t)n * rg.lo, o12 = (size_t)rg.lo + (size_t)n * mid;
       const size_t o22 = (size_t)mid + (size_t)n * mid;
       // T = W11 conj(U12) (W11 upper triangular); W12 = -T W22 (W22 upper)
-      if ((rc = zgemm(s, 'N', 'R', s1, s2, s1, 1.0, Wr + o11, Wi + o11, n, Ur + o12, Ui + o12, n, 0.0, Xr, Xi, n, 0, 1,
+      if ((rc = zgemm3(c, s, 'N', 'R', s1, s2, s1, 1.0, Wr + o11, Wi + o11, n, Ur + o12, Ui + o12, n, 0.0, Xr, Xi, n, 0, 1,
                       0, ws, wsn)))
         return rc;
-      if ((rc = zgemm(s, 'N', 'N', s1, s2, s2, -1.0, Xr, Xi, n, Wr + o22, Wi + o22, n, 0.0, Wr + o12, Wi + o12, n, 0,
+      if ((rc = zgemm3(c, s, 'N', 'N', s1, s2, s2, -1.0, Xr, Xi, n, Wr + o22, Wi + o22, n, 0.0, Wr + o12, Wi + o12, n, 0,
                       0, 1, ws, wsn)))
         return rc;
     }
-    if ((rc = zgemm(s, 'N', 'N', n, n, n, 1.0, Hr, Hi, n, Wr, Wi, n, 0.0, Xr, Xi, n, 0, 0, 1))) return rc;
-    if ((rc = zgemm(s, 'C', 'N', n, n, n, 1.0, Wr, Wi, n, Xr, Xi, n, 0.0, Hr, Hi, n, 1, 2, 0))) return rc;
+    if ((rc = zgemm3(c, s, 'N', 'N', n, n, n, 1.0, Hr, Hi, n, Wr, Wi, n, 0.0, Xr, Xi, n, 0, 0, 1))) return rc;
+    if ((rc = zgemm3(c, s, 'C', 'N', n, n, n, 1.0, Wr, Wi, n, Xr, Xi, n, 0.0, Hr, Hi, n, 1, 2, 0))) return rc;
     dim3 g2((n + 255) / 256, n);
     hipLaunchKernelGGL(k_zmirror_lower, g2, dim3(256), 0, s, n, Hr, Hi);
     W2K_STAGE(s, "k_zmirror_lower");
@@ -1040,7 +1109,7 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
     hipLaunchKernelGGL(k_zgather_cols, dim3((n + 255) / 256, m), dim3(256), 0, s, n, m, B.perm, Zt, Zi, Ur, Ui);
     W2K_STAGE(s, "k_zgather_cols");
     // Z = W Zs into the (free) H storage, then interleaved into d_z
-    if ((rc = zgemm(s, 'N', 'N', n, m, n, 1.0, Wr, Wi, n, Ur, Ui, n, 0.0, Hr, Hi, n, 0, 1, 0))) return rc;
+    if ((rc = zgemm3(c, s, 'N', 'N', n, m, n, 1.0, Wr, Wi, n, Ur, Ui, n, 0.0, Hr, Hi, n, 0, 1, 0))) return rc;
     hipLaunchKernelGGL(k_interleave, dim3((unsigned)((zc + 255) / 256)), dim3(256), 0, s, zc, Hr, Hi, c.d_z.p);
     W2K_STAGE(s, "k_interleave");
     W2K_HIP(hipMemcpyAsync(c.d_eig.p, B.wv, sizeof(double) * m, hipMemcpyDeviceToDevice, s));

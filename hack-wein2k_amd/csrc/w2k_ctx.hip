@@ -412,7 +412,7 @@ void drop_kpoint_state(Ctx &c)
 {
   DBuf<double> *bufs[] = {&c.d_kvec, &c.d_hs, &c.d_hsdiag, &c.d_al, &c.d_bl, &c.d_cs, &c.d_theta, &c.d_colcoef,
                           &c.d_hnsL, &c.d_hnsR, &c.h_small, &c.h_re, &c.h_im, &c.d_eig, &c.d_z, &c.w_a,
-                          &c.w_b, &c.w_c, &c.w_v, &c.w_x, &c.w_small, &c.w_fac, &c.w_bt, &c.w_cl, &c.w_gk};
+                          &c.w_b, &c.w_c, &c.w_v, &c.w_x, &c.w_small, &c.w_fac, &c.w_bt, &c.w_cl, &c.w_gk, &c.w_3m};
   for (DBuf<double> *b : bufs) b->release();
   c.d_kzz.release();
   c.h_int.release();

@@ -181,6 +181,7 @@ struct Ctx {
   DBuf<double> w_bt;    // back-transform panel work
   DBuf<double> w_cl;    // cluster orthogonalisation work
   DBuf<double> w_gk;    // split-K GEMM partial products
+  DBuf<double> w_3m;    // three-product complex GEMM: Re + Im of both operands, one real product
   // launch mode of the tridiagonalisation, chosen per shape by measurement:
   // direct launches or one captured HIP graph replayed (seclr4.hip)
   int graphs = 0;  // 1: capture and replay where it measures faster (off by default)
