@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include <hip/hip_ext.h>
@@ -860,33 +861,53 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
     dim3 g2((n + 255) / 256, n);
     hipLaunchKernelGGL(k_zprep, g2, dim3(256), 0, s, n, c.d_hs.p, c.d_hsdiag.p, Ur, Ui, Hr, Hi);
     W2K_STAGE(s, "k_zprep");
-    // W: the diagonal-block inverses U_kk^-1 (zero elsewhere), used by the
-    // row panels here and, conjugated, by the reduction
+    // W: U^-1 (zero below the diagonal), formed with U and conjugated by the
+    // reduction
     hipLaunchKernelGGL(k_zfill, dim3((unsigned)((2 * nn + 255) / 256)), dim3(256), 0, s, 2 * nn, Wr, 0.0);
     W2K_STAGE(s, "k_zfill");
-    for (int k0 = 0; k0 < n; k0 += NB) {
-      const int kb = std::min(NB, n - k0);
-      hipLaunchKernelGGL(k_zpotf2, dim3(1), dim3(256), 0, s, n, k0, kb, Ur, Ui, B.info);
-      W2K_STAGE(s, "k_zpotf2");
-      hipLaunchKernelGGL(k_ztrtri_diag, dim3(1), dim3(256), 0, s, n, k0, kb, Ur, Ui, 0, Wr, Wi);
-      W2K_STAGE(s, "k_ztrtri_diag");
-      const int rest = n - k0 - kb;
-      if (rest > 0) {
-        // U12 = U11^-H S12 with U11^-1 from W
-        const size_t o11 = (size_t)k0 + (size_t)n * k0, o12 = (size_t)k0 + (size_t)n * (k0 + kb);
-        if ((rc = zgemm(s, 'C', 'N', kb, rest, kb, 1.0, Wr + o11, Wi + o11, n, Ur + o12, Ui + o12, n, 0.0, Xr, Xi, kb,
-                        0, 2, 0)))
-          return rc;
-        W2K_HIP(hipMemcpy2DAsync(Ur + o12, sizeof(double) * n, Xr, sizeof(double) * kb, sizeof(double) * kb, rest,
-                                 hipMemcpyDeviceToDevice, s));
-        W2K_HIP(hipMemcpy2DAsync(Ui + o12, sizeof(double) * n, Xi, sizeof(double) * kb, sizeof(double) * kb, rest,
-                                 hipMemcpyDeviceToDevice, s));
-        const size_t o22 = (size_t)(k0 + kb) + (size_t)n * (k0 + kb);
-        if ((rc = zgemm(s, 'C', 'N', rest, rest, kb, -1.0, Ur + o12, Ui + o12, n, Ur + o12, Ui + o12, n, 1.0,
-                        Ur + o22, Ui + o22, n, 2)))
-          return rc;
+    // S = U^H U and W = U^-1 together, by recursive halving over the
+    // NB-blocks: with S = [S11 S12; . S22], U11 and W11 from the first half,
+    // U12 = W11^H S12, S22 -= U12^H U12, U22 and W22 from the second half and
+    // W12 = -W11 U12 W22.  The block kernels factor and invert one NB-block;
+    // the rest is a few large products instead of one thin product per
+    // block row.
+    std::function<int(int, int)> chol = [&](int lo, int hi) -> int {
+      const int nblk = (hi - lo + NB - 1) / NB;
+      if (nblk <= 1) {
+        const int kb = hi - lo;
+        hipLaunchKernelGGL(k_zpotf2, dim3(1), dim3(256), 0, s, n, lo, kb, Ur, Ui, B.info);
+        W2K_STAGE(s, "k_zpotf2");
+        hipLaunchKernelGGL(k_ztrtri_diag, dim3(1), dim3(256), 0, s, n, lo, kb, Ur, Ui, 0, Wr, Wi);
+        W2K_STAGE(s, "k_ztrtri_diag");
+        return 0;
       }
-    }
+      const int mid = lo + (nblk / 2) * NB;
+      const int s1 = mid - lo, s2 = hi - mid;
+      const size_t o11 = (size_t)lo + (size_t)n * lo, o12 = (size_t)lo + (size_t)n * mid;
+      const size_t o22 = (size_t)mid + (size_t)n * mid;
+      int r;
+      if ((r = chol(lo, mid))) return r;
+      // U12 = W11^H S12 (W11^H lower triangular), through X
+      if ((r = zgemm3(c, s, 'C', 'N', s1, s2, s1, 1.0, Wr + o11, Wi + o11, n, Ur + o12, Ui + o12, n, 0.0, Xr, Xi, n, 0,
+                      2, 0, ws, wsn)))
+        return r;
+      W2K_HIP(hipMemcpy2DAsync(Ur + o12, sizeof(double) * n, Xr, sizeof(double) * n, sizeof(double) * s1, s2,
+                               hipMemcpyDeviceToDevice, s));
+      W2K_HIP(hipMemcpy2DAsync(Ui + o12, sizeof(double) * n, Xi, sizeof(double) * n, sizeof(double) * s1, s2,
+                               hipMemcpyDeviceToDevice, s));
+      // S22 -= U12^H U12 (upper triangle)
+      if ((r = zgemm3(c, s, 'C', 'N', s2, s2, s1, -1.0, Ur + o12, Ui + o12, n, Ur + o12, Ui + o12, n, 1.0, Ur + o22,
+                      Ui + o22, n, 2)))
+        return r;
+      if ((r = chol(mid, hi))) return r;
+      // W12 = -(W11 U12) W22 (W11, W22 upper triangular)
+      if ((r = zgemm3(c, s, 'N', 'N', s1, s2, s1, 1.0, Wr + o11, Wi + o11, n, Ur + o12, Ui + o12, n, 0.0, Xr, Xi, n, 0,
+                      1, 0, ws, wsn)))
+        return r;
+      return zgemm3(c, s, 'N', 'N', s1, s2, s2, -1.0, Xr, Xi, n, Wr + o22, Wi + o22, n, 0.0, Wr + o12, Wi + o12, n, 0,
+                    0, 1, ws, wsn);
+    };
+    if ((rc = chol(0, n))) return rc;
   }
   W2K_HIP(hipEventRecord(c.ev[7], s));
   int hinfo[4];
@@ -895,50 +916,9 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
   if (hinfo[0] != 0) return hinfo[0];  // ZPOTRF INFO, seclr4.F:200-205
   // ---- 2. reduction with V = conj(U): W = V^-1, X = H W, C = W^H X
   {
-    // the diagonal blocks of V^-1 = conj(U^-1): conjugate the Cholesky's
-    // U_kk^-1 (the kernel's arithmetic mirrors exactly under conjugation, so
-    // this equals inverting conj(U_kk))
+    // V^-1 = conj(U^-1), the Cholesky's W conjugated
     hipLaunchKernelGGL(k_negate, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nn, Wi);
     W2K_STAGE(s, "k_negate");
-    // V^-1 by recursive halving over the NB-blocks: with V = [V11 V12; 0 V22]
-    // and the inverses W11, W22 of the halves, W12 = -W11 V12 W22.  Most of
-    // the arithmetic lands in a few large products (the top level alone is
-    // three quarters of it) instead of one thin product per block column.
-    struct Range {
-      int lo, hi;
-    };
-    std::vector<Range> order;  // halves in post-order: both halves before their parent
-    {
-      std::vector<std::pair<Range, int>> stack{{{0, n}, 0}};
-      while (!stack.empty()) {
-        auto [rg, seen] = stack.back();
-        stack.pop_back();
-        const int nblk = (rg.hi - rg.lo + NB - 1) / NB;
-        if (nblk <= 1) continue;
-        const int mid = rg.lo + (nblk / 2) * NB;
-        if (seen) {
-          order.push_back(rg);
-          continue;
-        }
-        stack.push_back({rg, 1});
-        stack.push_back({{mid, rg.hi}, 0});
-        stack.push_back({{rg.lo, mid}, 0});
-      }
-    }
-    for (const Range &rg : order) {
-      const int nblk = (rg.hi - rg.lo + NB - 1) / NB;
-      const int mid = rg.lo + (nblk / 2) * NB;
-      const int s1 = mid - rg.lo, s2 = rg.hi - mid;
-      const size_t o11 = (size_t)rg.lo + (size_t)n * rg.lo, o12 = (size_t)rg.lo + (size_t)n * mid;
-      const size_t o22 = (size_t)mid + (size_t)n * mid;
-      // T = W11 conj(U12) (W11 upper triangular); W12 = -T W22 (W22 upper)
-      if ((rc = zgemm3(c, s, 'N', 'R', s1, s2, s1, 1.0, Wr + o11, Wi + o11, n, Ur + o12, Ui + o12, n, 0.0, Xr, Xi, n, 0, 1,
-                      0, ws, wsn)))
-        return rc;
-      if ((rc = zgemm3(c, s, 'N', 'N', s1, s2, s2, -1.0, Xr, Xi, n, Wr + o22, Wi + o22, n, 0.0, Wr + o12, Wi + o12, n, 0,
-                      0, 1, ws, wsn)))
-        return rc;
-    }
     if ((rc = zgemm3(c, s, 'N', 'N', n, n, n, 1.0, Hr, Hi, n, Wr, Wi, n, 0.0, Xr, Xi, n, 0, 0, 1))) return rc;
     if ((rc = zgemm3(c, s, 'C', 'N', n, n, n, 1.0, Wr, Wi, n, Xr, Xi, n, 0.0, Hr, Hi, n, 1, 2, 0))) return rc;
     dim3 g2((n + 255) / 256, n);
