@@ -86,14 +86,20 @@ __global__ void k_z3m_comb(int m, int n, const double *__restrict__ T, double a,
 // quarter of the arithmetic off at the cost of three elementwise passes and
 // an error of the imaginary part bounded by |A| |B| rather than by the
 // products of the parts.  Smaller products, and calls inside the
-// allocation-free entry point, go to zgemm.
+// allocation-free entry point, go to the fused kernel (K <= 512) or zgemm.
 static int zgemm3(Ctx &c, hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *Ar,
                   const double *Ai, int lda, const double *Br, const double *Bi, int ldb, double beta, double *Cr,
                   double *Ci, int ldc, int tri = 0, int atri = 0, int btri = 0, double *ws = nullptr, size_t wsn = 0)
 {
   const bool big = (double)m * n * k >= 1073741824.0 && k > 256;
-  if (!big || g_arena || m <= 0 || n <= 0 || k <= 0)
+  if (!big || g_arena || m <= 0 || n <= 0 || k <= 0) {
+    // the triangular operands of these products (U, U^-1, V^-1) hold zeros in
+    // their other triangle, so the fused kernel may read them as dense: one
+    // launch instead of four for the many small products of the recursions
+    if (k <= 512)
+      return zgemm_fused(s, ta, tb, m, n, k, alpha, Ar, Ai, lda, Br, Bi, ldb, beta, Cr, Ci, ldc, tri);
     return zgemm(s, ta, tb, m, n, k, alpha, Ar, Ai, lda, Br, Bi, ldb, beta, Cr, Ci, ldc, tri, atri, btri, ws, wsn);
+  }
   const double sa = (ta == 'C' || ta == 'R') ? -1.0 : 1.0, sb = (tb == 'C' || tb == 'R') ? -1.0 : 1.0;
   const char ra = (ta == 'T' || ta == 'C') ? 'T' : 'N', rb = (tb == 'T' || tb == 'C') ? 'T' : 'N';
   // stored extents of A and B
