@@ -232,7 +232,8 @@ struct TriWork {
 };
 
 
-constexpr int COL_ROWS = 32;  // rows per block of the column update (8 threads per row)
+constexpr int COL_ROWS = 64;  // rows per block of the column update (one per lane)
+constexpr int COL_WAVES = 8;  // waves per block of the column update
 
 // DLARFG of column i on (A(i+1,i), A(i+2:n,i)), from the squared norm
 // partials of the column-update kernel; every wave of the SYMV evaluates it
@@ -398,179 +399,176 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
   }
 }
 
-// sum over the 8 lanes 8q..8q+7, the same value in each
-__device__ __forceinline__ double oct_sum(double v)
-{
-  v += bfly_partner<1>(v);
-  v += bfly_partner<2>(v);
-  v += bfly_partner<4>(v);
-  return v;
-}
-
 // Finish W(:, fin - p0) (fin >= 0) and update column col (col >= 0) with the
-// panel so far; write the squared-norm partial of its sub-subdiagonal part
-// per block (the reflector itself is formed by the next SYMV).  Eight
-// threads per row, COL_ROWS rows per block; each thread takes an eighth of
-// its row's SYMV partials and of the panel columns.  Every load that does
-// not depend on a value formed here is issued before any of them is used
-// (fewer than 64 per thread: one round of memory latency): the first batch
-// of every sum is unconditional, with clamped indices and masked values, and
-// the tile partials are read in pairs.  Three barriers per block.
-__global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, int ntile, int nvy)
+// panel so far (DLATRD); write the squared-norm partial of its
+// sub-subdiagonal part per block (DLARFG itself is evaluated by the next
+// SYMV's waves).  A block takes COL_ROWS = 64 consecutive rows, one per lane,
+// in each of its COL_WAVES waves, so that every load of a SYMV partial or a
+// panel column is one contiguous 512-byte row segment; the waves split the
+// SYMV partials (chunk w, w + 8, ...) and the panel columns (four each) and
+// their row sums meet in LDS.  Loads are issued in batches with clamped
+// indices and masked values, so that a batch is in flight at once.
+__global__ __launch_bounds__(64 * COL_WAVES) void k_tri_col(TriWork g, int fin, int col, int ntile, int nvy)
 {
-  constexpr int BD = 16;       // first batch of V^T v, W^T v pairs (ntile <= 8 BD)
-  constexpr int BV = 12;       // first batch of v^T y pairs (nvy <= 512 BV)
-  constexpr int BP = 12;       // first batch of SYMV partials of a row (ntile <= 8 BP)
-  constexpr int KO = TNB / 8;  // panel columns per thread
-  const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  const int q8 = t & 7;
+  constexpr int NT = 64 * COL_WAVES;   // threads
+  constexpr int KW = TNB / COL_WAVES;  // panel columns per wave
+  constexpr int YB = 16;               // SYMV-partial chunks per wave and batch
+  constexpr int VB = 10;               // first batch of v^T y pairs per thread (nvy <= 2 NT VB)
+  constexpr int DG = NT / 32;          // groups of the V^T v, W^T v sums (32 column pairs each)
+  constexpr int DB = 8;                // first batch of tiles per group (ntile <= DG DB)
+  const int n = g.n, p0 = g.p0, nb = g.nb, t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int nch = ntile;
   const int rbase = fin >= 0 ? fin + 1 : col;
-  const int r = rbase + blockIdx.x * COL_ROWS + (t >> 3);
+  const int r = rbase + blockIdx.x * COL_ROWS + lane;
   const bool live = r < n;
   const int rs = live ? r : rbase;  // rbase < n: a safe row for dead lanes
-  __shared__ double ab[2 * TNB];    // a_k = V_k^T v, b_k = W_k^T v
-  __shared__ double wc[TNB], vc[TNB];
-  __shared__ double redd[8][2 * TNB];
-  __shared__ double sh[2][4];
   const int ii = fin >= 0 ? fin - p0 : -1;
   const int jj = col - p0;  // columns 0..jj-1 of the panel update column col
+  __shared__ double ab[2 * TNB];  // a_k = V_k^T v, b_k = W_k^T v
+  __shared__ double wc[TNB], vc[TNB];
+  __shared__ double dred[DG][2 * TNB];
+  __shared__ double red[3][COL_WAVES][COL_ROWS];  // per wave: y, z (W a + V b), a0
+  __shared__ double sh[2][COL_WAVES];
   const double *V = g.P, *W = g.P + (size_t)n * nb;
   double *ac = g.a + (size_t)g.lda * (col >= 0 ? col : 0);
-  // ---- loads
-  const double acr = (col >= 0) ? ac[rs] : 0.0;
-  double vcol = 0.0, wcol = 0.0, pcol = 0.0;  // row col of the panel, SYMV partial slot t of row col
+  // ---- per-block scalars: row col of the panel, v^T y, V^T v and W^T v
   if (col >= 0 && t < jj) {
-    vcol = V[(size_t)n * t + col];
-    if (t < ii || fin < 0) wcol = W[(size_t)n * t + col];
+    vc[t] = V[(size_t)n * t + col];
+    if (t < ii || fin < 0) wc[t] = W[(size_t)n * t + col];
   }
-  if (fin >= 0 && col >= 0 && t < nch) pcol = g.part[(size_t)t * n + col];
-  double wr = 0.0, vii = 0.0, tauf = 0.0, alpha2 = 0.0;
-  double vk[KO], wk[KO];  // V(r,k), W(r,k) for k = KO q8 + u
-  double y = 0.0, vs = 0.0;
-  const int dq = t & 31, dp = t >> 5;  // V^T v, W^T v: columns 2dq, 2dq+1 of tiles dp + 8u
   if (fin >= 0) {
-    tauf = g.tau[fin];
-#pragma unroll
-    for (int u = 0; u < KO; u++) {
-      vk[u] = V[(size_t)n * (KO * q8 + u) + rs];
-      wk[u] = W[(size_t)n * (KO * q8 + u) + rs];
-    }
-    vii = V[(size_t)n * ii + rs];
-    double2 xd[BD], xv[BV];
-    double xp[BP];
-    const int vlast = (nvy - 1) & ~1;  // the last pair (vy has one spare entry)
-#pragma unroll
-    for (int u = 0; u < BD; u++)
-      xd[u] = *reinterpret_cast<const double2 *>(g.dots + (size_t)min(dp + 8 * u, ntile - 1) * 2 * TNB + 2 * dq);
-#pragma unroll
-    for (int u = 0; u < BV; u++) xv[u] = *reinterpret_cast<const double2 *>(g.vy + min(2 * t + 512 * u, vlast));
-#pragma unroll
-    for (int u = 0; u < BP; u++) xp[u] = g.part[(size_t)min(q8 + 8 * u, nch - 1) * n + rs];
-    // V^T v, W^T v: partials of tiles dp (mod 8)
     {
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      const int vlast = (nvy - 1) & ~1;  // the last pair (vy has one spare entry)
+      double2 xv[VB];
 #pragma unroll
-      for (int u = 0; u < BD; u += 2) {
-        const bool k0 = dp + 8 * u < ntile, k1 = dp + 8 * u + 8 < ntile;
-        s0 += k0 ? xd[u].x : 0.0;
-        s1 += k0 ? xd[u].y : 0.0;
-        s2 += k1 ? xd[u + 1].x : 0.0;
-        s3 += k1 ? xd[u + 1].y : 0.0;
+      for (int u = 0; u < VB; u++) xv[u] = *reinterpret_cast<const double2 *>(g.vy + min(2 * t + 2 * NT * u, vlast));
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+      for (int u = 0; u < VB; u++) {
+        const int j = 2 * t + 2 * NT * u;
+        s0 += (j < nvy) ? xv[u].x : 0.0;
+        s1 += (j + 1 < nvy) ? xv[u].y : 0.0;
       }
-      for (int b = dp + 8 * BD; b < ntile; b += 8) {
+      for (int j = 2 * t + 2 * NT * VB; j < nvy; j += 2 * NT) {
+        s0 += g.vy[j];
+        s1 += (j + 1 < nvy) ? g.vy[j + 1] : 0.0;
+      }
+      const double vs = wave_sum(s0 + s1);
+      if (lane == 0) sh[0][w] = vs;
+    }
+    if (col >= 0) {
+      double p = 0.0;
+      for (int c = t; c < nch; c += NT) p += g.part[(size_t)c * n + col];
+      p = wave_sum(p);
+      if (lane == 0) sh[1][w] = p;
+    }
+    {
+      const int dq = t & 31, dp = t >> 5;
+      double s0 = 0.0, s1 = 0.0;
+      {
+        double2 xd[DB];
+#pragma unroll
+        for (int u = 0; u < DB; u++)
+          xd[u] = *reinterpret_cast<const double2 *>(g.dots + (size_t)min(dp + DG * u, ntile - 1) * 2 * TNB + 2 * dq);
+#pragma unroll
+        for (int u = 0; u < DB; u++) {
+          const bool k = dp + DG * u < ntile;
+          s0 += k ? xd[u].x : 0.0;
+          s1 += k ? xd[u].y : 0.0;
+        }
+      }
+      for (int b = dp + DG * DB; b < ntile; b += DG) {
         const double2 x = *reinterpret_cast<const double2 *>(g.dots + (size_t)b * 2 * TNB + 2 * dq);
         s0 += x.x;
         s1 += x.y;
       }
-      redd[dp][2 * dq] = s0 + s2;
-      redd[dp][2 * dq + 1] = s1 + s3;
-    }
-    // v^T y: the tile partials, in pairs
-    {
-      double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-      for (int u = 0; u < BV; u++) {
-        const int j = 2 * t + 512 * u;
-        s0 += (j < nvy) ? xv[u].x : 0.0;
-        s1 += (j + 1 < nvy) ? xv[u].y : 0.0;
-      }
-      for (int j = 2 * t + 512 * BV; j < nvy; j += 512) {
-        s0 += g.vy[j];
-        s1 += (j + 1 < nvy) ? g.vy[j + 1] : 0.0;
-      }
-      vs = s0 + s1;
-    }
-    // y(r): this thread's eighth of the row's SYMV partials (slots q8 + 8j)
-    {
-      double y0 = 0.0, y1 = 0.0;
-#pragma unroll
-      for (int u = 0; u < BP; u += 2) {
-        y0 += (q8 + 8 * u < nch) ? xp[u] : 0.0;
-        y1 += (q8 + 8 * u + 8 < nch) ? xp[u + 1] : 0.0;
-      }
-      for (int c0 = q8 + 8 * BP; c0 < nch; c0 += 8) y0 += g.part[(size_t)c0 * n + rs];
-      y = y0 + y1;
-    }
-    for (int c = t + blockDim.x; c < nch && col >= 0; c += blockDim.x) pcol += g.part[(size_t)c * n + col];
-    vs = wave_sum(vs);
-    const double ps = wave_sum(pcol);
-    if (lane == 0) {
-      sh[0][wid] = vs;
-      sh[1][wid] = ps;
+      dred[dp][2 * dq] = s0;
+      dred[dp][2 * dq + 1] = s1;
     }
   }
-  if (col >= 0 && t < jj) {
-    vc[t] = vcol;
-    if (t < ii || fin < 0) wc[t] = wcol;
+  // ---- this wave's share of the rows' sums
+  double kv[KW], kw[KW];  // V(r, k), W(r, k), k = KW w + u
+#pragma unroll
+  for (int u = 0; u < KW; u++) {
+    kv[u] = V[(size_t)n * (KW * w + u) + rs];
+    kw[u] = W[(size_t)n * (KW * w + u) + rs];
   }
-  __syncthreads();
+  double y = 0.0;
   if (fin >= 0) {
-    if (t < 2 * TNB) {
-      const double q = ((redd[0][t] + redd[1][t]) + (redd[2][t] + redd[3][t])) +
-                       ((redd[4][t] + redd[5][t]) + (redd[6][t] + redd[7][t]));
-      ab[t] = ((t & (TNB - 1)) < ii) ? q : 0.0;
+    for (int c0 = w; c0 < nch; c0 += COL_WAVES * YB) {
+      double xp[YB];
+#pragma unroll
+      for (int u = 0; u < YB; u++) xp[u] = g.part[(size_t)min(c0 + COL_WAVES * u, nch - 1) * n + rs];
+#pragma unroll
+      for (int u = 0; u < YB; u++) y += (c0 + COL_WAVES * u < nch) ? xp[u] : 0.0;
     }
   }
   __syncthreads();
-  double wcii = 0.0;  // W(col, ii)
+  if (fin >= 0 && t < 2 * TNB) {
+    double q = 0.0;
+#pragma unroll
+    for (int p = 0; p < DG; p++) q += dred[p][t];
+    ab[t] = ((t & (TNB - 1)) < ii) ? q : 0.0;
+  }
+  __syncthreads();
+  {
+    double z = 0.0, a0 = 0.0;
+#pragma unroll
+    for (int u = 0; u < KW; u++) {
+      const int k = KW * w + u;
+      if (fin >= 0 && k < ii) z += kw[u] * ab[k] + kv[u] * ab[TNB + k];
+      // V(r,k) W(col,k) + W(r,k) V(col,k), k < jj, k != ii; panel columns
+      // k >= jj hold values of an earlier panel: selected away
+      const bool use = col >= 0 && k < jj && k != ii;
+      a0 += (use ? kv[u] : 0.0) * (use ? wc[min(k, TNB - 1)] : 0.0) +
+            (use ? kw[u] : 0.0) * (use ? vc[min(k, TNB - 1)] : 0.0);
+    }
+    red[0][w][lane] = y;
+    red[1][w][lane] = z;
+    red[2][w][lane] = a0;
+  }
+  __syncthreads();
+  // W(q, ii) = 0 above this kernel's rows
+  if (fin >= 0)
+    for (int q = blockIdx.x * NT + t; q < rbase; q += gridDim.x * NT) g.P[(size_t)n * (nb + ii) + q] = 0.0;
+  if (w != 0) return;
+  // ---- wave 0 finishes its 64 rows
+  double sy[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    double sum = 0.0;
+#pragma unroll
+    for (int p = 0; p < COL_WAVES; p++) sum += red[a][p][lane];
+    sy[a] = sum;
+  }
+  double wr = 0.0, vii = 0.0, wcii = 0.0;  // W(r, ii), V(r, ii), W(col, ii)
   if (fin >= 0) {
-    const double vty = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
-    const double scol = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+    const double tauf = g.tau[fin];
+    vii = V[(size_t)n * ii + rs];
+    double vty = 0.0, scol = 0.0;
+#pragma unroll
+    for (int p = 0; p < COL_WAVES; p++) {
+      vty += sh[0][p];
+      scol += sh[1][p];
+    }
     double ssum = 0.0;
 #pragma unroll
     for (int k = 0; k < TNB; k++) ssum += ab[k] * ab[TNB + k];
-    alpha2 = -0.5 * tauf * tauf * (vty - 2.0 * ssum);
-    double z = 0.0;
-#pragma unroll
-    for (int u = 0; u < KO; u++) {
-      const int k = KO * q8 + u;
-      z += (k < ii ? wk[u] : 0.0) * ab[k] + (k < ii ? vk[u] : 0.0) * ab[TNB + k];
-    }
-    y = oct_sum(y) - oct_sum(z);
-    wr = tauf * y + alpha2 * vii;
-    if (live && q8 == 0) g.P[(size_t)n * (nb + ii) + r] = wr;
+    const double alpha2 = -0.5 * tauf * tauf * (vty - 2.0 * ssum);
+    wr = tauf * (sy[0] - sy[1]) + alpha2 * vii;
+    if (live) g.P[(size_t)n * (nb + ii) + r] = wr;
     if (col >= 0) {
-      // row col = fin + 1 of the same formula (v(col) = 1), in every thread
+      // row col = fin + 1 of the same formula (v(col) = 1), in every lane
       double corr = 0.0;
       for (int k = 0; k < ii; k++) corr += wc[k] * ab[k] + vc[k] * ab[TNB + k];
       wcii = tauf * (scol - corr) + alpha2;
     }
   }
   if (col < 0) return;
-  double a = acr;
-  if (fin >= 0) {
-    double a0 = 0.0;
-#pragma unroll
-    for (int u = 0; u < KO; u++) {
-      const int k = KO * q8 + u;
-      const bool use = k < ii;
-      a0 += (use ? vk[u] : 0.0) * (use ? wc[k] : 0.0) + (use ? wk[u] : 0.0) * (use ? vc[k] : 0.0);
-    }
-    a -= oct_sum(a0) + (vii * wcii + wr * vc[ii]);
-  }
+  double a = ac[rs] - sy[2];
+  if (fin >= 0) a -= vii * wcii + wr * vc[ii];
   double nrm = 0.0;
-  if (live && q8 == 0) {
+  if (live) {
     ac[r] = a;
     if (r == col) {
       g.d[col] = a;
@@ -582,9 +580,7 @@ __global__ __launch_bounds__(256) void k_tri_col(TriWork g, int fin, int col, in
     if (r >= col + 2) nrm = a * a;
   }
   nrm = wave_sum(nrm);
-  if (lane == 0) redd[0][wid] = nrm;  // redd was last read before the second barrier
-  __syncthreads();
-  if (t == 0) g.norm[blockIdx.x] = (redd[0][0] + redd[0][1]) + (redd[0][2] + redd[0][3]);
+  if (lane == 0) g.norm[blockIdx.x] = nrm;
 }
 
 // reflectors of panel columns into A below the subdiagonal (read by the back-transform)
@@ -1655,7 +1651,8 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
         g.p0 = p0;
         g.nb = nb;
         // reflector of the panel's first column (already updated by earlier panels)
-        hipLaunchKernelGGL(k_tri_col, dim3((n - p0 + COL_ROWS - 1) / COL_ROWS), dim3(256), 0, s, g, -1, p0, 0, 0);
+        hipLaunchKernelGGL(k_tri_col, dim3((n - p0 + COL_ROWS - 1) / COL_ROWS), dim3(64 * COL_WAVES), 0, s, g, -1, p0, 0,
+                           0);
         W2K_STAGE(s, "k_tri_col");
         int nref = 0;  // panel columns that carry a reflector
         for (int k = 0; k < nb; k++) {
@@ -1673,8 +1670,8 @@ static int seclr4_real(Ctx &c, double el, double eu, int nume, int *ne_out, int 
           W2K_STAGE(s, "k_tri_symv");
           nsymv++;
           const int next = (k + 1 < nb) ? i + 1 : -1;
-          hipLaunchKernelGGL(k_tri_col, dim3((rows + COL_ROWS - 1) / COL_ROWS), dim3(256), 0, s, g, i, next, nt,
-                             nblk);
+          hipLaunchKernelGGL(k_tri_col, dim3((rows + COL_ROWS - 1) / COL_ROWS), dim3(64 * COL_WAVES), 0, s, g, i,
+                             next, nt, nblk);
           W2K_STAGE(s, "k_tri_col");
         }
         if (nref > 0) {
