@@ -32,7 +32,7 @@ def _unlimited_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
-@pytest.mark.parametrize("name,nk", [("rs8", 1), ("c1_fcc", 2), ("lo_cplx", 1)])
+@pytest.mark.parametrize("name,nk", [("rs8", 1), ("c1_fcc", 2), ("lo_cplx", 1), ("c2_rs64", 1)])
 def test_reference_program_with_dropin(name, nk, tmp_path):
     fx = np.load(os.path.join(GOLDEN, f"{name}.npz"))
     exe = EXE[bool(int(fx["meta/complex"]))]
@@ -45,6 +45,11 @@ def test_reference_program_with_dropin(name, nk, tmp_path):
     r = subprocess.run([exe, "eigonly"], cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                        preexec_fn=_unlimited_stack, timeout=300)
     assert r.returncode == 0, r.stdout.decode(errors="replace")[-2000:]
+    # the program's own phase clocks (HS handed over PCIe around each routine)
+    times = dict(l.split() for l in open(os.path.join(tmp_path, "ref_times.txt")))
+    print(f"\n{name}: drop-in per k-point: HAMILT {float(times['hamilt_wall_s']) / nk:.3f} s, "
+          f"HNS {float(times['hns_wall_s']) / nk:.3f} s, SECLR4 {float(times['seclr4_wall_s']) / nk:.3f} s, "
+          f"whole program {float(times['total_wall_s']):.2f} s")
     for i in range(nk):
         k = read_dump(os.path.join(tmp_path, f"k{i + 1:04d}.bin"))
         ne = int(k["NE"])
