@@ -212,27 +212,20 @@ __global__ __launch_bounds__(256) void k_zpotf2(int n, int k0, int kb, double *_
   }
 }
 
-// inverse of the kb x kb upper block at (j0, j0) of U (cj = 0) or of conj(U)
-// (cj = 1) into W (zero below the diagonal)
-__global__ __launch_bounds__(256) void k_ztrtri_diag(int n, int j0, int kb, const double *__restrict__ Ur,
-                                                     const double *__restrict__ Ui, int cj, double *__restrict__ Wr,
-                                                     double *__restrict__ Wi)
+// R = S^-1 for a 64 x 64 upper triangular complex S in LDS (zero below the
+// diagonal; R zero on entry), by all 256 threads of the block: the 16 x 16
+// diagonal blocks by column back-substitution (threads 0..63, at most 16
+// steps each), then two doubling steps X12 = -X11 (S12 X22) over the block's
+// threads, through the product buffer P.  The longest dependent chain is
+// about 140 complex multiply-adds instead of 2080 for back-substitution
+// over all 64 rows.
+__device__ void ztri_inv64(double (*sr)[NB + 1], double (*si)[NB + 1], double (*rr)[NB + 1], double (*ri)[NB + 1],
+                           double (*pr)[NB / 2 + 1], double (*pi)[NB / 2 + 1], int t)
 {
-  __shared__ double sr[NB][NB + 1], si[NB][NB + 1], rr[NB][NB + 1], ri[NB][NB + 1];
-  const int t = threadIdx.x;
-  const double sg = cj ? -1.0 : 1.0;
-  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
-    const int r = idx % kb, c = idx / kb;
-    const size_t o = (size_t)(j0 + r) + (size_t)n * (j0 + c);
-    sr[r][c] = (r <= c) ? Ur[o] : 0.0;
-    si[r][c] = (r <= c) ? sg * Ui[o] : 0.0;
-    rr[r][c] = 0.0;
-    ri[r][c] = 0.0;
-  }
-  __syncthreads();
-  if (t < kb) {  // column c of the inverse by back substitution
-    const int c = t;
-    for (int r = c; r >= 0; r--) {
+  static_assert(NB == 64, "64 x 64 blocks");
+  if (t < NB) {
+    const int b0 = (t / 16) * 16, c = t;
+    for (int r = c; r >= b0; r--) {
       double xr = (r == c) ? 1.0 : 0.0, xi = 0.0;
       for (int q = r + 1; q <= c; q++) {
         xr -= sr[r][q] * rr[q][c] - si[r][q] * ri[q][c];
@@ -244,6 +237,63 @@ __global__ __launch_bounds__(256) void k_ztrtri_diag(int n, int j0, int kb, cons
     }
   }
   __syncthreads();
+  for (int h = 16; h < NB; h *= 2) {
+    const int nout = (NB / (2 * h)) * h * h;  // outputs of all pairs of this step
+    for (int o = t; o < nout; o += blockDim.x) {
+      const int pp = o / (h * h), rc = o % (h * h), r = rc % h, c = rc / h, b = 2 * h * pp;
+      double ar = 0.0, ai = 0.0;  // P(r, c) = sum_{q <= c} S(b + r, b + h + q) X22(q, c)
+      for (int q = 0; q <= c; q++) {
+        const double s_r = sr[b + r][b + h + q], s_i = si[b + r][b + h + q];
+        const double x_r = rr[b + h + q][b + h + c], x_i = ri[b + h + q][b + h + c];
+        ar += s_r * x_r - s_i * x_i;
+        ai += s_r * x_i + s_i * x_r;
+      }
+      pr[h * pp + r][c] = ar;
+      pi[h * pp + r][c] = ai;
+    }
+    __syncthreads();
+    for (int o = t; o < nout; o += blockDim.x) {
+      const int pp = o / (h * h), rc = o % (h * h), r = rc % h, c = rc / h, b = 2 * h * pp;
+      double ar = 0.0, ai = 0.0;  // X12(r, c) = -sum_{q >= r} X11(r, q) P(q, c)
+      for (int q = r; q < h; q++) {
+        const double x_r = rr[b + r][b + q], x_i = ri[b + r][b + q];
+        const double p_r = pr[h * pp + q][c], p_i = pi[h * pp + q][c];
+        ar += x_r * p_r - x_i * p_i;
+        ai += x_r * p_i + x_i * p_r;
+      }
+      rr[b + r][b + h + c] = -ar;
+      ri[b + r][b + h + c] = -ai;
+    }
+    __syncthreads();
+  }
+}
+
+// inverse of the kb x kb upper block at (j0, j0) of U (cj = 0) or of conj(U)
+// (cj = 1) into W (zero below the diagonal); a block shorter than 64 is
+// padded with the identity
+__global__ __launch_bounds__(256) void k_ztrtri_diag(int n, int j0, int kb, const double *__restrict__ Ur,
+                                                     const double *__restrict__ Ui, int cj, double *__restrict__ Wr,
+                                                     double *__restrict__ Wi)
+{
+  __shared__ double sr[NB][NB + 1], si[NB][NB + 1], rr[NB][NB + 1], ri[NB][NB + 1];
+  __shared__ double pr[NB / 2][NB / 2 + 1], pi[NB / 2][NB / 2 + 1];
+  const int t = threadIdx.x;
+  const double sg = cj ? -1.0 : 1.0;
+  for (int idx = t; idx < NB * NB; idx += blockDim.x) {
+    const int r = idx % NB, c = idx / NB;
+    double vr = (r == c) ? 1.0 : 0.0, vi = 0.0;
+    if (r < kb && c < kb) {
+      const size_t o = (size_t)(j0 + r) + (size_t)n * (j0 + c);
+      vr = (r <= c) ? Ur[o] : 0.0;
+      vi = (r <= c) ? sg * Ui[o] : 0.0;
+    }
+    sr[r][c] = vr;
+    si[r][c] = vi;
+    rr[r][c] = 0.0;
+    ri[r][c] = 0.0;
+  }
+  __syncthreads();
+  ztri_inv64(sr, si, rr, ri, pr, pi, t);
   for (int idx = t; idx < kb * kb; idx += blockDim.x) {
     const int r = idx % kb, c = idx / kb;
     const size_t o = (size_t)(j0 + r) + (size_t)n * (j0 + c);
@@ -786,41 +836,48 @@ __global__ void k_zvpanel(int n, int p0, int nb, const double *__restrict__ ar, 
   vi[(size_t)r + (size_t)n * k] = xi;
 }
 
-// T of I - V T V^H from G = V^H V (ZLARFT 'F','C'): T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i),
-// nb <= BN <= 64 (one thread per row)
-template <int BN>
-__global__ __launch_bounds__(64) void k_zlarft(int nb, const double *__restrict__ gr, const double *__restrict__ gi,
-                                               const double *__restrict__ taur, const double *__restrict__ taui,
-                                               double *__restrict__ tmr, double *__restrict__ tmi)
+// T of I - V T V^H from G = V^H V (ZLARFT 'F','C'), whose recurrence
+// T(0:i,i) = -tau_i T(0:i,0:i) G(0:i,i) says T = M^-1 with M upper
+// triangular, M(i,i) = 1/tau_i and M(q,i) = G(q,i) above the diagonal.  A
+// reflector with tau_i = 0 is the identity: its row and column of M are
+// cleared (M(i,i) = 1) and T(i,i) is set to 0 afterwards.  nb <= 64.
+__global__ __launch_bounds__(256) void k_zlarft(int nb, const double *__restrict__ gr, const double *__restrict__ gi,
+                                                const double *__restrict__ taur, const double *__restrict__ taui,
+                                                double *__restrict__ tmr, double *__restrict__ tmi)
 {
-  static_assert(BN <= 64, "one thread per row");
-  __shared__ double gsr[BN][BN + 1], gsi[BN][BN + 1], tsr[BN][BN + 1], tsi[BN][BN + 1];
+  __shared__ double sr[NB][NB + 1], si[NB][NB + 1], rr[NB][NB + 1], ri[NB][NB + 1];
+  __shared__ double pr[NB / 2][NB / 2 + 1], pi[NB / 2][NB / 2 + 1];
+  __shared__ int zero[NB];
   const int t = threadIdx.x;
-  for (int idx = t; idx < nb * nb; idx += blockDim.x) {
-    const int r = idx % nb, c = idx / nb;
-    gsr[r][c] = gr[r + nb * c];
-    gsi[r][c] = gi[r + nb * c];
-    tsr[r][c] = 0.0;
-    tsi[r][c] = 0.0;
+  if (t < NB) zero[t] = !(t < nb && (taur[t] != 0.0 || taui[t] != 0.0));
+  __syncthreads();
+  for (int idx = t; idx < NB * NB; idx += blockDim.x) {
+    const int r = idx % NB, c = idx / NB;
+    double vr = 0.0, vi = 0.0;
+    if (r == c) {
+      if (zero[r]) {
+        vr = 1.0;
+      } else {
+        const double a = taur[r], b = taui[r], den = a * a + b * b;
+        vr = a / den;
+        vi = -b / den;
+      }
+    } else if (r < c && !zero[r] && !zero[c]) {
+      vr = gr[r + nb * c];
+      vi = gi[r + nb * c];
+    }
+    sr[r][c] = vr;
+    si[r][c] = vi;
+    rr[r][c] = 0.0;
+    ri[r][c] = 0.0;
   }
   __syncthreads();
-  for (int i = 0; i < nb; i++) {
-    const double a = taur[i], b = taui[i];
-    if (t < i) {
-      double sr = 0.0, si = 0.0;
-      for (int q = t; q < i; q++) cmad(tsr[t][q], tsi[t][q], gsr[q][i], gsi[q][i], sr, si);
-      tsr[t][i] = -(a * sr - b * si);
-      tsi[t][i] = -(a * si + b * sr);
-    } else if (t == i) {
-      tsr[i][i] = a;
-      tsi[i][i] = b;
-    }
-    __syncthreads();
-  }
+  ztri_inv64(sr, si, rr, ri, pr, pi, t);
   for (int idx = t; idx < nb * nb; idx += blockDim.x) {
     const int r = idx % nb, c = idx / nb;
-    tmr[r + nb * c] = tsr[r][c];
-    tmi[r + nb * c] = tsi[r][c];
+    const bool keep = r <= c && !(r == c && zero[r]);
+    tmr[r + nb * c] = keep ? rr[r][c] : 0.0;
+    tmi[r + nb * c] = keep ? ri[r][c] : 0.0;
   }
 }
 
@@ -1062,7 +1119,7 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
       if ((r2 = zgemm(sa, 'C', 'N', nb, nb, nr, 1.0, vr[u] + o, vi[u] + o, n, vr[u] + o, vi[u] + o, n, 0.0, gmr, gmi,
                       nb, 0, 0, 0, gws, GWS)))
         return r2;
-      hipLaunchKernelGGL(k_zlarft<ZBT>, dim3(1), dim3(64), 0, sa, nb, gmr, gmi, g.taur + p0, g.taui + p0, tr[u],
+      hipLaunchKernelGGL(k_zlarft, dim3(1), dim3(256), 0, sa, nb, gmr, gmi, g.taur + p0, g.taui + p0, tr[u],
                          ti[u]);
       W2K_STAGE(sa, "k_zlarft");
       W2K_HIP(hipEventRecord(c.bt_ready[u], sa));
