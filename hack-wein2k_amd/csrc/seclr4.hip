@@ -314,15 +314,15 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
       g.P[(size_t)n * (2 * nb + ii) + R] = vr;
     }
     if (ii > 0) {
-      // t[k] = X(R,k) v(R) for k < ii, summed over the lanes: lane q < TNB
-      // ends with the sum of column q
+      // t[k] = X(R,k) v(R) for k < ii, summed over the lanes: lanes 2q and
+      // 2q + 1 end with the sum of column q
 #pragma unroll
       for (int k = 0; k < TNB; k++) {
         const bool use = live && k < ii;
         t[k] = use ? t[k] * vr : 0.0;
       }
-      const double dq = wave_transpose_sum32(t, lane);
-      if (lane < TNB) g.dots[(size_t)I * 2 * TNB + half * TNB + lane] = dq;
+      const double dq = wave_transpose_sum32x(t, lane);  // lanes 2k, 2k + 1: column k
+      if (!(lane & 1)) g.dots[(size_t)I * 2 * TNB + half * TNB + (lane >> 1)] = dq;
     }
     return;
   }

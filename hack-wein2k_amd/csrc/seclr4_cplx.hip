@@ -410,6 +410,7 @@ __device__ __forceinline__ double dbl_readlane(double v, int src)
 // columns of the tile taken per step of the SYMV's tile loop (8-column steps
 // at four or five waves per SIMD measured slower on the 100-atom cell)
 constexpr int ZQC = 16;
+static_assert(ZQC == 16, "the column sums take 32 entries per step");
 
 __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
 {
@@ -458,8 +459,8 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
         const bool use = live && k < ii;
         t[k] = use ? (im ? ur * vri - ui * vrr : ur * vrr + ui * vri) : 0.0;
       }
-      const double sq = wave_transpose_sum32(t, lane);  // lane l < 32: column l
-      if (lane < TNB) (im ? di : dr)[half * TNB + lane] = sq;
+      const double sq = wave_transpose_sum32x(t, lane);  // lanes 2k, 2k + 1: column k
+      if (!(lane & 1)) (im ? di : dr)[half * TNB + (lane >> 1)] = sq;
     }
     return;
   }
@@ -501,9 +502,16 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
         q[ZQC + j] = ok ? q[ZQC + j] : 0.0;
       }
     }
+    {
+      // four independent accumulator pairs (a chain of 8 multiply-adds
+      // instead of 32 per quarter), added in a fixed order
+      double ar[4] = {0.0, 0.0, 0.0, 0.0}, ai[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int j = 0; j < ZQC; j++)
-      cmad(q[j], q[ZQC + j], dbl_readlane(vcr, c0 + j), dbl_readlane(vci, c0 + j), yr, yi);
+      for (int j = 0; j < ZQC; j++)
+        cmad(q[j], q[ZQC + j], dbl_readlane(vcr, c0 + j), dbl_readlane(vci, c0 + j), ar[j & 3], ai[j & 3]);
+      yr += (ar[0] + ar[1]) + (ar[2] + ar[3]);
+      yi += (ai[0] + ai[1]) + (ai[2] + ai[3]);
+    }
 #pragma unroll
     for (int j = 0; j < ZQC; j++) {  // conj(A(R,c)) v(R); the diagonal tile's diagonal only by rows
       const bool use = off || c0 + j < lane;
@@ -511,20 +519,20 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
       q[j] = use ? ar * vrr + ai * vri : 0.0;
       q[ZQC + j] = use ? ar * vri - ai * vrr : 0.0;
     }
-    // lane l: entry l % (2 ZQC) summed over the wave (Re of column l % ZQC
-    // in the lower half of each group of 2 ZQC lanes, Im in the upper half)
-    const double cs = ZQC == 16 ? wave_transpose_sum32(q, lane) : wave_transpose_sum16(q, lane);
+    // lanes 2e and 2e + 1: entry e summed over the wave (Re of column e in
+    // lanes 0..31, Im of column e - 16 in lanes 32..63)
+    const double cs = wave_transpose_sum32x(q, lane);
     if (off) {
-      const double co = __shfl_xor(cs, ZQC, 64);
-      const int cl = c0 + (lane & (ZQC - 1));
-      if (lane < 2 * ZQC && cl < m) ((lane & ZQC) ? g.parti : g.partr)[(size_t)I * n + C0 + cl] = cs;
+      const double co = xor32(cs, lane);  // the other part of the same column
+      const int cl = c0 + ((lane >> 1) & (ZQC - 1));
+      if (!(lane & 1) && cl < m) ((lane & 32) ? g.parti : g.partr)[(size_t)I * n + C0 + cl] = cs;
       const double vcl_r = __shfl(vcr, cl, 64), vcl_i = __shfl(vci, cl, 64);
-      if (lane < ZQC && cl < m) vyc += vcl_r * cs + vcl_i * co;  // Re(conj(v_c) (A^H v)_c)
+      if (lane < 32 && !(lane & 1) && cl < m) vyc += vcl_r * cs + vcl_i * co;  // Re(conj(v_c) (A^H v)_c)
     } else {
       // column c0 + j of the tile is row C0 + c0 + j: lane c0 + j takes it
       const int src = lane - c0;
       const bool mine = src >= 0 && src < ZQC;
-      const double fr = __shfl(cs, mine ? src : 0, 64), fi = __shfl(cs, mine ? src + ZQC : ZQC, 64);
+      const double fr = __shfl(cs, mine ? 2 * src : 0, 64), fi = __shfl(cs, mine ? 32 + 2 * src : 32, 64);
       if (mine) {
         yr += fr;
         yi += fi;

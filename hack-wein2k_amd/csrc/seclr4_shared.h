@@ -96,6 +96,75 @@ __device__ __forceinline__ double wave_transpose_sum16(double *t, int lane)
   return s + __shfl_xor(s, 32, 64);
 }
 
+// One halving step with the lane bit S of a permlane swap (S = 16: gfx950
+// v_permlane16_swap, S = 32: v_permlane32_swap): lanes with bit S clear keep
+// entry x summed with their partner's x, the others keep y summed with their
+// partner's y.  Swapping x and y across the two lane groups leaves (x,
+// partner's x) in the first group and (partner's y, y) in the second, so the
+// sum of the two swapped registers is the step's result in every lane: no
+// selects and no LDS.
+template <int S>
+__device__ __forceinline__ double swap_sum(double x, double y)
+{
+  const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+  const unsigned xl = (unsigned)(bx & 0xffffffffll), xh = (unsigned)((unsigned long long)bx >> 32);
+  const unsigned yl = (unsigned)(by & 0xffffffffll), yh = (unsigned)((unsigned long long)by >> 32);
+  unsigned a0, a1, b0, b1;
+  if constexpr (S == 16) {
+    const auto a = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+    a0 = a[0]; a1 = a[1]; b0 = b[0]; b1 = b[1];
+  } else {
+    static_assert(S == 32, "lane bit 4 or 5");
+    const auto a = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+    a0 = a[0]; a1 = a[1]; b0 = b[0]; b1 = b[1];
+  }
+  const double u = __longlong_as_double((long long)(((unsigned long long)b0 << 32) | a0));
+  const double v = __longlong_as_double((long long)(((unsigned long long)b1 << 32) | a1));
+  return u + v;
+}
+
+// v of the lane that differs in bit 5
+__device__ __forceinline__ double xor32(double v, int lane)
+{
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)((unsigned long long)b >> 32);
+  const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto c = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  const bool up = (lane & 32) != 0;
+  const unsigned rl = up ? a[0] : a[1], rh = up ? c[0] : c[1];
+  return __longlong_as_double((long long)(((unsigned long long)rh << 32) | rl));
+}
+
+// halving step on lane bit LB over entries j, j + H (j < H): DPP partner
+template <int LB, int H>
+__device__ __forceinline__ void bfly_step_h(double *t, int lane)
+{
+  const bool hi = (lane & LB) != 0;
+#pragma unroll
+  for (int j = 0; j < H; j++) {
+    const double keep = hi ? t[j + H] : t[j];
+    const double send = hi ? t[j] : t[j + H];
+    t[j] = keep + bfly_partner<LB>(send);
+  }
+}
+
+// t[0..31] per lane -> returns, in lanes 2e and 2e + 1, the sum over the 64
+// lanes of t[e]: halving steps on lane bits 5 and 4 by permlane swaps, then
+// 3, 2, 1 by DPP, and the pair added.  Overwrites t.
+__device__ __forceinline__ double wave_transpose_sum32x(double *t, int lane)
+{
+#pragma unroll
+  for (int j = 0; j < 16; j++) t[j] = swap_sum<32>(t[j], t[j + 16]);
+#pragma unroll
+  for (int j = 0; j < 8; j++) t[j] = swap_sum<16>(t[j], t[j + 8]);
+  bfly_step_h<8, 4>(t, lane);
+  bfly_step_h<4, 2>(t, lane);
+  bfly_step_h<2, 1>(t, lane);
+  return t[0] + bfly_partner<1>(t[0]);
+}
+
 // device work buffers of one SECLR4 call (carved from Ctx::w_small / w_int);
 // complex quantities are planar: real parts, then imaginary parts at +stride
 struct Seclr4Bufs {

@@ -370,6 +370,22 @@ static uint64_t wave_op(int op, const uint64_t *in, int nin, uint64_t *out4 = nu
   return f->out[0];
 }
 
+u2 permlane_swap(unsigned x, unsigned y, int s)
+{
+  const int lane = cur->lane;
+  const unsigned px = (unsigned)wave_shfl((uint64_t)x, lane ^ s);
+  const unsigned py = (unsigned)wave_shfl((uint64_t)y, lane ^ s);
+  u2 r;
+  if (lane & s) {
+    r[0] = py;  // the first register's upper group receives the second's lower group
+    r[1] = y;
+  } else {
+    r[0] = x;
+    r[1] = px;  // the second register's lower group receives the first's upper group
+  }
+  return r;
+}
+
 uint64_t wave_shfl(uint64_t v, int src)
 {
   const uint64_t in[2] = {v, (uint64_t)(uint32_t)src};

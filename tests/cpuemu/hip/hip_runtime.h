@@ -150,6 +150,8 @@ int wave_readlane(int v, int src);
 int wave_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl);
 int wave_swizzle(int v, int pattern);
 d4 wave_mfma_f64(double a, double b, d4 c);
+typedef unsigned u2 __attribute__((ext_vector_type(2)));
+u2 permlane_swap(unsigned x, unsigned y, int s);
 v4i wave_mfma_i8(v4i a, v4i b, v4i c);
 
 template <class T>
@@ -217,6 +219,10 @@ inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 
 #define __builtin_amdgcn_readlane(v, l) emu::wave_readlane((v), (l))
+// gfx950 permlane swaps: lanes with bit S set take the other register's value
+// of their partner (lane ^ S) into the first result, the others into the second
+#define __builtin_amdgcn_permlane16_swap(x, y, fi, bc) emu::permlane_swap((x), (y), 16)
+#define __builtin_amdgcn_permlane32_swap(x, y, fi, bc) emu::permlane_swap((x), (y), 32)
 #define __builtin_amdgcn_readfirstlane(v) ((int)(v))
 #define __builtin_amdgcn_update_dpp(o, v, c, rm, bm, bc) emu::wave_dpp((o), (v), (c), (rm), (bm), (bc))
 #define __builtin_amdgcn_ds_swizzle(v, p) emu::wave_swizzle((v), (p))
