@@ -396,9 +396,11 @@ __device__ __forceinline__ void ztri_reflector(const ZTriWork &g, int i, int lan
 // taken in four quarters of 16 columns: the real and imaginary parts of a
 // quarter-row fill 32 of the lane's registers, row sums stay in the lane and
 // column sums use a wave-wide reduce-scatter (no LDS); v(C0 + j) reaches the
-// row sums from lane j by readlane.  The nt waves after the tiles each
-// take one row block: they write V(:,k) and the partial dot products V^H v,
-// W^H v of the block's rows.
+// row sums from lane j by readlane.  The first quarter's loads are issued
+// before the reflector is formed and each further quarter's as soon as the
+// previous one has been reduced, so a tile wave always has a quarter in
+// flight.  The 4 nt waves after the tiles, four per row block, write V(:,k)
+// and the partial dot products V^H v, W^H v of the block's rows.
 __device__ __forceinline__ double dbl_readlane(double v, int src)
 {
   const long long b = __double_as_longlong(v);
@@ -419,6 +421,29 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
   const int nblk = nt * (nt + 1) / 2;
   const int bid = blockIdx.x;
   const double *air = g.ar + (size_t)n * i, *aii = g.ai + (size_t)n * i;
+  // tile waves: the tile (I, J), J <= I, and its first 16 columns, loaded
+  // before the reflector is formed so that their latency overlaps it
+  const bool tile = bid < nblk;
+  int I = 0, J = 0;
+  if (tile) {
+    I = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
+    while ((I + 1) * (I + 2) / 2 <= bid) I++;
+    while (I * (I + 1) / 2 > bid) I--;
+    J = bid - I * (I + 1) / 2;
+  }
+  const int C0t = r0 + SYMV_T * J;
+  const int mt = min(SYMV_T, n - C0t);
+  const int Rlt = min(r0 + SYMV_T * I + lane, n - 1);
+  const double *apr = g.ar + (size_t)Rlt + (size_t)n * C0t, *api = g.ai + (size_t)Rlt + (size_t)n * C0t;
+  double q[2 * ZQC];  // q[j] = Re A(R, C0 + c0 + j), q[ZQC + j] = Im
+  if (tile) {
+#pragma unroll
+    for (int j = 0; j < ZQC; j++) {
+      const int jj = min(j, mt - 1);
+      q[j] = apr[(size_t)n * jj];
+      q[ZQC + j] = api[(size_t)n * jj];
+    }
+  }
   double sr, si;
   ztri_reflector(g, i, lane, bid == 0, sr, si);
   // v(x): 1 at r0, A(x, i) (sr + i si) below it, 0 outside [r0, n)
@@ -464,21 +489,15 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
     }
     return;
   }
-  int I = (int)((sqrt(8.0 * (double)bid + 1.0) - 1.0) * 0.5);
-  while ((I + 1) * (I + 2) / 2 <= bid) I++;
-  while (I * (I + 1) / 2 > bid) I--;
-  const int J = bid - I * (I + 1) / 2;
   const int R = r0 + SYMV_T * I + lane;
   const int C0 = r0 + SYMV_T * J;
   const bool live = R < n;
   const bool off = (I != J);
   const int m = min(SYMV_T, n - C0);
-  const int Rl = min(R, n - 1);
   double vcr, vci, vrr, vri;
   vof(C0 + lane, vcr, vci);  // v of the lane's column
   vof(R, vrr, vri);          // v of the lane's row
   double yr = 0.0, yi = 0.0, vyc = 0.0;
-  const double *apr = g.ar + (size_t)Rl + (size_t)n * C0, *api = g.ai + (size_t)Rl + (size_t)n * C0;
   // a full off-diagonal tile (all 64 rows and columns inside the matrix)
   // needs no masks; wave-uniform
   const bool full = off && m == SYMV_T && r0 + SYMV_T * (I + 1) <= n;
@@ -486,14 +505,7 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
   // diagonal tile uses its lower part by rows and its strict lower part by
   // columns (A^H v), the column sums folded into the rows they belong to
 #pragma unroll 1
-  for (int c0 = 0; c0 < m; c0 += ZQC) {  // steps of ZQC columns (wave-uniform)
-    double q[2 * ZQC];  // q[j] = Re A(R, C0 + c0 + j), q[ZQC + j] = Im
-#pragma unroll
-    for (int j = 0; j < ZQC; j++) {
-      const int jj = min(c0 + j, m - 1);
-      q[j] = apr[(size_t)n * jj];
-      q[ZQC + j] = api[(size_t)n * jj];
-    }
+  for (int c0 = 0; c0 < m; c0 += ZQC) {  // steps of ZQC columns (wave-uniform); q holds step c0
     if (!full) {
 #pragma unroll
       for (int j = 0; j < ZQC; j++) {
@@ -522,6 +534,14 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
     // lanes 2e and 2e + 1: entry e summed over the wave (Re of column e in
     // lanes 0..31, Im of column e - 16 in lanes 32..63)
     const double cs = wave_transpose_sum32x(q, lane);
+    if (c0 + ZQC < m) {  // the next step's columns (q is free again)
+#pragma unroll
+      for (int j = 0; j < ZQC; j++) {
+        const int jj = min(c0 + ZQC + j, m - 1);
+        q[j] = apr[(size_t)n * jj];
+        q[ZQC + j] = api[(size_t)n * jj];
+      }
+    }
     if (off) {
       const double co = xor32(cs, lane);  // the other part of the same column
       const int cl = c0 + ((lane >> 1) & (ZQC - 1));
