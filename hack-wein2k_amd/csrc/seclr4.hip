@@ -1280,7 +1280,9 @@ int seclr4_buffers(Ctx &c, int n, int cplx, Seclr4Bufs &B)
   const size_t z = cplx ? 2 : 1;
   const size_t nts = (size_t)nt;
   const size_t nnorm = nts + (size_t)n / COL_ROWS + 64;  // partial norms per block of 64 (complex) or 32 rows
-  const size_t small = (size_t)12 * n + z * 2 * n + z * 3 * (size_t)n * TNB + z * nts * n + z * nts * 2 * TNB +
+  // SYMV partials: nt x n (real); 256 per tile of the lower triangle (complex)
+  const size_t npart = cplx ? std::max(2 * nts * n, (size_t)128 * nts * (nts + 1)) : nts * n;
+  const size_t small = (size_t)12 * n + z * 2 * n + z * 3 * (size_t)n * TNB + npart + z * nts * 2 * TNB +
                        nts * (nts + 1) / 2 + 2 + nnorm + 64 + 32 * 24;
   if ((rc = c.w_small.alloc(small))) return rc;
   if ((rc = c.w_int.alloc((size_t)6 * n + 32))) return rc;
@@ -1297,7 +1299,7 @@ int seclr4_buffers(Ctx &c, int n, int cplx, Seclr4Bufs &B)
   B.tau = take(z * n); B.scal = take(z * n);
   B.wv = take(n);
   B.P = take(z * 3 * (size_t)n * TNB);
-  B.part = take(z * nts * n);
+  B.part = take(npart);
   B.tdots = take(z * nts * 2 * TNB);
   B.tvy = take(nts * (nts + 1) / 2 + 2);
   B.tnorm = take(nnorm);

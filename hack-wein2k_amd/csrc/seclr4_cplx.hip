@@ -340,7 +340,7 @@ struct ZTriWork {
   double *d, *e;            // real tridiagonal
   double *taur, *taui;      // reflector scalars
   double *scr, *sci;        // 1 / (alpha - beta)
-  double *partr, *parti;    // SYMV partials (tiles x n)
+  double *part;             // SYMV partials: 256 per tile (zpart_off)
   double *dotr, *doti;      // tiles x 2 TNB partial V^H v, W^H v
   double *vy;               // partial Re(v^H y) per tile pair
   double *norm;             // per block partial |x|^2
@@ -401,6 +401,14 @@ __device__ __forceinline__ void ztri_reflector(const ZTriWork &g, int i, int lan
 // previous one has been reduced, so a tile wave always has a quarter in
 // flight.  The 4 nt waves after the tiles, four per row block, write V(:,k)
 // and the partial dot products V^H v, W^H v of the block's rows.
+//
+// Each tile wave writes its partial sums into a block of its own, 256
+// doubles at bid * 256: the row sums (Re, Im: 64 each), then the column sums
+// (Re, Im).  Consecutive waves so write consecutive memory.  The writes are
+// about 3% of the kernel's bytes but cost far more inside its read stream: on
+// the order-12445 trailing matrix the kernel takes 222 us without them, 268
+// us with the sums laid out as tiles x n vectors (every 512-byte segment in
+// another place) and 251 us with this layout.
 __device__ __forceinline__ double dbl_readlane(double v, int src)
 {
   const long long b = __double_as_longlong(v);
@@ -545,7 +553,7 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
     if (off) {
       const double co = xor32(cs, lane);  // the other part of the same column
       const int cl = c0 + ((lane >> 1) & (ZQC - 1));
-      if (!(lane & 1) && cl < m) ((lane & 32) ? g.parti : g.partr)[(size_t)I * n + C0 + cl] = cs;
+      if (!(lane & 1) && cl < m) g.part[(size_t)bid * 256 + 128 + ((lane & 32) ? 64 : 0) + cl] = cs;
       const double vcl_r = __shfl(vcr, cl, 64), vcl_i = __shfl(vci, cl, 64);
       if (lane < 32 && !(lane & 1) && cl < m) vyc += vcl_r * cs + vcl_i * co;  // Re(conj(v_c) (A^H v)_c)
     } else {
@@ -560,8 +568,8 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
     }
   }
   if (live) {
-    g.partr[(size_t)J * n + R] = yr;
-    g.parti[(size_t)J * n + R] = yi;
+    g.part[(size_t)bid * 256 + lane] = yr;
+    g.part[(size_t)bid * 256 + 64 + lane] = yi;
   }
   double vy = vrr * yr + vri * yi + vyc;  // Re(conj(v) y)
   vy = wave_sum(vy);
@@ -578,6 +586,16 @@ __global__ __launch_bounds__(64, 3) void k_ztri_symv(ZTriWork g, int i, int nt)
 // and their row sums meet in LDS.  Loads are issued in batches with clamped
 // indices and masked values, so that a batch is in flight at once.
 constexpr int ZCOL_WAVES = 8;
+static_assert(ZCOL_ROWS == SYMV_T, "a column-kernel block reads the partials of one SYMV row block");
+
+// offset of partial c (c < nt) of the rows of SYMV row block I, Re (Im at
+// +64): the row sums of tile (I, c) for c <= I, else the column sums of tile
+// (c, I); the lane's row at + lane
+__device__ __forceinline__ size_t zpart_off(int I, int c)
+{
+  return c <= I ? (size_t)(I * (I + 1) / 2 + c) * 256 : (size_t)(c * (c + 1) / 2 + I) * 256 + 128;
+}
+
 __global__ __launch_bounds__(64 * ZCOL_WAVES) void k_ztri_col(ZTriWork g, int fin, int col, int ntile, int nch,
                                                               int nvy)
 {
@@ -634,9 +652,10 @@ __global__ __launch_bounds__(64 * ZCOL_WAVES) void k_ztri_col(ZTriWork g, int fi
     // row col's SYMV partials (for W(col, ii))
     if (col >= 0) {
       double pr = 0.0, pi = 0.0;
-      for (int c = t; c < nch; c += NT) {
-        pr += g.partr[(size_t)c * n + col];
-        pi += g.parti[(size_t)c * n + col];
+      for (int c = t; c < nch; c += NT) {  // row col is row 0 of row block 0
+        const size_t o = zpart_off(0, c);
+        pr += g.part[o];
+        pi += g.part[o + 64];
       }
       pr = wave_sum(pr);
       pi = wave_sum(pi);
@@ -697,13 +716,13 @@ __global__ __launch_bounds__(64 * ZCOL_WAVES) void k_ztri_col(ZTriWork g, int fi
       double xr[YB], xi[YB];
 #pragma unroll
       for (int u = 0; u < YB; u++) {
-        const size_t o = (size_t)min(c0 + ZCOL_WAVES * u, nch - 1) * n + rs;
-        xr[u] = g.partr[o];
-        xi[u] = g.parti[o];
+        const size_t o = zpart_off(blockIdx.x, min(c0 + ZCOL_WAVES * u, nch - 1)) + lane;
+        xr[u] = g.part[o];
+        xi[u] = g.part[o + 64];
       }
 #pragma unroll
-      for (int u = 0; u < YB; u++) {
-        const bool k = c0 + ZCOL_WAVES * u < nch;
+      for (int u = 0; u < YB; u++) {  // rows past n: never written, selected away
+        const bool k = live && c0 + ZCOL_WAVES * u < nch;
         yr += k ? xr[u] : 0.0;
         yi += k ? xi[u] : 0.0;
       }
@@ -1025,7 +1044,7 @@ int seclr4_complex(Ctx &c, double el, double eu, int nume, int *ne_out, int *nbe
   g.d = B.d; g.e = B.e;
   g.taur = B.tau; g.taui = B.tau + n;
   g.scr = B.scal; g.sci = B.scal + n;
-  g.partr = B.part; g.parti = B.part + (size_t)nts * n;
+  g.part = B.part;
   g.dotr = B.tdots; g.doti = B.tdots + (size_t)nts * 2 * TNB;
   g.vy = B.tvy; g.norm = B.tnorm; g.counter = B.ticket;
   // optional per-launch events around the SYMV (kernel-level roofline)
