@@ -281,8 +281,12 @@ __device__ __forceinline__ double tri_reflector(const TriWork &g, int i, int lan
 // wave-wide reduce-scatter, no LDS); a diagonal tile adds its lower part by
 // rows and its strict lower part by columns.  Row r of tile R receives
 // exactly one partial per tile index: from (R, J) at slot J <= R and from
-// (I, R) at slot I > R, so y(r) = sum over slots of part[slot][r] in a fixed
-// order.  Every tile wave writes its partial v^T y.  The nt waves after the
+// (I, R) at slot I > R, so y(r) = sum over slots in a fixed order.  A tile
+// wave writes its partials into a block of its own, 128 doubles at bid * 128
+// (row sums, then column sums), so that consecutive waves write consecutive
+// memory (tiles x n vectors put every 512-byte segment in another place,
+// which costs the kernel's read stream more HBM time).  Every tile wave
+// writes its partial v^T y.  The nt waves after the
 // tiles each take one row block I: they write V(:,k) (v = A(:,i) * scal,
 // v(i+1) = 1) and the partial dot products V^T v, W^T v of the block's rows.
 __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
@@ -384,13 +388,13 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
     for (int j = 0; j < SYMV_T; j++) t[j] = (j < lane) ? t[j] * vr : 0.0;
     yrow += wave_transpose_sum(t, lane);  // column C0 + lane = row R of the strict lower part
   }
-  if (live) g.part[(size_t)J * n + R] = yrow;
+  if (live) g.part[(size_t)bid * 128 + lane] = yrow;
   // v^T y over the tile: an off-diagonal tile adds v_I^T A_IJ v_J through its
   // rows and the same number again through its columns
   double vy = wave_sum(yrow * vr);
   if (lane == 0) g.vy[bid] = off ? 2.0 * vy : vy;
   if (off) {
-    double *pc = g.part + (size_t)I * n + c;
+    double *pc = g.part + (size_t)bid * 128 + 64 + lane;
     const bool st = lane < m && c >= r0;
 #pragma unroll
     for (int j = 0; j < SYMV_T; j++) t[j] *= vr;
@@ -408,6 +412,14 @@ __global__ __launch_bounds__(64, 3) void k_tri_symv(TriWork g, int i, int nt)
 // SYMV partials (chunk w, w + 8, ...) and the panel columns (four each) and
 // their row sums meet in LDS.  Loads are issued in batches with clamped
 // indices and masked values, so that a batch is in flight at once.
+// offset of partial c of the rows of SYMV row block I (the lane's place in
+// the block at + l): the row sums of tile (I, c) for c <= I, else the column
+// sums of tile (c, I)
+__device__ __forceinline__ size_t tpart_off(int I, int c)
+{
+  return c <= I ? (size_t)(I * (I + 1) / 2 + c) * 128 : (size_t)(c * (c + 1) / 2 + I) * 128 + 64;
+}
+
 __global__ __launch_bounds__(64 * COL_WAVES) void k_tri_col(TriWork g, int fin, int col, int ntile, int nvy)
 {
   constexpr int NT = 64 * COL_WAVES;   // threads
@@ -458,7 +470,8 @@ __global__ __launch_bounds__(64 * COL_WAVES) void k_tri_col(TriWork g, int fin, 
     }
     if (col >= 0) {
       double p = 0.0;
-      for (int c = t; c < nch; c += NT) p += g.part[(size_t)c * n + col];
+      const int cl = col - (rbase & ~15);  // row col in row block 0 of the SYMV's grid
+      for (int c = t; c < nch; c += NT) p += g.part[tpart_off(0, c) + cl];
       p = wave_sum(p);
       if (lane == 0) sh[1][w] = p;
     }
@@ -495,12 +508,14 @@ __global__ __launch_bounds__(64 * COL_WAVES) void k_tri_col(TriWork g, int fin, 
   }
   double y = 0.0;
   if (fin >= 0) {
+    // the row's block and place in the SYMV's tile grid (origin rbase & ~15)
+    const int rg = rs - (rbase & ~15), I = rg >> 6, l = rg & 63;
     for (int c0 = w; c0 < nch; c0 += COL_WAVES * YB) {
       double xp[YB];
 #pragma unroll
-      for (int u = 0; u < YB; u++) xp[u] = g.part[(size_t)min(c0 + COL_WAVES * u, nch - 1) * n + rs];
+      for (int u = 0; u < YB; u++) xp[u] = g.part[tpart_off(I, min(c0 + COL_WAVES * u, nch - 1)) + l];
 #pragma unroll
-      for (int u = 0; u < YB; u++) y += (c0 + COL_WAVES * u < nch) ? xp[u] : 0.0;
+      for (int u = 0; u < YB; u++) y += (live && c0 + COL_WAVES * u < nch) ? xp[u] : 0.0;  // rows past n: never written
     }
   }
   __syncthreads();
@@ -1280,8 +1295,9 @@ int seclr4_buffers(Ctx &c, int n, int cplx, Seclr4Bufs &B)
   const size_t z = cplx ? 2 : 1;
   const size_t nts = (size_t)nt;
   const size_t nnorm = nts + (size_t)n / COL_ROWS + 64;  // partial norms per block of 64 (complex) or 32 rows
-  // SYMV partials: nt x n (real); 256 per tile of the lower triangle (complex)
-  const size_t npart = cplx ? std::max(2 * nts * n, (size_t)128 * nts * (nts + 1)) : nts * n;
+  // SYMV partials: 128 (real) or 256 (complex) per tile of the lower triangle
+  const size_t npart = cplx ? std::max(2 * nts * n, (size_t)128 * nts * (nts + 1))
+                            : std::max(nts * n, (size_t)64 * nts * (nts + 1));
   const size_t small = (size_t)12 * n + z * 2 * n + z * 3 * (size_t)n * TNB + npart + z * nts * 2 * TNB +
                        nts * (nts + 1) / 2 + 2 + nnorm + 64 + 32 * 24;
   if ((rc = c.w_small.alloc(small))) return rc;

@@ -174,7 +174,7 @@ struct Seclr4Bufs {
   double *tau, *scal;       // reflector scalars (complex: re[n], im[n])
   double *wv;               // eigenvalues (block order, then sorted)
   double *P;                // panel buffer n x 3 TNB (complex: twice)
-  double *part;             // SYMV partials nt x n (complex: 256 per tile of the lower triangle)
+  double *part;             // SYMV partials: 128 (complex 256) per tile of the lower triangle
   double *tdots;            // nt x 2 TNB (complex: twice)
   double *tvy;              // nt (nt + 1) / 2
   double *tnorm;            // nt

@@ -555,7 +555,7 @@ static size_t noalloc_bytes(size_t n, size_t m, int cplx)
   size_t b = 0;
   b += cplx ? 4 * r(2 * n * n) : 3 * r(n * n) + r(lda * n);                  // U, H/C, W, X
   b += r((size_t)1 << 22);                                                   // split-K partials
-  const size_t npart = cplx ? std::max(2 * nt * n, 128 * nt * (nt + 1)) : nt * n;  // SYMV partials
+  const size_t npart = cplx ? std::max(2 * nt * n, 128 * nt * (nt + 1)) : std::max(nt * n, 64 * nt * (nt + 1));
   b += r(12 * n + z * 2 * n + z * 3 * n * 32 + npart + z * nt * 64 + nt * (nt + 1) / 2 + 2 + nt + n / 32 +
          64 + 64 + 768);                                                     // tridiagonal work
   b += r((6 * n + 32 + 1) / 2);                                              // ints
