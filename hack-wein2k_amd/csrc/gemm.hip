@@ -456,9 +456,145 @@ __global__ __launch_bounds__(256, 2) void k_zgemm(ZGemmArgs g)
   }
 }
 
+// The fused kernel over one slice of K per blockIdx.z (slice z: k in
+// [z kslice, (z + 1) kslice)), its raw sums into wsr/wsi[(z n + j) m + i];
+// k_zsplitk_reduce adds the slices and applies alpha, beta and C.  A kernel
+// of its own: folding the K range into k_zgemm changed that kernel's code
+// and cost the tridiagonalisation's rank-64 updates 24 ms per 100-atom
+// k-point.
+template <int TA, int TB>
+__global__ __launch_bounds__(256, 2) void k_zgemm_split(ZGemmArgs g, int kslice, double *__restrict__ wsr, double *__restrict__ wsi)
+{
+  __shared__ double Ar_s[2][BK][BM + PAD], Ai_s[2][BK][BM + PAD];
+  __shared__ double Br_s[2][BK][BN + PAD], Bi_s[2][BK][BN + PAD];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN;
+  if (g.tri == 1 && i0 + BM - 1 < j0) return;
+  if (g.tri == 2 && j0 + BN - 1 < i0) return;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const int kbeg = blockIdx.z * kslice, kend = min(g.k, kbeg + kslice);
+  d4 cr[2][2], ci[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) {
+      cr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+      ci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+  // this thread's four elements of each operand per slab (rows or k steps as
+  // in k_dgemm), loaded unconditionally from clamped addresses and masked
+  double rar[4], rai[4], rbr[4], rbi[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = TA ? i0 + (tid >> 4) + 16 * r : i0 + (tid & 63);
+      const int kk = TA ? k0 + (tid & 15) : k0 + (tid >> 6) + 4 * r;
+      const bool ok = i < g.m && kk < kend;
+      const int ic = min(i, g.m - 1), kc = min(kk, g.k - 1);
+      const size_t o = TA ? (size_t)kc + (size_t)g.lda * ic : (size_t)ic + (size_t)g.lda * kc;
+      const double vr = g.Ar[o], vi = g.Ai[o];
+      rar[r] = ok ? vr : 0.0;
+      rai[r] = ok ? g.sa * vi : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int j = TB ? j0 + (tid & 63) : j0 + (tid >> 4) + 16 * r;
+      const int kk = TB ? k0 + (tid >> 6) + 4 * r : k0 + (tid & 15);
+      const bool ok = j < g.n && kk < kend;
+      const int jc = min(j, g.n - 1), kc = min(kk, g.k - 1);
+      const size_t o = TB ? (size_t)jc + (size_t)g.ldb * kc : (size_t)kc + (size_t)g.ldb * jc;
+      const double vr = g.Br[o], vi = g.Bi[o];
+      rbr[r] = ok ? vr : 0.0;
+      rbi[r] = ok ? g.sb * vi : 0.0;
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int ka = TA ? (tid & 15) : (tid >> 6) + 4 * r, ia = TA ? (tid >> 4) + 16 * r : (tid & 63);
+      Ar_s[buf][ka][ia] = rar[r];
+      Ai_s[buf][ka][ia] = rai[r];
+      const int kb = TB ? (tid >> 6) + 4 * r : (tid & 15), jb = TB ? (tid & 63) : (tid >> 4) + 16 * r;
+      Br_s[buf][kb][jb] = rbr[r];
+      Bi_s[buf][kb][jb] = rbi[r];
+    }
+  };
+  if (kbeg < kend) {
+    gload(kbeg);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+      const bool more = (k0 + BK < kend);
+      if (more) gload(k0 + BK);
+#pragma unroll
+      for (int kq = 0; kq < BK; kq += 4) {
+        double ar[2], ai[2], br[2], bi[2], nbi[2];
+        for (int a = 0; a < 2; a++) {
+          ar[a] = Ar_s[buf][kq + lk][wi + 16 * a + li];
+          ai[a] = Ai_s[buf][kq + lk][wi + 16 * a + li];
+        }
+        for (int b = 0; b < 2; b++) {
+          br[b] = Br_s[buf][kq + lk][wj + 16 * b + li];
+          bi[b] = Bi_s[buf][kq + lk][wj + 16 * b + li];
+          nbi[b] = -bi[b];
+        }
+        for (int a = 0; a < 2; a++)
+          for (int b = 0; b < 2; b++) {
+            cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cr[a][b], 0, 0, 0);
+            cr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], nbi[b], cr[a][b], 0, 0, 0);
+            ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], ci[a][b], 0, 0, 0);
+            ci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], ci[a][b], 0, 0, 0);
+          }
+      }
+      if (more) {
+        sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+      }
+    }
+  }
+  // raw sums of this K slice (no triangle: the split needs a full output)
+  for (int a = 0; a < 2; a++) {
+    for (int b = 0; b < 2; b++) {
+      const int j = j0 + wj + 16 * b + li;
+      for (int r = 0; r < 4; r++) {
+        const int i = i0 + wi + 16 * a + lk + 4 * r;
+        if (i >= g.m || j >= g.n) continue;
+        const size_t o = ((size_t)blockIdx.z * g.n + j) * g.m + i;
+        wsr[o] = cr[a][b][r];
+        wsi[o] = ci[a][b][r];
+      }
+    }
+  }
+}
+
+// C = alpha sum_z ws[z] + beta C for both planes, fixed summation order
+__global__ void k_zsplitk_reduce(int m, int n, int nz, const double *__restrict__ wsr, const double *__restrict__ wsi,
+                                 double alpha, double beta, double *Cr, double *Ci, int ldc)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= m) return;
+  double vr = 0.0, vi = 0.0;
+  for (int z = 0; z < nz; z++) {
+    const size_t o = ((size_t)z * n + j) * m + i;
+    vr += wsr[o];
+    vi += wsi[o];
+  }
+  const size_t o = (size_t)i + (size_t)ldc * j;
+  vr *= alpha;
+  vi *= alpha;
+  if (beta != 0.0) {
+    vr += beta * Cr[o];
+    vi += beta * Ci[o];
+  }
+  Cr[o] = vr;
+  Ci[o] = vi;
+}
+
 int zgemm_fused(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *Ar, const double *Ai,
                 int lda, const double *Br, const double *Bi, int ldb, double beta, double *Cr, double *Ci, int ldc,
-                int tri)
+                int tri, double *ws, size_t wsn)
 {
   if (m <= 0 || n <= 0) return 0;
   ZGemmArgs g;
@@ -469,6 +605,29 @@ int zgemm_fused(hipStream_t s, char ta, char tb, int m, int n, int k, double alp
   g.tri = tri;
   const bool ttA = (ta == 'T' || ta == 'C'), ttB = (tb == 'T' || tb == 'C');
   dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
+  // few output tiles and a long K (the rule of dgemm's first split): K in
+  // slices, partial sums in ws (two planes of m x n per slice)
+  constexpr int ksl = W2K_ZSPLIT_KMIN / 2;
+  const long tiles = (long)grid.x * grid.y;
+  if (ws && tri == 0 && tiles < 256 && k >= W2K_ZSPLIT_KMIN) {
+    int nz = (int)std::min<long>((k + ksl - 1) / ksl, std::max<long>(1, 1024 / tiles));
+    nz = (int)std::min<size_t>((size_t)nz, wsn / ((size_t)2 * m * n));
+    if (nz >= 2) {
+      const int kslice = ((k + nz - 1) / nz + BK - 1) / BK * BK;
+      nz = (k + kslice - 1) / kslice;
+      double *wsr = ws, *wsi = ws + (size_t)nz * m * n;
+      grid.z = nz;
+      if (!ttA && !ttB) hipLaunchKernelGGL((k_zgemm_split<0, 0>), grid, dim3(256), 0, s, g, kslice, wsr, wsi);
+      else if (ttA && !ttB) hipLaunchKernelGGL((k_zgemm_split<1, 0>), grid, dim3(256), 0, s, g, kslice, wsr, wsi);
+      else if (!ttA && ttB) hipLaunchKernelGGL((k_zgemm_split<0, 1>), grid, dim3(256), 0, s, g, kslice, wsr, wsi);
+      else hipLaunchKernelGGL((k_zgemm_split<1, 1>), grid, dim3(256), 0, s, g, kslice, wsr, wsi);
+      W2K_STAGE(s, "k_zgemm_split");
+      hipLaunchKernelGGL(k_zsplitk_reduce, dim3((m + 255) / 256, n), dim3(256), 0, s, m, n, nz, wsr, wsi, alpha, beta,
+                         Cr, Ci, ldc);
+      W2K_STAGE(s, "k_zsplitk_reduce");
+      return 0;
+    }
+  }
   if (!ttA && !ttB) hipLaunchKernelGGL((k_zgemm<0, 0>), grid, dim3(256), 0, s, g);
   else if (ttA && !ttB) hipLaunchKernelGGL((k_zgemm<1, 0>), grid, dim3(256), 0, s, g);
   else if (!ttA && ttB) hipLaunchKernelGGL((k_zgemm<0, 1>), grid, dim3(256), 0, s, g);

@@ -35,6 +35,14 @@ static int zgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double al
   // (C read and written once); short K (rank updates) is where it pays
   if (atri == 0 && btri == 0 && ws == nullptr && k <= 256)
     return zgemm_fused(s, ta, tb, m, n, k, alpha, Ar, Ai, lda, Br, Bi, ldb, beta, Cr, Ci, ldc, tri);
+  // dense products with a long K onto few output tiles (the back-transform's
+  // V^H Z and Gram matrices): the fused kernel with split K reads each
+  // operand plane once and feeds 16 MFMAs from 8 LDS reads per step, against
+  // 4 from 4 in the real 64 x 64 kernel
+  const long ztiles = (long)((m + 63) / 64) * ((n + 63) / 64);
+  if (atri == 0 && btri == 0 && tri == 0 && ws != nullptr && ztiles < 256 && k >= W2K_ZSPLIT_KMIN &&
+      wsn >= (size_t)4 * m * n)
+    return zgemm_fused(s, ta, tb, m, n, k, alpha, Ar, Ai, lda, Br, Bi, ldb, beta, Cr, Ci, ldc, tri, ws, wsn);
   const double sa = (ta == 'C' || ta == 'R') ? -1.0 : 1.0, sb = (tb == 'C' || tb == 'R') ? -1.0 : 1.0;
   const char ra = (ta == 'T' || ta == 'C') ? 'T' : 'N', rb = (tb == 'T' || tb == 'C') ? 'T' : 'N';
   int rc;

@@ -268,11 +268,19 @@ int split_gemm_nt(hipStream_t s, int m, int n, int k, const double *A, int lda, 
 int dgemm(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
           const double *B, int ldb, double beta, double *C, int ldc, int tri = 0, int atri = 0, int btri = 0,
           double *ws = nullptr, size_t wsn = 0);
+// K from which a complex product onto few output tiles is split (slices of
+// half of it); a smaller value in a test build puts small products on that path
+#ifndef W2K_ZSPLIT_KMIN
+#define W2K_ZSPLIT_KMIN 512
+#endif
 // planar complex C = alpha op(A) op(B) + beta C in one kernel (alpha, beta
-// real; op 'N', 'T', 'C' conjugate transpose, 'R' conjugate); tri as dgemm
+// real; op 'N', 'T', 'C' conjugate transpose, 'R' conjugate); tri as dgemm.
+// With a workspace ws (wsn doubles), few output tiles (under 256 of 64 x 64)
+// and K >= W2K_ZSPLIT_KMIN on a full output, K is split over up to 1024 / tiles slices
+// whose partial sums a second kernel adds in a fixed order.
 int zgemm_fused(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *Ar, const double *Ai,
                 int lda, const double *Br, const double *Bi, int ldb, double beta, double *Cr, double *Ci, int ldc,
-                int tri);
+                int tri, double *ws = nullptr, size_t wsn = 0);
 // the same product on a given tile kernel (0: 64x64, 1: 128x128; no split K)
 int dgemm_tiles(hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *A, int lda,
                 const double *B, int ldb, double beta, double *C, int ldc, int tri, int atri, int btri, int variant);
