@@ -89,12 +89,17 @@ __global__ void k_z3m_comb(int m, int n, const double *__restrict__ T, double a,
   }
 }
 
+// K up to which a complex product below the three-product size runs on the
+// fused kernel (triangular operands read as dense): 1024 measured 2 ms
+// faster than 512 on the 100-atom Cholesky, 4096 no faster than 1024
+constexpr int ZFUSED_KMAX = 1024;
+
 // zgemm for the large products: three real products instead of four
 // (Ci = (Ar + Im op A)(Br + Im op B) - Ar Br - Im op A Im op B), which takes a
 // quarter of the arithmetic off at the cost of three elementwise passes and
 // an error of the imaginary part bounded by |A| |B| rather than by the
 // products of the parts.  Smaller products, and calls inside the
-// allocation-free entry point, go to the fused kernel (K <= 512) or zgemm.
+// allocation-free entry point, go to the fused kernel (K <= ZFUSED_KMAX) or zgemm.
 static int zgemm3(Ctx &c, hipStream_t s, char ta, char tb, int m, int n, int k, double alpha, const double *Ar,
                   const double *Ai, int lda, const double *Br, const double *Bi, int ldb, double beta, double *Cr,
                   double *Ci, int ldc, int tri = 0, int atri = 0, int btri = 0, double *ws = nullptr, size_t wsn = 0)
@@ -104,7 +109,7 @@ static int zgemm3(Ctx &c, hipStream_t s, char ta, char tb, int m, int n, int k, 
     // the triangular operands of these products (U, U^-1, V^-1) hold zeros in
     // their other triangle, so the fused kernel may read them as dense: one
     // launch instead of four for the many small products of the recursions
-    if (k <= 512)
+    if (k <= ZFUSED_KMAX)
       return zgemm_fused(s, ta, tb, m, n, k, alpha, Ar, Ai, lda, Br, Bi, ldb, beta, Cr, Ci, ldc, tri);
     return zgemm(s, ta, tb, m, n, k, alpha, Ar, Ai, lda, Br, Bi, ldb, beta, Cr, Ci, ldc, tri, atri, btri, ws, wsn);
   }
