@@ -32,7 +32,7 @@ def _unlimited_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
-@pytest.mark.parametrize("name,nk", [("rs8", 1), ("c1_fcc", 2), ("lo_cplx", 1), ("c2_rs64", 1)])
+@pytest.mark.parametrize("name,nk", [("rs8", 1), ("c1_fcc", 2), ("lo_cplx", 1), ("c2_rs64", 1), ("c3_lowsym100", 1)])
 def test_reference_program_with_dropin(name, nk, tmp_path):
     fx = np.load(os.path.join(GOLDEN, f"{name}.npz"))
     exe = EXE[bool(int(fx["meta/complex"]))]
@@ -53,5 +53,18 @@ def test_reference_program_with_dropin(name, nk, tmp_path):
     for i in range(nk):
         k = read_dump(os.path.join(tmp_path, f"k{i + 1:04d}.bin"))
         ne = int(k["NE"])
-        assert ne == int(fx[f"k{i}/NE"]) and int(k["NBELW"]) == int(fx[f"k{i}/NBELW"]), f"k{i}"
-        assert np.max(np.abs(k["EIGVAL"][:ne] - fx[f"k{i}/EIGVAL"])) < 1e-10, f"k{i}"
+        ref = np.atleast_1d(fx[f"k{i}/EIGVAL"])[: int(fx[f"k{i}/NE"])]
+        # the deck carries the window to one decimal (w2k_inputs.write_case);
+        # the 100-atom fixture's upper bound was narrowed after its run to a
+        # point between two eigenvalues, so the program runs with the deck's
+        # bound and is compared on the fixture's eigenvalues below it
+        eu_fx = float(fx[f"k{i}/EU"])
+        eu = float(f"{eu_fx:.1f}")
+        if eu < eu_fx:
+            ref = ref[ref <= eu]
+        assert int(k["NBELW"]) == int(fx[f"k{i}/NBELW"]), f"k{i}"
+        if eu > eu_fx:  # the program's window holds more than the fixture: compare what it holds
+            assert ne >= len(ref), f"k{i}: NE {ne} < {len(ref)}"
+        else:
+            assert ne == len(ref), f"k{i}: NE {ne} against {len(ref)} at EU {eu}"
+        assert np.max(np.abs(k["EIGVAL"][:len(ref)] - ref)) < 1e-10, f"k{i}"
